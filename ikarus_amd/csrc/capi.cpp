@@ -1,0 +1,523 @@
+// capi.cpp — the extern "C" boundary plus the host-side NewtonRaphson / LoadControl loops.
+//
+// The loops restate NewtonRaphson::solve (ikarus/solver/nonlinearsolver/newtonraphson.hh:196-257) and
+// LoadControl::run (ikarus/controlroutines/loadcontrol.inl:21-57) with the iterate, residual and
+// tangent resident in device memory: per Newton iteration the host reads back two scalars (||R||
+// and the PCG state), nothing else.
+#include <algorithm>
+#include <cmath>
+#include <exception>
+
+#include "ik_internal.hh"
+
+using ik::Error;
+using ik::require;
+
+namespace {
+
+template <class F>
+int guard(ik_handle* h, F&& f) {
+  try {
+    f();
+    if (h) h->err.clear();
+    return IK_OK;
+  } catch (const Error& e) {
+    if (h) h->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    if (h) h->err = e.what();
+    return IK_EINVAL;
+  }
+}
+
+__global__ void negateKernel(double* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) x[i] = -x[i];
+}
+__global__ void gatherKernel(const double* __restrict__ src, const int32_t* __restrict__ idx, int64_t n,
+                             double* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[idx[i]];
+}
+// u_{n+1} = 2 u_n - u_{n-1}; uPrev <- u_n (secant predictor)
+__global__ void extrapolateKernel(double* __restrict__ u, double* __restrict__ uPrev, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double cur = u[i];
+    u[i]             = 2.0 * cur - uPrev[i];
+    uPrev[i]         = cur;
+  }
+}
+
+inline int grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 4096);
+}
+
+void copyIn(ik_handle* h, double* dst, const double* src, int64_t n, int where) {
+  IK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        h->stream));
+}
+void copyOut(ik_handle* h, double* dst, const double* src, int64_t n, int where) {
+  IK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        h->stream));
+  IK_HIP(hipStreamSynchronize(h->stream));
+}
+
+void ensurePattern(ik_handle* h) {
+  if (!h->patternBuilt) ik::buildPattern(h);
+}
+
+void requireBound(ik_handle* h) {
+  require(h->bound, IK_ENOTBOUND, "The assembler is not bound to a requirement (call ik_set_requirement first)");
+}
+
+double normR(ik_handle* h) { return std::sqrt(ik::dot(h, h->R.p, h->R.p, h->ndof)); }
+
+// residual at the bound requirement into h->R (FULL)
+void assembleR(ik_handle* h, int dbc) {
+  ensurePattern(h);
+  ik::assembleKR(h, dbc, IK_FP64, false, true);
+}
+void assembleK(ik_handle* h, int dbc, int precision) {
+  ensurePattern(h);
+  ik::assembleKR(h, dbc, precision, true, false);
+  h->kPrecision = precision;
+  h->kDbc       = dbc;
+  h->kValid     = true;
+}
+
+void linearSolve(ik_handle* h, const double* rhs, double* x, const ik_solver_opts* o, double rtol,
+                 ik_solver_info* info) {
+  require(h->kValid && h->kDbc == IK_DBC_FULL, IK_EINVAL, "ik_solve needs a FULL-DBC matrix assembled first");
+  if (o->kind == IK_PCG_JACOBI_FP64) {
+    require(h->kPrecision == IK_FP64, IK_EINVAL, "FP64 PCG needs an FP64 tangent");
+    int it     = 0;
+    double rel = 0;
+    ik::pcgJacobi64(h, rhs, x, rtol, o->max_iter, &it, &rel);
+    info->iterations   = it;
+    info->refinements  = 0;
+    info->rel_residual = rel;
+    info->converged    = rel <= rtol ? 1 : 0;
+  } else if (o->kind == IK_PCG_BJACOBI_FP32_IR) {
+    require(h->kPrecision == IK_FP32, IK_EINVAL, "mixed PCG needs an FP32 tangent");
+    ik_solver_opts oo = *o;
+    oo.rtol           = rtol;
+    ik::pcgMixed(h, rhs, x, &oo, info);
+  } else {
+    throw Error{IK_ENOTIMPL, "unknown linear solver kind"};
+  }
+}
+
+void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_info* info) {
+  (void)stepSize;  // inhomogeneous Dirichlet forces are not part of the device path
+  requireBound(h);
+  ensurePattern(h);
+  const int64_t n = h->ndof;
+  h->du.ensure(n);
+  *info         = ik_nr_info{};
+  info->success = 1;
+  assembleR(h, IK_DBC_FULL);
+  double rNorm        = normR(h);
+  const double r0     = rNorm;
+  info->residual_norm = rNorm;
+  int iter            = 0;
+  h->kValid           = false;
+  while ((rNorm > s->tol && iter < s->max_iter) || iter < s->min_iter) {
+    assembleK(h, IK_DBC_FULL, s->precision);
+    // K x = R ; correction = -x (solve(correction, -rx), newtonraphson.hh:222-223)
+    double rtol = s->linear.rtol;
+    if (s->forcing > 0.0 && r0 > 0.0) rtol = std::max(s->linear.rtol, std::min(0.5, s->forcing * rNorm / r0));
+    ik_solver_info li{};
+    linearSolve(h, h->R.p, h->du.p, &s->linear, rtol, &li);
+    info->linear_iterations += li.iterations;
+    hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->du.p, n);
+    info->correction_norm = std::sqrt(ik::dot(h, h->du.p, h->du.p, n));
+    if (h->neas) ik::easUpdate(h, h->du.p);  // CORRECTION_UPDATED (newtonraphson.hh:230)
+    ik::axpy(h, 1.0, h->du.p, h->u.p, n);
+    h->kValid = false;
+    assembleR(h, IK_DBC_FULL);
+    rNorm               = normR(h);
+    info->residual_norm = rNorm;
+    ++iter;
+  }
+  if (iter == s->max_iter) info->success = 0;
+  info->iterations = iter;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ik_create(ik_handle** out, const ik_problem_desc* d) {
+  if (!out || !d) return IK_EINVAL;
+  auto* h = new ik_handle;
+  const int rc = guard(h, [&] {
+    require(d->dim == 3, IK_ENOTIMPL, "the device path supports 3D hex8 meshes only");
+    require(d->num_nodes > 0 && d->num_elements > 0 && d->X && d->conn, IK_EINVAL, "empty mesh");
+    require(d->eas_modes == 0 || d->eas_modes == 21, IK_ENOTIMPL, "eas_modes must be 0 or 21");
+    require(d->material == IK_NEOHOOKE || d->material == IK_SVK, IK_EINVAL, "unknown material");
+    require(d->num_nodes * 3 < (int64_t)INT32_MAX, IK_EINVAL, "too many DOFs for int32 indices");
+    IK_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    IK_HIP(hipEventCreate(&h->ev0));
+    IK_HIP(hipEventCreate(&h->ev1));
+    h->nn       = d->num_nodes;
+    h->ne       = d->num_elements;
+    h->ndof     = h->nn * 3;
+    h->material = d->material;
+    h->lam      = d->lambda;
+    h->mu       = d->mu;
+    h->neas     = d->eas_modes;
+    h->hX.assign(d->X, d->X + h->nn * 3);
+    h->hconn.assign(d->conn, d->conn + h->ne * 8);
+    for (int64_t i = 0; i < h->ne * 8; ++i)
+      require(h->hconn[i] >= 0 && h->hconn[i] < h->nn, IK_EINVAL, "connectivity index out of range");
+    if (d->dirichlet)
+      h->hfixed.assign(d->dirichlet, d->dirichlet + h->ndof);
+    else
+      h->hfixed.assign(h->ndof, 0);
+    std::vector<int32_t> fl;
+    for (int64_t i = 0; i < h->ndof; ++i)
+      if (h->hfixed[i]) fl.push_back((int32_t)i);
+    h->nfixed = (int64_t)fl.size();
+    h->X.upload(h->hX.data(), h->hX.size(), h->stream);
+    h->conn.upload(h->hconn.data(), h->hconn.size(), h->stream);
+    h->fixed.upload(h->hfixed.data(), h->hfixed.size(), h->stream);
+    h->fixedList.upload(fl.data(), fl.size(), h->stream);
+    h->nfaces = d->num_faces;
+    if (h->nfaces) {
+      require(d->face_elem && d->face_axis && d->face_side, IK_EINVAL, "face arrays missing");
+      for (int64_t f = 0; f < h->nfaces; ++f)
+        require(d->face_elem[f] >= 0 && d->face_elem[f] < h->ne && d->face_axis[f] >= 0 && d->face_axis[f] < 3 &&
+                    (d->face_side[f] == 0 || d->face_side[f] == 1),
+                IK_EINVAL, "bad traction face");
+      h->faceElem.upload(d->face_elem, h->nfaces, h->stream);
+      h->faceAxis.upload(d->face_axis, h->nfaces, h->stream);
+      h->faceSide.upload(d->face_side, h->nfaces, h->stream);
+    }
+    for (int k = 0; k < 3; ++k) h->trac[k] = d->traction[k];
+    h->npl = d->num_point_loads;
+    if (h->npl) {
+      require(d->point_dof && d->point_coef, IK_EINVAL, "point load arrays missing");
+      for (int64_t i = 0; i < h->npl; ++i)
+        require(d->point_dof[i] >= 0 && d->point_dof[i] < h->ndof, IK_EINVAL, "point load dof out of range");
+      h->hplDof.assign(d->point_dof, d->point_dof + h->npl);
+      h->hplCoef.assign(d->point_coef, d->point_coef + h->npl);
+      h->plDof.upload(d->point_dof, h->npl, h->stream);
+      h->plCoef.upload(d->point_coef, h->npl, h->stream);
+    }
+    h->u.ensure(h->ndof);
+    h->R.ensure(h->ndof);
+    ik::launchZero(h->u.p, h->ndof, h->stream);
+    h->errFlag.ensure(1);
+    const unsigned long long none = ~0ull;
+    IK_HIP(hipMemcpyAsync(h->errFlag.p, &none, sizeof(none), hipMemcpyHostToDevice, h->stream));
+    if (h->neas) {
+      h->alpha.ensure((size_t)h->ne * h->neas);
+      ik::launchZero(h->alpha.p, h->ne * h->neas, h->stream);
+    }
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+  if (rc != IK_OK) {
+    // keep the handle so that ik_last_error works; caller must destroy
+  }
+  *out = h;
+  return rc;
+}
+
+void ik_destroy(ik_handle* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->hsc) (void)hipHostFree(h->hsc);
+  if (h->hred) (void)hipHostFree(h->hred);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  hipStream_t s = h->stream;
+  delete h;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char* ik_last_error(const ik_handle* h) { return h ? h->err.c_str() : "null handle"; }
+int64_t ik_bad_element(const ik_handle* h) { return h ? h->bad_elem : -1; }
+int64_t ik_size(const ik_handle* h) { return h ? h->ndof : -1; }
+int64_t ik_reduced_size(const ik_handle* h) { return h ? h->ndof - h->nfixed : -1; }
+
+int ik_set_requirement(ik_handle* h, const double* u, double lambda, int where) {
+  return guard(h, [&] {
+    if (u) copyIn(h, h->u.p, u, h->ndof, where);
+    h->lambda = lambda;
+    h->bound  = true;
+    h->kValid = false;
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where) {
+  return guard(h, [&] {
+    if (u) copyOut(h, u, h->u.p, h->ndof, where);
+    if (lambda) *lambda = h->lambda;
+  });
+}
+
+int ik_vector(ik_handle* h, int dbc, double* R, int where) {
+  return guard(h, [&] {
+    requireBound(h);
+    require(dbc >= 0 && dbc <= 2, IK_EINVAL, "bad DBC option");
+    require(R != nullptr, IK_EINVAL, "null output");
+    assembleR(h, dbc == IK_DBC_FULL ? IK_DBC_FULL : IK_DBC_RAW);
+    if (dbc != IK_DBC_REDUCED) {
+      copyOut(h, R, h->R.p, h->ndof, where);
+      return;
+    }
+    const int64_t nr = h->ndof - h->nfixed;
+    std::vector<int32_t> freeList;
+    freeList.reserve(nr);
+    for (int64_t i = 0; i < h->ndof; ++i)
+      if (!h->hfixed[i]) freeList.push_back((int32_t)i);
+    ik::DevBuf<int32_t> fl;
+    fl.upload(freeList.data(), freeList.size(), h->stream);
+    h->tmp.ensure(std::max<int64_t>(nr, 1));
+    hipLaunchKernelGGL(gatherKernel, dim3(grid(nr)), dim3(256), 0, h->stream, h->R.p, fl.p, nr, h->tmp.p);
+    copyOut(h, R, h->tmp.p, nr, where);
+  });
+}
+
+int ik_matrix(ik_handle* h, int dbc, int precision) {
+  return guard(h, [&] {
+    requireBound(h);
+    require(dbc >= 0 && dbc <= 2, IK_EINVAL, "bad DBC option");
+    require(precision == IK_FP64 || precision == IK_FP32, IK_EINVAL, "bad precision");
+    assembleK(h, dbc == IK_DBC_FULL ? IK_DBC_FULL : IK_DBC_RAW, precision);
+    h->kDbc = dbc;
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* outer, int32_t* inner, double* values) {
+  return guard(h, [&] {
+    ensurePattern(h);
+    require(dbc >= 0 && dbc <= 2, IK_EINVAL, "bad DBC option");
+    const bool reduced = dbc == IK_DBC_REDUCED;
+    // block transpose map and per-node block counts
+    const int64_t nn = h->nn;
+    std::vector<int64_t> cb(h->ndof + 1, 0);  // constraints below
+    for (int64_t i = 0; i < h->ndof; ++i) cb[i + 1] = cb[i] + (h->hfixed[i] ? 1 : 0);
+    const int64_t N = reduced ? h->ndof - h->nfixed : h->ndof;
+    // count entries per column (column c = 3*i + k of the full matrix)
+    std::vector<int64_t> colStart(N + 1, 0);
+    for (int64_t i = 0; i < nn; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const int64_t c = 3 * i + k;
+        if (reduced && h->hfixed[c]) continue;
+        int64_t cnt = 0;
+        for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1]; ++b)
+          for (int l = 0; l < 3; ++l) {
+            const int64_t r = 3 * (int64_t)h->hcolidx[b] + l;
+            if (reduced && h->hfixed[r]) continue;
+            ++cnt;
+          }
+        colStart[(reduced ? c - cb[c] : c) + 1] = cnt;
+      }
+    for (int64_t c = 0; c < N; ++c) colStart[c + 1] += colStart[c];
+    *n   = N;
+    *nnz = colStart[N];
+    if (!outer && !inner && !values) return;
+    std::vector<double> K;
+    if (values) {
+      require(h->kValid, IK_EINVAL, "no matrix assembled");
+      require((dbc == IK_DBC_FULL) == (h->kDbc == IK_DBC_FULL), IK_EINVAL,
+              "requested DBC layout differs from the assembled matrix");
+      K.resize((size_t)h->nnzb * 9);
+      if (h->kPrecision == IK_FP64) {
+        IK_HIP(hipMemcpyAsync(K.data(), h->K64.p, K.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        IK_HIP(hipStreamSynchronize(h->stream));
+      } else {
+        std::vector<float> Kf(K.size());
+        IK_HIP(hipMemcpyAsync(Kf.data(), h->K32.p, Kf.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        IK_HIP(hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < K.size(); ++i) K[i] = Kf[i];
+      }
+    }
+    if (outer)
+      for (int64_t c = 0; c <= N; ++c) outer[c] = (int32_t)colStart[c];
+    for (int64_t i = 0; i < nn; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const int64_t c = 3 * i + k;
+        if (reduced && h->hfixed[c]) continue;
+        int64_t p = colStart[reduced ? c - cb[c] : c];
+        for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1]; ++b) {
+          const int32_t j = h->hcolidx[b];
+          // entry (row 3j+l, col 3i+k) lives in block (j, i), element (l, k)
+          int64_t bt = -1;
+          if (values) {
+            const auto first = h->hcolidx.begin() + h->hrowptr[j], last = h->hcolidx.begin() + h->hrowptr[j + 1];
+            bt = std::lower_bound(first, last, (int32_t)i) - h->hcolidx.begin();
+          }
+          for (int l = 0; l < 3; ++l) {
+            const int64_t r = 3 * (int64_t)j + l;
+            if (reduced && h->hfixed[r]) continue;
+            if (inner) inner[p] = (int32_t)(reduced ? r - cb[r] : r);
+            if (values) values[p] = K[(size_t)bt * 9 + l * 3 + k];
+            ++p;
+          }
+        }
+      }
+  });
+}
+
+int ik_matrix_bsr(ik_handle* h, int64_t* nbr, int64_t* nnzb, const int32_t** rowptr, const int32_t** colidx,
+                  const void** values, int* precision) {
+  return guard(h, [&] {
+    ensurePattern(h);
+    if (nbr) *nbr = h->nn;
+    if (nnzb) *nnzb = h->nnzb;
+    if (rowptr) *rowptr = h->rowptr.p;
+    if (colidx) *colidx = h->colidx.p;
+    if (values) *values = h->kPrecision == IK_FP32 ? (const void*)h->K32.p : (const void*)h->K64.p;
+    if (precision) *precision = h->kPrecision;
+  });
+}
+
+int ik_element(ik_handle* h, int64_t e, double* Ke, double* Re) {
+  return guard(h, [&] {
+    requireBound(h);
+    ensurePattern(h);
+    ik::elementHost(h, e, Ke, Re);
+  });
+}
+
+int ik_update_state(ik_handle* h, const double* du, int where) {
+  return guard(h, [&] {
+    requireBound(h);
+    if (!h->neas) return;
+    h->du.ensure(h->ndof);
+    copyIn(h, h->du.p, du, h->ndof, where);
+    ik::easUpdate(h, h->du.p);
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int ik_get_alpha(ik_handle* h, double* alpha, int where) {
+  return guard(h, [&] {
+    require(h->neas > 0, IK_EINVAL, "no EAS modes");
+    copyOut(h, alpha, h->alpha.p, h->ne * h->neas, where);
+  });
+}
+
+int ik_set_alpha(ik_handle* h, const double* alpha, int where) {
+  return guard(h, [&] {
+    require(h->neas > 0, IK_EINVAL, "no EAS modes");
+    copyIn(h, h->alpha.p, alpha, h->ne * h->neas, where);
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int ik_solve(ik_handle* h, const double* rhs, double* x, int where, const ik_solver_opts* o, ik_solver_info* info) {
+  return guard(h, [&] {
+    require(rhs && x && o && info, IK_EINVAL, "null argument");
+    const int64_t n = h->ndof;
+    h->rhs.ensure(n);
+    h->px.ensure(n);
+    copyIn(h, h->rhs.p, rhs, n, where);
+    linearSolve(h, h->rhs.p, h->px.p, o, o->rtol, info);
+    copyOut(h, x, h->px.p, n, where);
+  });
+}
+
+int ik_newton_solve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_info* info) {
+  return guard(h, [&] {
+    require(s && info, IK_EINVAL, "null argument");
+    newtonSolve(h, s, stepSize, info);
+  });
+}
+
+int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s, ik_control_info* info,
+                    int32_t* iterations, double* residuals) {
+  return guard(h, [&] {
+    require(c && s && info, IK_EINVAL, "null argument");
+    require(c->load_steps > 0, IK_EINVAL, "Number of load steps should be greater than zero.");
+    requireBound(h);
+    *info           = ik_control_info{};
+    const double dl = (c->tend - c->tbegin) / c->load_steps;
+    ik_nr_info ni{};
+    newtonSolve(h, s, 0.0, &ni);
+    if (iterations) iterations[0] = ni.iterations;
+    if (residuals) residuals[0] = ni.residual_norm;
+    info->solves = 1;
+    info->total_iterations += ni.iterations;
+    info->linear_iterations += ni.linear_iterations;
+    if (!ni.success) return;
+    ik::DevBuf<double> uPrev;
+    if (c->predictor == IK_PRED_LINEAR) {
+      uPrev.ensure(h->ndof);
+      IK_HIP(hipMemcpyAsync(uPrev.p, h->u.p, h->ndof * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else {
+      require(c->predictor == IK_PRED_CONSTANT, IK_ENOTIMPL, "predictor not available in this build");
+    }
+    for (int ls = 0; ls < c->load_steps; ++ls) {
+      if (c->predictor == IK_PRED_LINEAR && ls > 0)
+        hipLaunchKernelGGL(extrapolateKernel, dim3(grid(h->ndof)), dim3(256), 0, h->stream, h->u.p, uPrev.p, h->ndof);
+      else if (c->predictor == IK_PRED_LINEAR)
+        IK_HIP(hipMemcpyAsync(uPrev.p, h->u.p, h->ndof * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      h->lambda += dl;  // predictor: lambda += dlambda (loadcontrol.inl:55-57)
+      newtonSolve(h, s, dl, &ni);
+      if (iterations) iterations[ls + 1] = ni.iterations;
+      if (residuals) residuals[ls + 1] = ni.residual_norm;
+      info->solves += 1;
+      info->total_iterations += ni.iterations;
+      info->linear_iterations += ni.linear_iterations;
+      if (!ni.success) return;
+    }
+    info->success = 1;
+  });
+}
+
+int64_t ik_mlp_num_params(void) { return 14 * 128 + 128 + 2 * (128 * 128 + 128) + 3 * 128 + 3; }
+
+int ik_mlp_set_params(ik_handle* h, const float* params, int where) {
+  return guard(h, [&] {
+    const int64_t np = ik_mlp_num_params();
+    h->mlpParams.ensure(np);
+    IK_HIP(hipMemcpyAsync(h->mlpParams.p, params, np * sizeof(float),
+                          where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream));
+    IK_HIP(hipStreamSynchronize(h->stream));
+    h->mlpLoaded = true;
+  });
+}
+
+int ik_mlp_get_params(ik_handle* h, float* params, int where) {
+  return guard(h, [&] {
+    require(h->mlpLoaded, IK_EINVAL, "no MLP parameters set");
+    IK_HIP(hipMemcpyAsync(params, h->mlpParams.p, ik_mlp_num_params() * sizeof(float),
+                          where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int ik_mlp_set_load(ik_handle* h, const double traction[3], double charLen) {
+  return guard(h, [&] {
+    for (int k = 0; k < 3; ++k) h->mlpTraction[k] = traction[k];
+    h->mlpCharLen = charLen;
+  });
+}
+
+int ik_timing_enable(ik_handle* h, int on) {
+  return guard(h, [&] { h->timing = on != 0; });
+}
+int ik_timing_read(ik_handle* h, int kernel, int64_t* launches, double* totalMs) {
+  return guard(h, [&] {
+    require(kernel >= 0 && kernel < ik::KCOUNT, IK_EINVAL, "bad kernel id");
+    if (launches) *launches = h->tLaunches[kernel];
+    if (totalMs) *totalMs = h->tMs[kernel];
+  });
+}
+int ik_timing_reset(ik_handle* h) {
+  return guard(h, [&] {
+    for (int i = 0; i < ik::KCOUNT; ++i) {
+      h->tLaunches[i] = 0;
+      h->tMs[i]       = 0;
+    }
+  });
+}
+int ik_synchronize(ik_handle* h) {
+  return guard(h, [&] { IK_HIP(hipStreamSynchronize(h->stream)); });
+}
+
+}  // extern "C"

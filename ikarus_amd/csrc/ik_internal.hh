@@ -1,0 +1,185 @@
+// ik_internal.hh — handle state and shared helpers of the MI355X hot-path library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ikarus_amd.h"
+
+namespace ik {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define IK_HIP(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) throw ::ik::Error{IK_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+inline void require(bool ok, int code, const std::string& msg) {
+  if (!ok) throw Error{code, msg};
+}
+
+// Owning device buffer. Grows only; contents are not preserved on growth.
+template <class T>
+struct DevBuf {
+  T* p     = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    IK_HIP(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    ensure(count);
+    if (count) IK_HIP(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+// Device scalars of one PCG run (Eigen ConjugateGradient state, linearsolver.cpp:23-24)
+struct PcgScalars {
+  double rz;        // r . z of the current iterate
+  double rz_new;
+  double pq;        // p . (K p)
+  double rr;        // r . r
+  double alpha, beta;
+  double threshold; // tol^2 ||b||^2
+  int32_t done;
+  int32_t iters;
+  double bb;        // ||b||^2
+};
+
+enum KernelId { KASM_KR = 0, KASM_R = 1, KSPMV = 2, KMLP = 3, KCOUNT = 4 };
+
+// Reduction partials: one double per block, final fold in fixed order (deterministic).
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace ik
+
+struct ik_handle {
+  std::string err;
+  int64_t bad_elem = -1;
+  hipStream_t stream = nullptr;
+
+  // problem
+  int64_t nn = 0, ne = 0, ndof = 0;
+  int material = IK_NEOHOOKE;
+  double lam = 0, mu = 0;
+  int neas = 0;
+  std::vector<double> hX;
+  std::vector<int32_t> hconn;
+  std::vector<uint8_t> hfixed;
+  int64_t nfixed = 0;
+  ik::DevBuf<double> X;
+  ik::DevBuf<int32_t> conn;
+  ik::DevBuf<uint8_t> fixed;
+  ik::DevBuf<int32_t> fixedList;  // constrained DOF indices
+  // loads
+  int64_t nfaces = 0;
+  ik::DevBuf<int32_t> faceElem;
+  ik::DevBuf<int8_t> faceAxis, faceSide;
+  double trac[3] = {0, 0, 0};
+  int64_t npl = 0;
+  ik::DevBuf<int64_t> plDof;
+  ik::DevBuf<double> plCoef;
+  std::vector<int64_t> hplDof;
+  std::vector<double> hplCoef;
+
+  // 3x3 block CSR pattern (block row = node) and element block-slot maps
+  bool patternBuilt = false;
+  std::vector<int32_t> hrowptr, hcolidx, hdiag;
+  int64_t nnzb = 0;
+  ik::DevBuf<int32_t> rowptr, colidx, diagBlk, bslot;
+  // element coloring-free scatter uses atomics; K storage
+  ik::DevBuf<double> K64;
+  ik::DevBuf<float> K32;
+  int kPrecision = -1, kDbc = -1;
+  bool kValid = false;
+
+  // requirement and vectors
+  bool bound = false;
+  double lambda = 0;
+  ik::DevBuf<double> u, R, du, rhs, tmp;
+
+  // EAS
+  ik::DevBuf<double> alpha;
+
+  // error flag (lowest element index with det C <= 0)
+  ik::DevBuf<unsigned long long> errFlag;
+
+  // solver workspace
+  ik::DevBuf<double> pr, pz, pp, pq, px, pdiag, pb, partial;
+  ik::DevBuf<float> fr, fp, fq, fx, fdiag;  // FP32 PCG vectors; fdiag holds 3x3 block inverses
+  ik::DevBuf<ik::PcgScalars> sc;
+  ik::PcgScalars* hsc = nullptr;  // pinned host mirror
+  double* hred = nullptr;         // pinned host scalar for reductions
+
+  // learned predictor
+  ik::DevBuf<float> mlpParams;
+  bool mlpLoaded = false;
+  double mlpTraction[3] = {0, 0, 0};
+  double mlpCharLen = 1.0;
+
+  // timing
+  bool timing = false;
+  int64_t tLaunches[ik::KCOUNT] = {0, 0, 0, 0};
+  double tMs[ik::KCOUNT] = {0, 0, 0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace ik {
+// pattern.cpp
+void buildPattern(ik_handle* h);
+// assemble.hip
+void launchZero(double* p, int64_t n, hipStream_t s);
+void launchZeroF(float* p, int64_t n, hipStream_t s);
+void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR);
+void launchDbcDiag(ik_handle* h, int precision);
+void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re);
+void easUpdate(ik_handle* h, const double* duDev);
+// sparse.hip
+void spmv64(ik_handle* h, const double* x, double* y);
+void spmv32(ik_handle* h, const float* x, float* y);
+double dot(ik_handle* h, const double* a, const double* b, int64_t n);
+void axpy(ik_handle* h, double a, const double* x, double* y, int64_t n);  // y += a x
+void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxIter, int* iters, double* relres);
+void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info);
+void residualBlock64(ik_handle* h, const double* b, const double* x, double* r);  // r = b - K x (FP64 K)
+// mlp.hip
+void mlpPredict(ik_handle* h, const double* uPrev, const double* uPrev2, double lamNext, double dlam,
+                double* uOut);
+
+// timing helpers (events on the handle's stream around one launch)
+struct ScopedKernelTimer {
+  ik_handle* h;
+  int id;
+  ScopedKernelTimer(ik_handle* hh, int kid) : h(hh), id(kid) {
+    if (h->timing) IK_HIP(hipEventRecord(h->ev0, h->stream));
+  }
+  ~ScopedKernelTimer() {
+    if (!h->timing) return;
+    (void)hipEventRecord(h->ev1, h->stream);
+    (void)hipEventSynchronize(h->ev1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    h->tMs[id] += ms;
+    h->tLaunches[id] += 1;
+  }
+};
+}  // namespace ik

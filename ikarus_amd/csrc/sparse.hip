@@ -1,0 +1,541 @@
+// sparse.hip — 3x3-block CSR SpMV, reductions and the preconditioned CG solvers.
+//
+// FP64 Jacobi-PCG follows Eigen's ConjugateGradient with DiagonalPreconditioner, the reference's
+// si_ConjugateGradient (ikarus/solver/linearsolver/linearsolver.cpp:23-24): x0 = 0,
+// stop when ||r||^2 < tol^2 ||b||^2, iteration count as Eigen reports it.
+// The mixed form runs the same recurrence in FP32 on an FP32 tangent with a 3x3 block-Jacobi
+// preconditioner, wrapped in FP64 iterative refinement (r = b - K x accumulated in FP64).
+//
+// All PCG scalars live on the device; each kernel reads a `done` flag so the host can enqueue
+// iterations in chunks and poll the flag asynchronously: the iterate after convergence is exactly the
+// one Eigen's loop returns.
+#include "ik_internal.hh"
+
+namespace ik {
+namespace {
+
+constexpr int SPMV_BS   = 256;            // threads per block
+constexpr int GROUP     = 8;              // lanes per block row
+constexpr int ROWS_PER_BLOCK = SPMV_BS / GROUP;
+
+__device__ inline double waveSum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block-wide sum; result valid in thread 0
+template <int BS>
+__device__ inline double blockSum(double v) {
+  __shared__ double red[BS / 64];
+  v = waveSum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+// y = K x over block rows; optional fused partial dot d = sum_i x_i y_i per block.
+// Group of 8 lanes walks the contiguous value run of one block row with stride 8 (coalesced).
+template <typename TV, typename TX, bool DOT>
+__global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ colidx,
+                                                      const TV* __restrict__ val, const TX* __restrict__ x,
+                                                      TX* __restrict__ y, int64_t nrows,
+                                                      double* __restrict__ partial,
+                                                      const PcgScalars* __restrict__ sc) {
+  if (sc && sc->done) return;
+  const int g        = threadIdx.x & (GROUP - 1);
+  const int64_t row  = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / GROUP);
+  TX a0 = 0, a1 = 0, a2 = 0;
+  if (row < nrows) {
+    const int64_t rb = rowptr[row], re = rowptr[row + 1];
+    const int64_t L  = (re - rb) * 9;
+    const TV* V      = val + rb * 9;
+    for (int64_t idx = g; idx < L; idx += GROUP) {
+      const int64_t blk = idx / 9;
+      const int ent     = (int)(idx - blk * 9);
+      const int k = ent / 3, l = ent - 3 * (ent / 3);
+      const TX prod     = (TX)V[idx] * x[(int64_t)colidx[rb + blk] * 3 + l];
+      a0 += (k == 0) ? prod : TX(0);
+      a1 += (k == 1) ? prod : TX(0);
+      a2 += (k == 2) ? prod : TX(0);
+    }
+  }
+#pragma unroll
+  for (int o = GROUP / 2; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, GROUP);
+    a1 += __shfl_xor(a1, o, GROUP);
+    a2 += __shfl_xor(a2, o, GROUP);
+  }
+  double d = 0.0;
+  if (row < nrows && g == 0) {
+    y[row * 3 + 0] = a0;
+    y[row * 3 + 1] = a1;
+    y[row * 3 + 2] = a2;
+    if (DOT) d = (double)x[row * 3] * a0 + (double)x[row * 3 + 1] * a1 + (double)x[row * 3 + 2] * a2;
+  }
+  if (DOT) {
+    const double s = blockSum<SPMV_BS>(d);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+// r = b - K x with FP64 accumulation (K may be FP32); used by iterative refinement
+template <typename TV>
+__global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restrict__ rowptr,
+                                                          const int32_t* __restrict__ colidx,
+                                                          const TV* __restrict__ val,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ b, double* __restrict__ r,
+                                                          int64_t nrows, double* __restrict__ partial) {
+  const int g       = threadIdx.x & (GROUP - 1);
+  const int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / GROUP);
+  double a0 = 0, a1 = 0, a2 = 0;
+  if (row < nrows) {
+    const int64_t rb = rowptr[row], re = rowptr[row + 1];
+    const int64_t L  = (re - rb) * 9;
+    const TV* V      = val + rb * 9;
+    for (int64_t idx = g; idx < L; idx += GROUP) {
+      const int64_t blk = idx / 9;
+      const int ent     = (int)(idx - blk * 9);
+      const int k = ent / 3, l = ent - 3 * (ent / 3);
+      const double prod = (double)V[idx] * x[(int64_t)colidx[rb + blk] * 3 + l];
+      a0 += (k == 0) ? prod : 0.0;
+      a1 += (k == 1) ? prod : 0.0;
+      a2 += (k == 2) ? prod : 0.0;
+    }
+  }
+#pragma unroll
+  for (int o = GROUP / 2; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, GROUP);
+    a1 += __shfl_xor(a1, o, GROUP);
+    a2 += __shfl_xor(a2, o, GROUP);
+  }
+  double d = 0.0;
+  if (row < nrows && g == 0) {
+    const double r0 = b[row * 3] - a0, r1 = b[row * 3 + 1] - a1, r2 = b[row * 3 + 2] - a2;
+    r[row * 3]     = r0;
+    r[row * 3 + 1] = r1;
+    r[row * 3 + 2] = r2;
+    d              = r0 * r0 + r1 * r1 + r2 * r2;
+  }
+  const double s = blockSum<SPMV_BS>(d);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// partial dot products (grid-stride, fixed grid) — deterministic given the launch shape
+__global__ __launch_bounds__(256) void dotKernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                 int64_t n, double* __restrict__ partial) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += a[i] * b[i];
+  s = blockSum<256>(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void foldKernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) s += partial[i];
+  s = blockSum<1024>(s);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void axpyKernel(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] += a * x[i];
+}
+
+// ---------------- PCG (Jacobi, FP64) ----------------
+__global__ void jacobiDiagKernel(const int32_t* __restrict__ diag, const double* __restrict__ K, int64_t nrows,
+                                 double* __restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrows * 3) return;
+  const int64_t node = i / 3;
+  const int k        = (int)(i - node * 3);
+  const double d     = K[(int64_t)diag[node] * 9 + k * 4];
+  dinv[i]            = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// r = b, x = 0, p = Dinv r; partials of rr and rz
+__global__ __launch_bounds__(256) void pcgInitKernel(const double* __restrict__ b, const double* __restrict__ dinv,
+                                                     double* __restrict__ x, double* __restrict__ r,
+                                                     double* __restrict__ p, int64_t n,
+                                                     double* __restrict__ partial /* 2*grid */) {
+  double rr = 0.0, rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double bi = b[i];
+    x[i]            = 0.0;
+    r[i]            = bi;
+    const double z  = dinv[i] * bi;
+    p[i]            = z;
+    rr += bi * bi;
+    rz += bi * z;
+  }
+  rr = blockSum<256>(rr);
+  rz = blockSum<256>(rz);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x]             = rr;
+    partial[gridDim.x + blockIdx.x] = rz;
+  }
+}
+
+__global__ __launch_bounds__(1024) void pcgInitFold(const double* __restrict__ partial, int nb, double tol2,
+                                                    PcgScalars* __restrict__ sc) {
+  double rr = 0.0, rz = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 1024) {
+    rr += partial[i];
+    rz += partial[nb + i];
+  }
+  rr = blockSum<1024>(rr);
+  rz = blockSum<1024>(rz);
+  if (threadIdx.x == 0) {
+    sc->bb        = rr;
+    sc->rr        = rr;
+    sc->rz        = rz;
+    const double thr = tol2 * rr;
+    sc->threshold = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+    sc->iters     = 0;
+    sc->done      = (rr == 0.0 || rr < sc->threshold) ? 1 : 0;
+    sc->alpha = sc->beta = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(1024) void pcgAlphaFold(const double* __restrict__ partial, int nb,
+                                                     PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 1024) s += partial[i];
+  s = blockSum<1024>(s);
+  if (threadIdx.x == 0) {
+    sc->pq    = s;
+    sc->alpha = sc->rz / s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void pcgUpdateKernel(T* __restrict__ x, T* __restrict__ r,
+                                                       const T* __restrict__ p, const T* __restrict__ q,
+                                                       const T* __restrict__ pre, int blockPre, int64_t n,
+                                                       const PcgScalars* __restrict__ sc,
+                                                       double* __restrict__ partial /* 2*grid */) {
+  if (sc->done) return;
+  const T alpha = (T)sc->alpha;
+  double rr = 0.0, rz = 0.0;
+  if (!blockPre) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+      x[i] += alpha * p[i];
+      const T ri = r[i] - alpha * q[i];
+      r[i]       = ri;
+      rr += (double)ri * (double)ri;
+      rz += (double)ri * (double)(pre[i] * ri);
+    }
+  } else {
+    const int64_t nn = n / 3;
+    for (int64_t nd = (int64_t)blockIdx.x * 256 + threadIdx.x; nd < nn; nd += (int64_t)gridDim.x * 256) {
+      T rl[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int64_t i = nd * 3 + k;
+        x[i] += alpha * p[i];
+        rl[k] = r[i] - alpha * q[i];
+        r[i]  = rl[k];
+        rr += (double)rl[k] * (double)rl[k];
+      }
+      const T* B = pre + nd * 9;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) rz += (double)rl[k] * (double)(B[k * 3] * rl[0] + B[k * 3 + 1] * rl[1] + B[k * 3 + 2] * rl[2]);
+    }
+  }
+  rr = blockSum<256>(rr);
+  rz = blockSum<256>(rz);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x]             = rr;
+    partial[gridDim.x + blockIdx.x] = rz;
+  }
+}
+
+__global__ __launch_bounds__(1024) void pcgBetaFold(const double* __restrict__ partial, int nb,
+                                                    PcgScalars* __restrict__ sc, int maxIter) {
+  if (sc->done) return;
+  double rr = 0.0, rz = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 1024) {
+    rr += partial[i];
+    rz += partial[nb + i];
+  }
+  rr = blockSum<1024>(rr);
+  rz = blockSum<1024>(rz);
+  if (threadIdx.x == 0) {
+    sc->rr = rr;
+    if (rr < sc->threshold) {
+      sc->done = 1;
+    } else {
+      sc->beta   = rz / sc->rz;
+      sc->rz     = rz;
+      sc->rz_new = rz;
+      sc->iters += 1;
+      if (sc->iters >= maxIter) sc->done = 2;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void pcgPKernel(T* __restrict__ p, const T* __restrict__ r,
+                                                  const T* __restrict__ pre, int blockPre, int64_t n,
+                                                  const PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const T beta = (T)sc->beta;
+  if (!blockPre) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+      p[i] = pre[i] * r[i] + beta * p[i];
+  } else {
+    const int64_t nn = n / 3;
+    for (int64_t nd = (int64_t)blockIdx.x * 256 + threadIdx.x; nd < nn; nd += (int64_t)gridDim.x * 256) {
+      const T r0 = r[nd * 3], r1 = r[nd * 3 + 1], r2 = r[nd * 3 + 2];
+      const T* B = pre + nd * 9;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) p[nd * 3 + k] = B[k * 3] * r0 + B[k * 3 + 1] * r1 + B[k * 3 + 2] * r2 + beta * p[nd * 3 + k];
+    }
+  }
+}
+
+// ---------------- FP32 block-Jacobi helpers ----------------
+__global__ void blockInvKernel(const int32_t* __restrict__ diag, const float* __restrict__ K, int64_t nrows,
+                               float* __restrict__ binv) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nrows) return;
+  const float* A = K + (int64_t)diag[i] * 9;
+  // invert in double for robustness, store FP32
+  const double a00 = A[0], a01 = A[1], a02 = A[2], a10 = A[3], a11 = A[4], a12 = A[5], a20 = A[6], a21 = A[7],
+               a22 = A[8];
+  const double det = a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20) + a02 * (a10 * a21 - a11 * a20);
+  float* B = binv + i * 9;
+  if (det == 0.0) {
+    for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  const double id = 1.0 / det;
+  B[0] = (float)((a11 * a22 - a12 * a21) * id);
+  B[1] = (float)((a02 * a21 - a01 * a22) * id);
+  B[2] = (float)((a01 * a12 - a02 * a11) * id);
+  B[3] = (float)((a12 * a20 - a10 * a22) * id);
+  B[4] = (float)((a00 * a22 - a02 * a20) * id);
+  B[5] = (float)((a02 * a10 - a00 * a12) * id);
+  B[6] = (float)((a10 * a21 - a11 * a20) * id);
+  B[7] = (float)((a01 * a20 - a00 * a21) * id);
+  B[8] = (float)((a00 * a11 - a01 * a10) * id);
+}
+
+// FP32 PCG init from an FP64 right-hand side scaled by 1/s (s = ||r||, keeps FP32 in range)
+__global__ __launch_bounds__(256) void pcgInitF32(const double* __restrict__ b, double invScale,
+                                                  const float* __restrict__ binv, float* __restrict__ x,
+                                                  float* __restrict__ r, float* __restrict__ p, int64_t nn,
+                                                  double* __restrict__ partial) {
+  double rr = 0.0, rz = 0.0;
+  for (int64_t nd = (int64_t)blockIdx.x * 256 + threadIdx.x; nd < nn; nd += (int64_t)gridDim.x * 256) {
+    float rl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      rl[k]         = (float)(b[nd * 3 + k] * invScale);
+      r[nd * 3 + k] = rl[k];
+      x[nd * 3 + k] = 0.0f;
+      rr += (double)rl[k] * (double)rl[k];
+    }
+    const float* B = binv + nd * 9;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float z = B[k * 3] * rl[0] + B[k * 3 + 1] * rl[1] + B[k * 3 + 2] * rl[2];
+      p[nd * 3 + k] = z;
+      rz += (double)rl[k] * (double)z;
+    }
+  }
+  rr = blockSum<256>(rr);
+  rz = blockSum<256>(rz);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x]             = rr;
+    partial[gridDim.x + blockIdx.x] = rz;
+  }
+}
+
+// x64 += scale * x32
+__global__ void accumulateF32(double* __restrict__ x, const float* __restrict__ d, double scale, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    x[i] += scale * (double)d[i];
+}
+
+inline int vecGrid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > kReduceBlocks) g = kReduceBlocks;
+  return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+static void ensureWork(ik_handle* h) {
+  const int64_t n = h->ndof;
+  const int64_t spmvBlocks = (h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+  h->partial.ensure((size_t)std::max<int64_t>(spmvBlocks, 2 * kReduceBlocks) + 8);
+  if (!h->sc.p) {
+    h->sc.ensure(1);
+    IK_HIP(hipHostMalloc((void**)&h->hsc, sizeof(PcgScalars), hipHostMallocDefault));
+    IK_HIP(hipHostMalloc((void**)&h->hred, sizeof(double) * 4, hipHostMallocDefault));
+  }
+  (void)n;
+}
+
+void spmv64(ik_handle* h, const double* x, double* y) {
+  ensureWork(h);
+  const unsigned grid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  ScopedKernelTimer t(h, KSPMV);
+  hipLaunchKernelGGL((spmvKernel<double, double, false>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
+                     h->colidx.p, h->K64.p, x, y, h->nn, nullptr, nullptr);
+}
+
+void spmv32(ik_handle* h, const float* x, float* y) {
+  ensureWork(h);
+  const unsigned grid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  ScopedKernelTimer t(h, KSPMV);
+  hipLaunchKernelGGL((spmvKernel<float, float, false>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
+                     h->colidx.p, h->K32.p, x, y, h->nn, nullptr, nullptr);
+}
+
+double dot(ik_handle* h, const double* a, const double* b, int64_t n) {
+  ensureWork(h);
+  const int g = vecGrid(n);
+  hipLaunchKernelGGL(dotKernel, dim3(g), dim3(256), 0, h->stream, a, b, n, h->partial.p);
+  hipLaunchKernelGGL(foldKernel, dim3(1), dim3(1024), 0, h->stream, h->partial.p, g, h->partial.p + h->partial.n - 1);
+  IK_HIP(hipMemcpyAsync(h->hred, h->partial.p + h->partial.n - 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  IK_HIP(hipStreamSynchronize(h->stream));
+  return h->hred[0];
+}
+
+void axpy(ik_handle* h, double a, const double* x, double* y, int64_t n) {
+  hipLaunchKernelGGL(axpyKernel, dim3(vecGrid(n)), dim3(256), 0, h->stream, a, x, y, n);
+}
+
+void residualBlock64(ik_handle* h, const double* b, const double* x, double* r) {
+  ensureWork(h);
+  const unsigned grid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  hipLaunchKernelGGL((residualKernel<double>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p, h->colidx.p,
+                     h->K64.p, x, b, r, h->nn, h->partial.p);
+}
+
+// Poll interval: iterations enqueued between host checks of the done flag.
+static constexpr int kChunk = 16;
+
+void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxIter, int* iters, double* relres) {
+  ensureWork(h);
+  const int64_t n = h->ndof;
+  h->pr.ensure(n);
+  h->pp.ensure(n);
+  h->pq.ensure(n);
+  h->pdiag.ensure(n);
+  const int g = vecGrid(n);
+  const unsigned sgrid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  hipLaunchKernelGGL(jacobiDiagKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->diagBlk.p,
+                     h->K64.p, h->nn, h->pdiag.p);
+  hipLaunchKernelGGL(pcgInitKernel, dim3(g), dim3(256), 0, h->stream, b, h->pdiag.p, x, h->pr.p, h->pp.p, n,
+                     h->partial.p);
+  hipLaunchKernelGGL(pcgInitFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, g, rtol * rtol, h->sc.p);
+  if (maxIter <= 0) maxIter = (int)(2 * n);
+  int enq = 0;
+  for (;;) {
+    for (int c = 0; c < kChunk && enq < maxIter; ++c, ++enq) {
+      {
+        ScopedKernelTimer t(h, KSPMV);
+        hipLaunchKernelGGL((spmvKernel<double, double, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
+                           h->colidx.p, h->K64.p, h->pp.p, h->pq.p, h->nn, h->partial.p, h->sc.p);
+      }
+      hipLaunchKernelGGL(pcgAlphaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, (int)sgrid, h->sc.p);
+      hipLaunchKernelGGL((pcgUpdateKernel<double>), dim3(g), dim3(256), 0, h->stream, x, h->pr.p, h->pp.p, h->pq.p,
+                         h->pdiag.p, 0, n, h->sc.p, h->partial.p);
+      hipLaunchKernelGGL(pcgBetaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, g, h->sc.p, maxIter);
+      hipLaunchKernelGGL((pcgPKernel<double>), dim3(g), dim3(256), 0, h->stream, h->pp.p, h->pr.p, h->pdiag.p, 0, n,
+                         h->sc.p);
+    }
+    IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
+    IK_HIP(hipStreamSynchronize(h->stream));
+    if (h->hsc->done || enq >= maxIter) break;
+  }
+  *iters  = h->hsc->iters;
+  *relres = h->hsc->bb > 0 ? std::sqrt(h->hsc->rr / h->hsc->bb) : 0.0;
+}
+
+void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info) {
+  ensureWork(h);
+  const int64_t n = h->ndof;
+  h->fr.ensure(n);
+  h->fp.ensure(n);
+  h->fq.ensure(n);
+  h->fx.ensure(n);
+  h->fdiag.ensure((size_t)h->nn * 9);
+  h->pr.ensure(n);
+  const int g = vecGrid(n);
+  const int gn = vecGrid(h->nn);
+  const unsigned sgrid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  hipLaunchKernelGGL(blockInvKernel, dim3((unsigned)((h->nn + 255) / 256)), dim3(256), 0, h->stream, h->diagBlk.p,
+                     h->K32.p, h->nn, h->fdiag.p);
+  // x = 0, r = b
+  launchZero(x, n, h->stream);
+  IK_HIP(hipMemcpyAsync(h->pr.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  const double bnorm = std::sqrt(dot(h, b, b, n));
+  info->iterations  = 0;
+  info->refinements = 0;
+  info->converged   = 0;
+  if (bnorm == 0.0) {
+    info->rel_residual = 0.0;
+    info->converged    = 1;
+    return;
+  }
+  const int maxIter = o->max_iter > 0 ? o->max_iter : (int)(2 * n);
+  double rnorm      = bnorm;
+  const int maxRef  = o->max_refine > 0 ? o->max_refine : 10;
+  for (int sweep = 0; sweep < maxRef; ++sweep) {
+    // inner FP32 PCG on K32 d = r / ||r||
+    hipLaunchKernelGGL(pcgInitF32, dim3(gn), dim3(256), 0, h->stream, h->pr.p, 1.0 / rnorm, h->fdiag.p, h->fx.p,
+                       h->fr.p, h->fp.p, h->nn, h->partial.p);
+    hipLaunchKernelGGL(pcgInitFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, gn,
+                       o->inner_rtol * o->inner_rtol, h->sc.p);
+    int enq = 0;
+    for (;;) {
+      for (int c = 0; c < kChunk && enq < maxIter; ++c, ++enq) {
+        {
+          ScopedKernelTimer t(h, KSPMV);
+          hipLaunchKernelGGL((spmvKernel<float, float, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream,
+                             h->rowptr.p, h->colidx.p, h->K32.p, h->fp.p, h->fq.p, h->nn, h->partial.p, h->sc.p);
+        }
+        hipLaunchKernelGGL(pcgAlphaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, (int)sgrid, h->sc.p);
+        hipLaunchKernelGGL((pcgUpdateKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fx.p, h->fr.p, h->fp.p,
+                           h->fq.p, h->fdiag.p, 1, n, h->sc.p, h->partial.p);
+        hipLaunchKernelGGL(pcgBetaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, gn, h->sc.p, maxIter);
+        hipLaunchKernelGGL((pcgPKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fp.p, h->fr.p, h->fdiag.p, 1,
+                           n, h->sc.p);
+      }
+      IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
+      IK_HIP(hipStreamSynchronize(h->stream));
+      if (h->hsc->done || enq >= maxIter) break;
+    }
+    info->iterations += h->hsc->iters + 1;
+    info->refinements += 1;
+    // x += ||r|| d ; r = b - K x in FP64
+    hipLaunchKernelGGL(accumulateF32, dim3(g), dim3(256), 0, h->stream, x, h->fx.p, rnorm, n);
+    hipLaunchKernelGGL((residualKernel<float>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p, h->colidx.p,
+                       h->K32.p, x, b, h->pr.p, h->nn, h->partial.p);
+    hipLaunchKernelGGL(foldKernel, dim3(1), dim3(1024), 0, h->stream, h->partial.p, (int)sgrid,
+                       h->partial.p + h->partial.n - 1);
+    IK_HIP(hipMemcpyAsync(h->hred, h->partial.p + h->partial.n - 1, sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    IK_HIP(hipStreamSynchronize(h->stream));
+    rnorm              = std::sqrt(h->hred[0]);
+    info->rel_residual = rnorm / bnorm;
+    if (info->rel_residual <= o->rtol) {
+      info->converged = 1;
+      break;
+    }
+  }
+}
+
+}  // namespace ik

@@ -1,0 +1,195 @@
+/* ikarus_amd.h — C ABI of the MI355X nonlinear-mechanics hot path.
+ *
+ * Plain C: pointers, sizes, enums. No C++, no torch types. One handle = one mesh with its elements,
+ * Dirichlet flags, loads and device state; one HIP stream per handle; a handle is not thread-safe.
+ * The library owns all device memory it allocates. Every function returns IK_OK (0) or a negative
+ * IK_E* code; ik_last_error() gives the message.
+ *
+ * Each entry point names the Ikarus v0.4.9 interface it replaces (paths relative to the Ikarus tree).
+ */
+#ifndef IKARUS_AMD_H
+#define IKARUS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ik_handle ik_handle;
+
+/* ---- error codes ----------------------------------------------------------------------------- */
+enum {
+  IK_OK         = 0,
+  IK_EINVAL     = -1, /* bad argument / shape (InvalidStateException, interface.hh:187-193)      */
+  IK_EHIP       = -2, /* HIP runtime failure                                                     */
+  IK_ENEGDET    = -3, /* det C <= 0 in an element; replaces abort() in materialhelpers.hh:120-126 */
+  IK_ENOTIMPL   = -4, /* unsupported affordance / option (DUNE_THROW(NotImplemented))           */
+  IK_ENOTBOUND  = -5, /* requirement not set before matrix()/vector() (interface.hh:187-193)     */
+  IK_ENOCONV    = -6  /* linear solver did not reach its tolerance                               */
+};
+
+/* DBCOption (ikarus/assembler/dirichletbcenforcement.hh:8) */
+enum { IK_DBC_RAW = 0, IK_DBC_FULL = 1, IK_DBC_REDUCED = 2 };
+/* Materials (neohooke.hh:38, svk.hh:37) */
+enum { IK_NEOHOOKE = 0, IK_SVK = 1 };
+/* storage / arithmetic precision of the tangent */
+enum { IK_FP64 = 0, IK_FP32 = 1 };
+/* where a pointer lives */
+enum { IK_HOST = 0, IK_DEVICE = 1 };
+/* linear solvers (si_ConjugateGradient, linearsolver.cpp:23-24, and the mixed-precision form) */
+enum { IK_PCG_JACOBI_FP64 = 0, IK_PCG_BJACOBI_FP32_IR = 1 };
+/* load-step predictors: constant (loadcontrol.inl:55-57), secant/linear extrapolation
+ * (pathfollowingfunctions.hh:188-195 semantics), learned per-node MLP (no reference counterpart) */
+enum { IK_PRED_CONSTANT = 0, IK_PRED_LINEAR = 1, IK_PRED_MLP = 2 };
+
+/* ---- problem description (replaces makeBasis + makeFE(skills(nonLinearElastic, eas, traction))
+ *      + DirichletValues + makeSparseFlatAssembler: tests/src/testcantileverbeam.hh:102-151) ---- */
+typedef struct {
+  int32_t dim;                 /* 3: trilinear hex8 with 2x2x2 Gauss (nonlinearelastic.hh:112-125)   */
+  int64_t num_nodes;
+  int64_t num_elements;
+  const double* X;             /* [num_nodes*3] reference coordinates, host                          */
+  const int32_t* conn;         /* [num_elements*8] Dune reference-cube corner order, host            */
+  int32_t material;            /* IK_NEOHOOKE | IK_SVK                                                */
+  double lambda, mu;           /* Lame parameters (physicshelper.hh:276-281)                          */
+  int32_t eas_modes;           /* 0 (NonLinearElastic) or 21 (EAS GL E21, linearandglstrains.hh:280)  */
+  const uint8_t* dirichlet;    /* [num_nodes*3] fixed-DOF flags (dirichletvalues.hh:86-131), host     */
+  /* Neumann traction t = lambda * traction on element faces (traction.hh:106-138) */
+  int64_t num_faces;
+  const int32_t* face_elem;    /* [num_faces] element of the face                                    */
+  const int8_t* face_axis;     /* [num_faces] reference axis normal to the face (0,1,2)              */
+  const int8_t* face_side;     /* [num_faces] 0: xi_axis = 0, 1: xi_axis = 1                          */
+  double traction[3];
+  /* point loads R[dof] += coef * lambda (AssemblerManipulator callbacks,
+   * assemblermanipulatorfuser.hh:243; tests/src/testcantileverbeam.hh:66-80) */
+  int64_t num_point_loads;
+  const int64_t* point_dof;
+  const double* point_coef;
+} ik_problem_desc;
+
+/* Create / destroy. Builds the CSR/BSR pattern and element block-slot maps on the first matrix call
+ * like SparseFlatAssembler::preProcessSparseMatrix (simpleassemblers.inl:290-293). */
+int ik_create(ik_handle** h, const ik_problem_desc* desc);
+void ik_destroy(ik_handle* h);
+const char* ik_last_error(const ik_handle* h);
+/* first element whose det C <= 0 in the last failing call, or -1 */
+int64_t ik_bad_element(const ik_handle* h);
+
+/* FlatAssemblerBase::size / reducedSize (interface.hh:68-74) */
+int64_t ik_size(const ik_handle* h);
+int64_t ik_reduced_size(const ik_handle* h);
+
+/* FERequirement (u, lambda) binding (ferequirements.hh:223-407; interface.hh:150-155).
+ * u is the full DOF vector (FlatInterleaved: dof = 3*node + component). */
+int ik_set_requirement(ik_handle* h, const double* u, double lambda, int where);
+int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where);
+
+/* VectorAssembler::vector(req, forces, dbc) (interface.hh:355-365; simpleassemblers.inl:60-118).
+ * R has size() entries (RAW/FULL) or reducedSize() entries (REDUCED). */
+int ik_vector(ik_handle* h, int dbc, double* R, int where);
+
+/* MatrixAssembler::matrix(req, stiffness, dbc) (interface.hh:430-440; simpleassemblers.inl:121-204).
+ * Assembles into library-owned 3x3-block CSR storage on the device (FP64, or FP32 for IK_FP32). */
+int ik_matrix(ik_handle* h, int dbc, int precision);
+
+/* Eigen ColMajor CSC layout of the last assembled matrix (simpleassemblers.inl:207-251):
+ * pass outer/inner/values = NULL to query n and nnz. outer: n+1, inner: nnz (int32), values: nnz (FP64). */
+int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* outer, int32_t* inner,
+                  double* values);
+
+/* Device view of the block storage: block rows = nodes, 3x3 row-major blocks (values FP64 or FP32). */
+int ik_matrix_bsr(ik_handle* h, int64_t* num_block_rows, int64_t* nnzb, const int32_t** rowptr,
+                  const int32_t** colidx, const void** values, int* precision);
+
+/* calculateMatrix / calculateVector for one element at the bound requirement (mixin.hh:137-152):
+ * Ke row-major 24x24 (local order 3a+k, fehelper.hh:146-152), Re 24. Host outputs, either may be NULL. */
+int ik_element(ik_handle* h, int64_t e, double* Ke, double* Re);
+
+/* EAS state update on CORRECTION_UPDATED: alpha -= D^{-1}(Rtilde + L du_e)
+ * (enhancedassumedstrains.hh:225-248, 350-359). du is the full correction vector. */
+int ik_update_state(ik_handle* h, const double* du, int where);
+/* EAS internal variables, [num_elements * eas_modes] (enhancedassumedstrains.hh:208) */
+int ik_get_alpha(ik_handle* h, double* alpha, int where);
+int ik_set_alpha(ik_handle* h, const double* alpha, int where);
+
+/* ---- linear solve (LinearSolverTemplate::analyzePattern/factorize/solve, linearsolver.hh:50-235) -- */
+typedef struct {
+  int32_t kind;        /* IK_PCG_JACOBI_FP64 | IK_PCG_BJACOBI_FP32_IR                       */
+  double rtol;         /* ||r|| <= rtol ||b||   (Eigen ConjugateGradient semantics)         */
+  int32_t max_iter;    /* <= 0: 2n (Eigen default)                                         */
+  double inner_rtol;   /* FP32 inner PCG tolerance per refinement sweep (IR form)          */
+  int32_t max_refine;  /* maximum FP64 refinement sweeps (IR form)                         */
+} ik_solver_opts;
+
+typedef struct {
+  int32_t iterations;    /* total PCG iterations                      */
+  int32_t refinements;   /* FP64 refinement sweeps (IR form)          */
+  double rel_residual;   /* final ||b - K x|| / ||b|| in FP64         */
+  int32_t converged;
+} ik_solver_info;
+
+/* Solve K x = rhs with the last assembled (FULL) matrix. */
+int ik_solve(ik_handle* h, const double* rhs, double* x, int where, const ik_solver_opts* opts,
+             ik_solver_info* info);
+
+/* ---- Newton-Raphson (newtonraphson.hh:196-257) and LoadControl (loadcontrol.inl:21-57) -------- */
+typedef struct {
+  double tol;            /* absolute ||R||_2 tolerance (NRSettings, newtonraphson.hh:27-32) */
+  int32_t max_iter;
+  int32_t min_iter;
+  int32_t precision;     /* tangent precision IK_FP64 | IK_FP32                             */
+  ik_solver_opts linear;
+  double forcing;        /* > 0: inexact Newton, linear rtol = min(linear.rtol,
+                            forcing * ||R_k|| / ||R_0||) ; 0: fixed linear.rtol          */
+} ik_nr_settings;
+
+typedef struct {
+  int32_t success;
+  int32_t iterations;
+  double residual_norm;
+  double correction_norm;
+  int64_t linear_iterations;
+} ik_nr_info;
+
+/* NewtonRaphson::solve(x, stepSize) on the bound requirement; state stays on the device. */
+int ik_newton_solve(ik_handle* h, const ik_nr_settings* s, double step_size, ik_nr_info* info);
+
+typedef struct {
+  int32_t load_steps;
+  double tbegin, tend;
+  int32_t predictor;      /* IK_PRED_*                                                      */
+} ik_lc_config;
+
+typedef struct {
+  int32_t success;
+  int32_t total_iterations;
+  int64_t linear_iterations;
+  int32_t solves;         /* number of Newton solves run (initial + load steps)              */
+} ik_control_info;
+
+/* LoadControl::run. iterations/residuals: optional arrays of load_steps+1 entries. */
+int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s, ik_control_info* info,
+                    int32_t* iterations, double* residuals);
+
+/* ---- learned per-node predictor (no reference counterpart) ------------------------------------ */
+/* MLP 14 -> 128 -> 128 -> 128 -> 3, SiLU, BF16 weights on MFMA with FP32 accumulation. Parameters are
+ * FP32 master copies laid out W1[128x14],b1,W2[128x128],b2,W3[128x128],b3,W4[3x128],b4. */
+int64_t ik_mlp_num_params(void);
+int ik_mlp_set_params(ik_handle* h, const float* params, int where);
+int ik_mlp_get_params(ik_handle* h, float* params, int where);
+/* load-path descriptor for the features: traction direction*magnitude, current and next lambda */
+int ik_mlp_set_load(ik_handle* h, const double traction[3], double char_length);
+
+/* ---- timing of the hot kernels on the handle's stream (HIP events) ----------------------------- */
+enum { IK_KERNEL_ASSEMBLE_KR = 0, IK_KERNEL_ASSEMBLE_R = 1, IK_KERNEL_SPMV = 2, IK_KERNEL_MLP = 3,
+       IK_KERNEL_COUNT = 4 };
+int ik_timing_enable(ik_handle* h, int on);
+int ik_timing_read(ik_handle* h, int kernel, int64_t* launches, double* total_ms);
+int ik_timing_reset(ik_handle* h);
+int ik_synchronize(ik_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IKARUS_AMD_H */

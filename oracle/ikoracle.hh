@@ -834,16 +834,20 @@ struct Assembler {
       const auto d = elementDofs(*P, e);
       for (int i = 0; i < nd; ++i) R[d[i]] += eo.R[i];
     }
-    for (auto& pl : P->pointLoads) R[pl.first] += pl.second * lambda;
-    if (dbc == Raw) return R;
-    if (dbc == Full) {
-      for (int64_t i = 0; i < P->ndof(); ++i)
-        if (P->fixed[i]) R[i] = 0.0;
+    // Point loads are AssemblerManipulator callbacks: they run after the DBC treatment of the wrapped
+    // assembler (assemblermanipulatorbuildingblocks.hh:103-122); reduced vectors take the reduced index.
+    if (dbc == Raw || dbc == Full) {
+      if (dbc == Full)
+        for (int64_t i = 0; i < P->ndof(); ++i)
+          if (P->fixed[i]) R[i] = 0.0;
+      for (auto& pl : P->pointLoads) R[pl.first] += pl.second * lambda;
       return R;
     }
     std::vector<double> Rr;
     for (int64_t i = 0; i < P->ndof(); ++i)
       if (!P->fixed[i]) Rr.push_back(R[i]);
+    for (auto& pl : P->pointLoads)
+      if (!P->fixed[pl.first]) Rr[pl.first - cb[pl.first]] += pl.second * lambda;
     return Rr;
   }
 

@@ -1,0 +1,172 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle on the same seeded inputs.
+
+Bars: sparsity pattern bit-exact; FP64 element/assembled values within 1e-10 relative (atomics make the
+summation order nondeterministic at the 1e-16 level); converged displacements within 1e-8 relative.
+"""
+import numpy as np
+import pytest
+
+import ikarus_amd as ia
+import pyoracle as po
+from ikarus_amd import _capi as capi
+from ikarus_amd import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_for(p, neas=None):
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, p.eas if neas is None else neas, 2)
+    o.set_fixed(p.fixed)
+    if len(p.face_elem):
+        o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction)
+    for d, c in zip(p.point_dof, p.point_coef):
+        o.add_point_load(int(d), float(c))
+    return o
+
+
+def _relerr(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_element_matrix_and_vector(material):
+    p = pb.hex_block((3, 3, 3), material=material, jitter=0.08, seed=11)
+    print(f"case: element parity material={material}", flush=True)
+    dev = ia.Device(p)
+    o = _oracle_for(p)
+    u = 0.03 * np.random.default_rng(1).standard_normal(p.ndof)
+    dev.set_requirement(u, 0.0)
+    for e in (0, 13, 26):
+        K, R = dev.element(e)
+        Ko, Ro = o.element(e, u, 0.0)
+        assert _relerr(K, Ko) < 1e-12, e
+        assert _relerr(R, Ro) < 1e-12, e
+
+
+@pytest.mark.parametrize("dbc", [capi.DBC_RAW, capi.DBC_FULL, capi.DBC_REDUCED])
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_assembled_matrix_and_vector(dbc, material):
+    p = pb.hex_block((6, 5, 4), material=material, jitter=0.05, seed=5)
+    print(f"case: assembly parity dbc={dbc} material={material}", flush=True)
+    dev = ia.Device(p)
+    o = _oracle_for(p)
+    u = 0.02 * np.random.default_rng(2).standard_normal(p.ndof)
+    lam = 0.6
+    dev.set_requirement(u, lam)
+    dev.matrix(dbc)
+    outer, inner, vals = dev.csc(dbc)
+    oo, io = o.pattern(dbc == capi.DBC_REDUCED)
+    assert np.array_equal(outer, oo) and np.array_equal(inner, io)
+    Ko = o.matrix(u, lam, dbc)
+    assert _relerr(vals, Ko) < 1e-10
+    np.testing.assert_allclose(vals, Ko, rtol=1e-9, atol=1e-12 * np.abs(Ko).max())
+    R = dev.vector(dbc)
+    Ro = o.vector(u, lam, dbc)
+    assert R.shape == Ro.shape
+    assert _relerr(R, Ro) < 1e-10
+    if dbc == capi.DBC_FULL:
+        f = p.fixed.astype(bool)
+        assert np.all(R[f] == 0.0)
+
+
+def test_fp32_tangent_close_to_fp64():
+    p = pb.hex_block((5, 5, 5), jitter=0.05, seed=9)
+    print("case: fp32 tangent", flush=True)
+    dev = ia.Device(p)
+    u = 0.02 * np.random.default_rng(4).standard_normal(p.ndof) * (1 - p.fixed)
+    dev.set_requirement(u, 1.0)
+    dev.matrix(capi.DBC_FULL, capi.FP64)
+    _, _, k64 = dev.csc()
+    dev.matrix(capi.DBC_FULL, capi.FP32)
+    _, _, k32 = dev.csc()
+    assert _relerr(k32, k64) < 2e-6
+
+
+def test_pattern_and_values_at_16cubed():
+    p = pb.hex_block((16, 16, 16), jitter=0.05, seed=1)
+    print("case: 16^3 parity", flush=True)
+    dev = ia.Device(p)
+    o = _oracle_for(p)
+    u = 0.01 * np.random.default_rng(6).standard_normal(p.ndof)
+    dev.set_requirement(u, 0.3)
+    dev.matrix(capi.DBC_FULL)
+    outer, inner, vals = dev.csc()
+    oo, io = o.pattern()
+    assert np.array_equal(outer, oo) and np.array_equal(inner, io)
+    assert _relerr(vals, o.matrix(u, 0.3, po.FULL)) < 1e-10
+    assert _relerr(dev.vector(), o.vector(u, 0.3, po.FULL)) < 1e-10
+
+
+@pytest.mark.parametrize("kind", [capi.PCG_JACOBI_FP64, capi.PCG_BJACOBI_FP32_IR])
+def test_linear_solve_matches_direct(kind):
+    p = pb.hex_block((5, 4, 6), jitter=0.05, seed=3)
+    print(f"case: linear solve kind={kind}", flush=True)
+    dev = ia.Device(p)
+    o = _oracle_for(p)
+    u = 0.01 * np.random.default_rng(7).standard_normal(p.ndof) * (1 - p.fixed)
+    dev.set_requirement(u, 0.5)
+    dev.matrix(capi.DBC_FULL, capi.FP64 if kind == capi.PCG_JACOBI_FP64 else capi.FP32)
+    b = np.random.default_rng(8).standard_normal(p.ndof) * (1 - p.fixed)
+    x, info = dev.solve(b, ia.solver_opts(kind=kind, rtol=1e-12, inner_rtol=1e-3))
+    xd = o.direct_solve(u, 0.5, b)
+    assert info.converged == 1
+    assert _relerr(x, xd) < (1e-9 if kind == capi.PCG_JACOBI_FP64 else 1e-5)
+    if kind == capi.PCG_JACOBI_FP64:
+        xc, it, _ = o.cg(u, 0.5, b, tol=1e-12)
+        assert abs(info.iterations - it) <= 2  # Eigen ConjugateGradient iteration count
+        assert _relerr(x, xc) < 1e-9
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_load_control_matches_oracle(material):
+    p = pb.hex_block((4, 4, 4), material=material, jitter=0.05, seed=21)
+    print(f"case: load control material={material}", flush=True)
+    dev = ia.Device(p)
+    o = _oracle_for(p)
+    r = o.load_control(3, 0.0, 1.0, tol=1e-8)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.LCConfig()
+    cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 3, 0.0, 1.0, capi.PRED_CONSTANT
+    info, iters, res = dev.load_control(cfg, ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-13)))
+    ug, lam = dev.get_requirement()
+    assert info.success and abs(lam - 1.0) < 1e-14
+    assert list(iters) == list(r["iterations"])
+    assert np.all(res[1:] <= 1e-8)
+    assert _relerr(ug, r["u"]) < 1e-8
+
+
+def test_mixed_precision_newton_reaches_fp64_tolerance():
+    p = pb.hex_block((6, 6, 6), jitter=0.05, seed=4)
+    print("case: mixed newton", flush=True)
+    o = _oracle_for(p)
+    r = o.load_control(2, 0.0, 1.0, tol=1e-8)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.LCConfig()
+    cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 2, 0.0, 1.0, capi.PRED_LINEAR
+    s = ia.nr_settings(tol=1e-8, precision=capi.FP32,
+                       linear=ia.solver_opts(kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+    info, iters, res = dev.load_control(cfg, s)
+    ug, _ = dev.get_requirement()
+    assert info.success and res[-1] <= 1e-8
+    assert _relerr(ug, r["u"]) < 1e-8
+
+
+def test_invalid_state_reports_element():
+    """A non-finite displacement makes det C fail the positivity check: the call returns IK_ENEGDET and
+    names the lowest element touching the node (the reference aborts, materialhelpers.hh:120-126)."""
+    p = pb.hex_block((3, 3, 3), jitter=0.0, seed=0)
+    print("case: invalid state", flush=True)
+    dev = ia.Device(p)
+    u = np.zeros(p.ndof)
+    node = 21  # vertex (1,1,1)
+    u[3 * node] = np.nan
+    dev.set_requirement(u, 0.0)
+    with pytest.raises(capi.IkError) as ei:
+        dev.vector()
+    assert ei.value.code == capi.IK_ENEGDET
+    first = int(np.nonzero(np.any(p.conn == node, axis=1))[0].min())
+    assert ei.value.bad_element == first
+    # the handle stays usable
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    assert np.all(dev.vector() == 0.0)

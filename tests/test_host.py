@@ -1,0 +1,77 @@
+"""Host logic and the C-ABI boundary without a GPU: mesh generation parity with the oracle, problem
+builders, and that libikarus_amd.so loads and exports every symbol of include/ikarus_amd.h."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from ikarus_amd import _capi as capi
+from ikarus_amd import problems as pb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("nel,L,jit,seed", [([3, 2, 4], [1.0, 2.0, 0.5], 0.05, 7), ([5, 3], [4.0, 1.0], 0.0, 0),
+                                            ([4, 4, 4], [1.0, 1.0, 1.0], 0.05, 123456789)])
+def test_mesh_matches_oracle_bitwise(nel, L, jit, seed):
+    X, conn = pb.structured_mesh(nel, L, jit, seed)
+    Xo, co = po.structured_mesh(nel, L, jit, seed)
+    assert np.array_equal(conn, co)
+    assert np.array_equal(X, Xo)
+
+
+def test_splitmix_known_values():
+    # SplitMix64 reference outputs for seed 0 (Steele, Lea, Flood 2014; as in java.util.SplittableRandom)
+    v = pb.splitmix64(0, 3)
+    assert [int(x) for x in v] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_jitter_interior_only_and_bounded():
+    X, _ = pb.structured_mesh([6, 5, 4], [1.0, 1.0, 1.0], 0.05, 3)
+    X0, _ = pb.structured_mesh([6, 5, 4], [1.0, 1.0, 1.0], 0.0, 3)
+    d = X - X0
+    h = np.array([1 / 6, 1 / 5, 1 / 4])
+    assert np.all(np.abs(d) <= 0.05 * h + 1e-15)
+    for k in range(3):
+        on = (np.abs(X0[:, k]) < 1e-12) | (np.abs(X0[:, k] - 1.0) < 1e-12)
+        assert np.all(d[on] == 0.0)
+
+
+def test_hex_block_problem():
+    p = pb.hex_block((4, 4, 4), seed=2)
+    assert p.fixed.sum() == 25 * 3
+    assert len(p.face_elem) == 16 and np.all(p.face_axis == 2) and np.all(p.face_side == 1)
+    m = np.linalg.norm(p.traction)
+    assert 10.0 <= m <= 40.0
+    c = pb.cantilever3d()
+    assert c.fixed.sum() == 4 * 3 and len(c.point_dof) == 2
+
+
+def test_library_exports_header_symbols():
+    lib = capi.lib()
+    header = open(os.path.join(ROOT, "include", "ikarus_amd.h")).read()
+    names = set(re.findall(r"\b(ik_[a-z0-9_]+)\s*\(", header))
+    assert names, "no declarations found"
+    assert names == set(capi.EXPORTED), names ^ set(capi.EXPORTED)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert lib.ik_mlp_num_params() == 14 * 128 + 128 + 2 * (128 * 128 + 128) + 3 * 128 + 3
+
+
+def test_capi_rejects_bad_input_without_gpu_compute():
+    """Validation happens before any device work: a 2D mesh and out-of-range connectivity are refused."""
+    lib = capi.lib()
+    X = np.zeros((4, 2))
+    conn = np.array([[0, 1, 2, 3]], np.int32)
+    d = capi.ProblemDesc()
+    d.dim = 2
+    d.num_nodes, d.num_elements = 4, 1
+    d.X, d.conn = capi.ptr(X), capi.ptr(conn)
+    h = C.c_void_p()
+    rc = lib.ik_create(C.byref(h), C.byref(d))
+    assert rc == capi.IK_ENOTIMPL
+    assert b"3D" in lib.ik_last_error(h)
+    lib.ik_destroy(h)

@@ -1,0 +1,269 @@
+"""The CPU oracle against the reference's own golden values and against mathematical invariants.
+
+Golden values are quoted from the reference test suite (file:line below). Invariants are chosen so a
+dropped term, a wrong sign or a transposed operand in the oracle fails at least one of them.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from ikarus_amd import problems as pb
+
+
+def _cantilever(dim, kind, neas):
+    nel = [10, 1] if dim == 2 else [10, 1, 1]
+    L = [10.0, 2.0] if dim == 2 else [10.0, 2.0, 2.0]
+    X, conn = po.structured_mesh(nel, L)
+    lam, mu = po.lame(100.0, 0.3)
+    o = po.Oracle(X, conn, kind, lam, mu, neas, 2)
+    o.set_fixed(pb.clamp_flags(X, 0, 0.0))
+    pts = [[10, 2]] if dim == 2 else [[10, 2, 2], [10, 2, 0]]
+    for p in pts:
+        o.add_point_load(pb.vertex_at(X, p) * dim + 1, 1.0)
+    return X, o
+
+
+# tests/src/testcantileverbeamEAS.cpp:25-29 (2D, EAS-4 GL) and :61-67 (3D, EAS-21 GL); tol 1e-10, 20 steps
+@pytest.mark.parametrize("dim,kind,neas,iters,maxu", [
+    (3, po.NEOHOOKE, 21, 80, 4.781820664768682),
+    (3, po.SVK, 21, 80, 4.7692391315649365),
+    (2, po.NEOHOOKE, 4, 80, 4.479930218997457),
+    (2, po.SVK, 4, 80, 4.459851990257645),
+])
+def test_cantilever_eas_golden(dim, kind, neas, iters, maxu):
+    X, o = _cantilever(dim, kind, neas)
+    r = o.load_control(20, 0.0, 1.0, tol=1e-10)
+    assert r["success"]
+    assert r["total"] == iters
+    assert abs(np.max(np.abs(r["u"])) - maxu) <= 1e-10 * maxu
+    assert abs(r["lam"] - 1.0) < 1e-12
+
+
+def _strip(kind, lam, mu):
+    X, conn = po.structured_mesh([10, 10], [10.0, 10.0])
+    o = po.Oracle(X, conn, kind, lam, mu, 0, 2)
+    fixed = pb.clamp_flags(X, 0, 0.0)
+    right = np.abs(X[:, 0] - 10.0) < 1e-8
+    fixed[np.nonzero(right)[0] * 2 + 1] = 1
+    fixed[np.nonzero(right)[0] * 2] = 1          # set by storeInhomogeneousBoundaryCondition
+    ubar = np.zeros(X.size)
+    ubar[np.nonzero(right)[0] * 2] = 10.0
+    o.set_fixed(fixed)
+    o.set_ubar(ubar)
+    return X, o, fixed
+
+
+# tests/src/testinhomogeneousdbc.cpp:24-31 (Q1 rows), strip problem tests/src/testelasticstrip.hh:43-223
+@pytest.mark.parametrize("kind,mat,iters,uy", [
+    (po.NEOHOOKE, (24.0, 6.0), 7, 2.207111977584091),
+    (po.SVK, "E100", 6, 1.814746879163122),
+])
+def test_elastic_strip_idbc_golden(kind, mat, iters, uy):
+    lam, mu = po.lame(100.0, 0.3) if mat == "E100" else mat
+    X, o, fixed = _strip(kind, lam, mu)
+    assert fixed.sum() == 4 * 11  # testelasticstrip.hh:117-118
+    r = o.load_control(1, 0.0, 1.0, tol=1e-8)
+    assert r["success"] and r["total"] == iters
+    v = pb.vertex_at(X, (5.0, 0.0))
+    assert abs(r["u"][2 * v + 1] - uy) <= 1e-10 * abs(uy)
+    right = np.nonzero(np.abs(X[:, 0] - 10.0) < 1e-8)[0]
+    assert np.allclose(r["u"][2 * right], 10.0, rtol=1e-10)
+
+
+# tests/src/testnonlinearelasticity.hh:248-338: single distorted Q1, SVK E=1000 nu=0
+@pytest.mark.parametrize("nq,expected", [
+    (2, [1845.6296388251504753, 14192.4707553121224317, 19964.32719133414782, 29973.7943273325380486,
+         46641.183728849332812, 95447.6156712376251918]),
+    (1, [1476.2425432014910, 20458.563308258160, 33153.273934018240, 92733.379734804420]),
+])
+def test_single_element_eigenvalues_golden(nq, expected):
+    X = np.array([[0, 0], [1, 0], [0, 1], [1, 1]], float) + np.array(
+        [[-0.2, -0.05], [-0.15, 0.05], [0.15, 0.15], [-0.05, -0.1]])  # testcommon.hh:208-213
+    lam, mu = po.lame(1000.0, 0.0)
+    o = po.Oracle(X, np.array([[0, 1, 2, 3]], np.int32), po.SVK, lam, mu, 0, nq)
+    d = np.array([2, 4, 3.25, -1.2, 0.003, 6, 3, 2.864])
+    K, _ = o.element(0, d, 0.0)
+    ev = np.sort(np.abs(np.linalg.eigvalsh(K)))
+    assert np.all(ev[:8 - len(expected)] < 1e-8)
+    np.testing.assert_allclose(ev[8 - len(expected):], expected, rtol=1e-10)
+
+
+def _hex(kind=po.NEOHOOKE, neas=0, seed=1):
+    X, conn = po.structured_mesh([2, 2, 2], [1.0, 1.2, 0.9], 0.1, seed)
+    lam, mu = po.lame(1000.0, 0.3)
+    return X, conn, po.Oracle(X, conn, kind, lam, mu, neas, 2)
+
+
+@pytest.mark.parametrize("kind", [po.NEOHOOKE, po.SVK])
+def test_tangent_is_derivative_of_residual(kind):
+    """K = dR/du by central differences (checkfebyautodiff.hh:19-115 restated with FD)."""
+    X, conn, o = _hex(kind, 0)
+    rng = np.random.default_rng(3)
+    u = 0.02 * rng.standard_normal(X.size)
+    K = o.matrix_dense(u, 0.0, po.RAW)
+    h = 1e-6
+    for j in rng.choice(X.size, 6, replace=False):
+        e = np.zeros(X.size)
+        e[j] = h
+        fd = (o.vector(u + e, 0.0, po.RAW) - o.vector(u - e, 0.0, po.RAW)) / (2 * h)
+        np.testing.assert_allclose(K[:, j], fd, rtol=1e-6, atol=1e-6 * np.abs(K).max())
+
+
+@pytest.mark.parametrize("kind", [po.NEOHOOKE, po.SVK])
+def test_eas_blocks_are_derivatives(kind):
+    """EAS element (enhancedassumedstrains.hh:258-434): with R_u = R + L^T D^-1 Rt and
+    K_uu = K + L^T D^-1 L recovered from the condensed outputs, K_uu = dR_u/du, L = dRt/du and
+    D = dRt/dalpha (GL enhancement has E_aa = E_ad = 0), by central differences at alpha != 0."""
+    X, conn, o = _hex(kind, 21)
+    rng = np.random.default_rng(8)
+    u = 0.02 * rng.standard_normal(X.size)
+    alpha = 1e-5 * rng.standard_normal((conn.shape[0], 21))
+    o.set_alpha(alpha)
+    e0 = 2
+    dofs = (conn[e0][:, None] * 3 + np.arange(3)).ravel()
+
+    def parts(uu):
+        K, R, D, L, Rt = o.element(e0, uu, 0.0)
+        Di = np.linalg.inv(D)
+        return K + L.T @ Di @ L, R + L.T @ Di @ Rt, D, L, Rt
+
+    Kuu, Ru, D, L, Rt = parts(u)
+    np.testing.assert_allclose(Kuu, Kuu.T, rtol=1e-10, atol=1e-10 * np.abs(Kuu).max())
+    h = 1e-6
+    for j in rng.choice(24, 6, replace=False):
+        e = np.zeros(X.size)
+        e[dofs[j]] = h
+        p, m = parts(u + e), parts(u - e)
+        np.testing.assert_allclose(Kuu[:, j], (p[1] - m[1]) / (2 * h), rtol=1e-5, atol=1e-6 * np.abs(Kuu).max())
+        np.testing.assert_allclose(L[:, j], (p[4] - m[4]) / (2 * h), rtol=1e-5, atol=1e-6 * np.abs(L).max())
+    ha = 1e-8
+    for i in rng.choice(21, 5, replace=False):
+        ap, am = alpha.copy(), alpha.copy()
+        ap[e0, i] += ha
+        am[e0, i] -= ha
+        o.set_alpha(ap)
+        rp = o.element(e0, u, 0.0)[4]
+        o.set_alpha(am)
+        rm = o.element(e0, u, 0.0)[4]
+        np.testing.assert_allclose(D[:, i], (rp - rm) / (2 * ha), rtol=1e-5, atol=1e-6 * np.abs(D).max())
+    o.set_alpha(alpha)
+
+
+@pytest.mark.parametrize("kind", [po.NEOHOOKE, po.SVK])
+def test_residual_is_derivative_of_energy(kind):
+    X, conn, o = _hex(kind)
+    rng = np.random.default_rng(5)
+    u = 0.02 * rng.standard_normal(X.size)
+    R = o.vector(u, 0.0, po.RAW)
+    h = 1e-6
+    for j in rng.choice(X.size, 6, replace=False):
+        e = np.zeros(X.size)
+        e[j] = h
+        fd = (o.energy(u + e, 0.0) - o.energy(u - e, 0.0)) / (2 * h)
+        assert abs(fd - R[j]) <= 1e-6 * max(1.0, np.abs(R).max())
+
+
+def test_rigid_body_modes_and_linear_limit():
+    """K(u=0) has the 6 rigid-body modes of a free 3D body in its null space; NH and SVK coincide at u=0
+    (GreenLagrangeStrainTest, testnonlinearelasticity.hh:194-246)."""
+    X, conn, o_nh = _hex(po.NEOHOOKE)
+    _, _, o_svk = _hex(po.SVK)
+    K = o_nh.matrix_dense(np.zeros(X.size), 0.0, po.RAW)
+    Ks = o_svk.matrix_dense(np.zeros(X.size), 0.0, po.RAW)
+    np.testing.assert_allclose(K, Ks, rtol=1e-12, atol=1e-9)
+    ev = np.sort(np.linalg.eigvalsh(K))
+    assert np.all(np.abs(ev[:6]) < 1e-8 * ev[-1]) and ev[6] > 1e-6 * ev[-1]
+    # translations and an infinitesimal rotation are in the null space
+    t = np.tile([1.0, 0.0, 0.0], X.shape[0])
+    rot = np.column_stack([-X[:, 1], X[:, 0], np.zeros(X.shape[0])]).ravel()
+    assert np.linalg.norm(K @ t) < 1e-9 * np.linalg.norm(K)
+    assert np.linalg.norm(K @ rot) < 1e-9 * np.linalg.norm(K)
+
+
+def test_eas_zero_alpha_orthogonality():
+    """EAS ansatz: int M detJ dxi = 0 over the element (testnonlineareas.cpp:95-143): with u = 0 and
+    alpha = 0 the enhanced residual Rtilde vanishes, and D is symmetric positive definite."""
+    X, conn, o = _hex(po.NEOHOOKE, 21)
+    K, R, D, L, Rt = o.element(0, np.zeros(X.size), 0.0)
+    assert np.abs(Rt).max() < 1e-12
+    np.testing.assert_allclose(D, D.T, rtol=1e-12, atol=1e-10)
+    assert np.linalg.eigvalsh(D).min() > 0
+    # with zero modes the element equals the plain displacement element
+    _, _, o0 = _hex(po.NEOHOOKE, 0)
+    u = 0.01 * np.random.default_rng(0).standard_normal(X.size)
+    K0, R0 = o0.element(1, u, 0.0)
+    Ke, Re, *_ = o.element(1, u, 0.0)
+    # condensation subtracts L^T D^-1 L (PSD): K0 - Ke is positive semidefinite
+    assert np.linalg.eigvalsh(K0 - Ke).min() > -1e-8 * np.abs(K0).max()
+
+
+def test_pattern_equals_dense_union_of_cliques():
+    X, conn, o = _hex()
+    fixed = pb.clamp_flags(X, 2, 0.0)
+    o.set_fixed(fixed)
+    for reduced in (False, True):
+        outer, inner = o.pattern(reduced)
+        n = X.size if not reduced else int(X.size - fixed.sum())
+        dense = np.zeros((X.size, X.size), bool)
+        for e in conn:
+            d = (e[:, None] * 3 + np.arange(3)).ravel()
+            dense[np.ix_(d, d)] = True
+        if reduced:
+            free = ~fixed.astype(bool)
+            dense = dense[np.ix_(free, free)]
+        assert len(outer) == n + 1
+        for c in range(n):
+            rows = inner[outer[c]:outer[c + 1]]
+            assert np.all(np.diff(rows) > 0)
+            assert np.array_equal(rows, np.nonzero(dense[:, c])[0])
+    # slot maps point at the (r, c) entry, column-major over local pairs
+    outer, inner = o.pattern(False)
+    s = o.slots(3)
+    d = (conn[3][:, None] * 3 + np.arange(3)).ravel()
+    for li, (c, r) in enumerate([(c, r) for c in d for r in d]):
+        assert inner[s[li]] == r and outer[c] <= s[li] < outer[c + 1]
+
+
+def test_dbc_full_and_reduced_semantics():
+    """testassembler.cpp:122-201: Full zeroes rows/cols with unit diagonal; Reduced = Raw restricted."""
+    X, conn, o = _hex()
+    fixed = pb.clamp_flags(X, 2, 0.0)
+    o.set_fixed(fixed)
+    u = 0.01 * np.random.default_rng(1).standard_normal(X.size)
+    Kraw = o.matrix_dense(u, 0.0, po.RAW)
+    Kfull = o.matrix_dense(u, 0.0, po.FULL)
+    Kred = o.matrix_dense(u, 0.0, po.REDUCED)
+    f = fixed.astype(bool)
+    assert np.all(Kfull[f][:, ~f] == 0) and np.all(Kfull[~f][:, f] == 0)
+    assert np.all(np.diag(Kfull)[f] == 1.0) and np.trace(Kfull[np.ix_(f, f)]) == f.sum()
+    np.testing.assert_array_equal(Kfull[np.ix_(~f, ~f)], Kraw[np.ix_(~f, ~f)])
+    np.testing.assert_array_equal(Kred, Kraw[np.ix_(~f, ~f)])
+    Rraw = o.vector(u, 0.0, po.RAW)
+    np.testing.assert_array_equal(o.vector(u, 0.0, po.FULL), np.where(f, 0.0, Rraw))
+    np.testing.assert_array_equal(o.vector(u, 0.0, po.REDUCED), Rraw[~f])
+
+
+def test_cg_matches_direct_solve():
+    X, conn = po.structured_mesh([3, 3, 3], [1.0, 1.0, 1.0], 0.05, 2)
+    lam, mu = po.lame(1000.0, 0.3)
+    o = po.Oracle(X, conn, po.NEOHOOKE, lam, mu, 0, 2)
+    o.set_fixed(pb.clamp_flags(X, 2, 0.0))
+    b = np.random.default_rng(2).standard_normal(X.size) * (1 - pb.clamp_flags(X, 2, 0.0))
+    u = np.zeros(X.size)
+    xd = o.direct_solve(u, 0.0, b)
+    K = o.matrix_dense(u, 0.0, po.FULL)
+    np.testing.assert_allclose(K @ xd, b, atol=1e-10 * np.abs(b).max())
+    xc, it, err = o.cg(u, 0.0, b, tol=1e-13)
+    assert err < 1e-13 and it > 0
+    np.testing.assert_allclose(xc, xd, rtol=0, atol=1e-9 * np.abs(xd).max())
+
+
+def test_newton_quadratic_convergence():
+    X, conn = po.structured_mesh([2, 2, 2], [1.0, 1.0, 1.0], 0.05, 4)
+    lam, mu = po.lame(1000.0, 0.3)
+    o = po.Oracle(X, conn, po.NEOHOOKE, lam, mu, 0, 2)
+    o.set_fixed(pb.clamp_flags(X, 2, 0.0))
+    o.set_traction(pb.face_elements([2, 2, 2], 2, 1), 2, 1, [20.0, -10.0, 30.0])
+    r = o.load_control(2, 0.0, 1.0, tol=1e-12)
+    assert r["success"] and np.all(r["iterations"][1:] <= 6)
+    assert r["residuals"][-1] <= 1e-12
