@@ -1,0 +1,205 @@
+"""Benchmark of the hex8 Neo-Hooke hot path on one MI355X.
+
+Workload (default): 3D hex8 Neo-Hooke block, 64x64x64 elements (823,875 DOFs), unit cube, E=1000, nu=0.3,
+seeded +-5% interior jitter (seed 0), bottom face clamped, seeded top-face traction, FP64.
+
+One step = one full Newton-iteration assembly of the hot path: tangent K (FP64, FULL Dirichlet
+treatment, scattered into the 3x3-block CSR pattern) plus residual R, inputs resident in HBM.
+value = elements assembled per second (Melem/s) over the timed steps.
+
+Also reported (same run): the dominant kernel's roofline figures (HIP events on the library stream),
+the CPU baseline (oracle restatement timed on a bounded sample on the host cores), SpMV bandwidth,
+and the Newton load path (5 load steps to ||R|| <= 1e-8) in the plain FP64 form and the mixed form.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import ikarus_amd as ia  # noqa: E402
+from ikarus_amd import _capi as capi  # noqa: E402
+from ikarus_amd import problems as pb  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+FLOP_PER_ELEM_KR = 86176.0   # 8 Gauss points x 10,772 (dense B^T C B + kinematics + material + kg + B^T S)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds):
+    """Oracle (reference-cost port: tangent per node pair, serial scatter) K+R assembly on a 16^3 sample,
+    repeated until `seconds` of CPU time, single thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    p = pb.hex_block((16, 16, 16), seed=0)
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, 0, 2)
+    o.set_fixed(p.fixed)
+    o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction)
+    o.set_faithful(True)
+    u = 0.005 * np.random.default_rng(0).standard_normal(p.ndof) * (1 - p.fixed)
+    o.pattern()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.matrix(u, 1.0)
+        o.vector(u, 1.0)
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=round(n * p.ne / dt / 1e6, 6), unit="Melem/s", cores=1, kind="port",
+                sample=f"{n}x K+R assembly of a 16^3 hex8 NH mesh (4096 elements), reference cost structure "
+                       f"(tangent per node pair), serial, {dt:.1f} s")
+
+
+def newton_path(p, mode, steps=5):
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.LCConfig()
+    cfg.load_steps, cfg.tbegin, cfg.tend = steps, 0.0, 1.0
+    if mode == "fp64":
+        cfg.predictor = capi.PRED_CONSTANT
+        s = ia.nr_settings(tol=1e-8, precision=capi.FP64, linear=ia.solver_opts(capi.PCG_JACOBI_FP64, rtol=1e-10))
+    else:
+        cfg.predictor = capi.PRED_LINEAR
+        s = ia.nr_settings(tol=1e-8, precision=capi.FP32,
+                           linear=ia.solver_opts(capi.PCG_BJACOBI_FP32_IR, rtol=1e-6, inner_rtol=1e-3, max_refine=20),
+                           forcing=1e-2)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    info, iters, res = dev.load_control(cfg, s)
+    dev.synchronize()
+    dt = time.perf_counter() - t0
+    u, _ = dev.get_requirement()
+    dev.close()
+    return dict(seconds=round(dt, 4), success=bool(info.success), newton_iterations=[int(i) for i in iters],
+                linear_iterations=int(info.linear_iterations), final_residual=float(res[info.solves - 1])), u
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="elements per direction")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding design is "
+              "described in DESIGN.md, nothing here uses a second GPU.", file=sys.stderr)
+        sys.exit(2)
+
+    n = args.n
+    p = pb.hex_block((n, n, n), seed=0)
+    log(f"[bench] problem {p.name}: {p.ne} elements, {p.ndof} dofs")
+    dev = ia.Device(p)
+    u = 0.005 * np.random.default_rng(0).standard_normal(p.ndof) * (1 - p.fixed)
+    dev.set_requirement(u, 1.0)
+    def step():
+        dev.check(capi.lib().ik_matrix(dev.h, capi.DBC_FULL, capi.FP64))
+        dev.check(capi.lib().ik_vector(dev.h, capi.DBC_FULL, None, capi.DEVICE))
+
+    for _ in range(args.warmup):
+        step()
+    dev.synchronize()
+    log("[bench] warmup done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dev.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    value = p.ne * args.steps / dt / 1e6
+    log(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} Melem/s")
+
+    # dominant kernel: tangent assembly, timed with HIP events on the library's stream
+    dev.timing(True)
+    dev.timing_reset()
+    for _ in range(args.steps):
+        step()
+    nk, msk = dev.timing_read(capi.KERNEL_ASSEMBLE_KR)
+    nr, msr = dev.timing_read(capi.KERNEL_ASSEMBLE_R)
+    dev.timing(False)
+    k_ms = msk / max(nk, 1)                      # average element-kernel launch (one element chunk)
+    elems_per_launch = p.ne * args.steps / max(nk, 1)
+    achieved = FLOP_PER_ELEM_KR * elems_per_launch / (k_ms * 1e-3) / 1e12
+    ng, msg = dev.timing_read(capi.KERNEL_GATHER_K)
+    # SpMV bandwidth: algorithmic bytes 8*nnz + 4*nnzb + 4*(nn+1) + 16*ndof
+    nbr, nnzb = np_i64(), np_i64()
+    dev.check(capi.lib().ik_matrix_bsr(dev.h, capi.ptr(nbr), capi.ptr(nnzb), None, None, None, None))
+    nnzb = int(nnzb[0])
+    x = np.random.default_rng(1).standard_normal(p.ndof)
+    dev.timing(True)
+    dev.timing_reset()
+    opts = ia.solver_opts(capi.PCG_JACOBI_FP64, rtol=1e-30, max_iter=50)
+    dev.solve(x, opts)
+    ns, mss = dev.timing_read(capi.KERNEL_SPMV)
+    dev.timing(False)
+    spmv_ms = mss / max(ns, 1)
+    spmv_bytes = 8 * 9 * nnzb + 4 * nnzb + 4 * (p.X.shape[0] + 1) + 16 * p.ndof
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    dev.close()
+    log(f"[bench] element kernel {k_ms:.3f} ms/launch x {nk / args.steps:.0f} launches ({achieved:.1f} TFLOP/s "
+        f"algorithmic), K gather {msg / max(ng, 1):.3f} ms/launch, R kernel {msr / max(nr, 1):.3f} ms/launch, "
+        f"SpMV {spmv_ms:.3f} ms ({spmv_gbs:.0f} GB/s)")
+
+    out = {
+        "metric": "Newton solve time to ||R||<=1e-8 (mixed+learned vs FP64); assembly Melem/s",
+        "value": round(value, 3),
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"hex8 Neo-Hooke K+R assembly, {n}x{n}x{n} elements, unit cube, 5% jitter seed 0, "
+                               "FULL Dirichlet, 3x3-block CSR", "elements": p.ne, "dofs": p.ndof,
+                   "block_nnz": nnzb, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "kernel": "hex8Element<double,NeoHooke,K>", "kernel_ms_per_launch": round(k_ms, 4),
+                     "elements_per_launch": int(elems_per_launch), "flop_per_elem": FLOP_PER_ELEM_KR},
+        "assembly_breakdown_ms_per_step": {
+            "element": round(msk / args.steps, 4), "gather_K": round(msg / args.steps, 4),
+            "residual": round(msr / args.steps, 4)},
+        "spmv": {"ms": round(spmv_ms, 4), "GBs": round(spmv_gbs, 1), "frac_hbm": round(spmv_gbs / PEAK_HBM_GBS, 4),
+                 "bytes": spmv_bytes},
+        "residual_kernel_ms": round(msr / max(nr, 1), 4),
+    }
+    if not args.no_newton:
+        log("[bench] newton fp64 ...")
+        r64, u64 = newton_path(p, "fp64")
+        log(f"[bench] newton fp64 {r64}")
+        log("[bench] newton mixed ...")
+        rmx, umx = newton_path(p, "mixed")
+        log(f"[bench] newton mixed {rmx}")
+        out["newton"] = {"load_steps": 5, "fp64": r64, "mixed": rmx,
+                         "speedup_mixed_vs_fp64": round(r64["seconds"] / rmx["seconds"], 3),
+                         "rel_diff_u": float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))}
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def np_i64():
+    return np.zeros(1, dtype=np.int64)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,7 +17,7 @@ FP64, FP32 = 0, 1
 HOST, DEVICE = 0, 1
 PCG_JACOBI_FP64, PCG_BJACOBI_FP32_IR = 0, 1
 PRED_CONSTANT, PRED_LINEAR, PRED_MLP = 0, 1, 2
-KERNEL_ASSEMBLE_KR, KERNEL_ASSEMBLE_R, KERNEL_SPMV, KERNEL_MLP = 0, 1, 2, 3
+KERNEL_ASSEMBLE_KR, KERNEL_ASSEMBLE_R, KERNEL_SPMV, KERNEL_MLP, KERNEL_GATHER_K = 0, 1, 2, 3, 4
 
 EXPORTED = [
     "ik_create", "ik_destroy", "ik_last_error", "ik_bad_element", "ik_size", "ik_reduced_size",

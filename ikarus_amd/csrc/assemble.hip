@@ -1,20 +1,22 @@
-// assemble.hip — hex8 total-Lagrangian element residual/tangent and scatter into the block CSR matrix.
+// assemble.hip — hex8 total-Lagrangian element residual/tangent and their assembly into the block CSR.
 //
 // Restates NonLinearElastic::calculateMatrixImpl / calculateVectorImpl
 // (ikarus/finiteelements/mechanics/nonlinearelastic.hh:375-430) with the Neo-Hooke
 // (materials/hyperelastic/neohooke.hh:79-142) and St.Venant-Kirchhoff (materials/svk.hh:78-164) laws,
-// the Traction skill (mechanics/loads/traction.hh:106-138) and the SparseFlatAssembler scatter
-// (ikarus/assembler/simpleassemblers.inl:121-169).
+// the Traction skill (mechanics/loads/traction.hh:106-138) and the SparseFlatAssembler scatter with its
+// Dirichlet treatment (ikarus/assembler/simpleassemblers.inl:60-169).
 //
-// Work decomposition (one 64-lane wavefront = 8 elements, 8 lanes per element):
-//   phase 0  lane (el, q) stages node q of element el (X, u, Dirichlet flags) in LDS;
-//   phase 1  lane (el, q) evaluates Gauss point q: J, J^-1, dN/dX for all 8 nodes, F, S*dV, C*dV (LDS);
-//   phase 2  lane (el, a) accumulates block row a of K_e (8 blocks of 3x3) and R_e[a] over the 8 Gauss
-//            points, reusing C*B_a for all b (the tangent is evaluated once per Gauss point, not per
-//            node pair as at nonlinearelastic.hh:318);
-//   phase 3  lane (el, a) adds its blocks into the block CSR values through the precomputed
-//            block-slot map with hardware FP64/FP32 atomics (no sort, no coloring).
-// The 8 Gauss points of a hex8 and its 8 nodes map onto the same 8 lanes, so no lane idles.
+// Element kernel (one 64-lane wavefront = 8 elements, 8 lanes per element):
+//   phase 0  lane (el, q) stages node q of element el (X, u) in LDS;
+//   phase 1  lane (el, q) evaluates Gauss point q: J, J^-1, dN/dX of all 8 nodes, F, S*dV, C*dV (LDS);
+//   phase 2  lane (el, a) accumulates 4-5 of the 36 upper 3x3 blocks of K_e (blocks (a, a+j mod 8),
+//            j = 0..3, plus j = 4 on lanes 0-3, so every lane has the same work) and R_e[a] over the
+//            8 Gauss points; C*B_a is formed once per Gauss point and reused for every block;
+//   phase 3  the blocks and R_e are stored element-by-element with plain stores.
+// Gather kernels then sum, for every 3x3 block of the CSR pattern, the element blocks listed for it
+// (a fixed order, so the result is bitwise reproducible), apply the FULL Dirichlet treatment on the fly
+// and write each block once. Elements are processed in chunks; each chunk's gather runs right behind
+// it on the rows the chunk completes, so the staged blocks are read back while still cached.
 #include "ik_internal.hh"
 
 namespace ik {
@@ -149,20 +151,19 @@ struct Lds {
   T F[EPB][8][9];
   T S[EPB][8][6];     // S * dV
   T C[EPB][8][21];    // C * dV (packed)
-  int32_t node[EPB][8];
-  uint8_t fix[EPB][24];
 };
 
-// One launch covers elements [e0, e0 + count). Kdense != nullptr writes K_e/R_e of element e0 densely
-// (row-major 24x24, local order 3a+k) instead of scattering.
+// upper-block index of (a, b), a <= b: row-major over the upper triangle of the 8x8 node pairs
+__host__ __device__ constexpr int upperBlock(int a, int b) { return 8 * a - (a * (a - 1)) / 2 + (b - a); }
+
+// One launch covers elements [e0, e0 + count). Outputs per element e (absolute index):
+//   Kebe[(e*36 + sb)*9 + 3k + l] = K_e[3a+k][3b+l] for the upper block sb = upperBlock(a, b)
+//   Rebe[e*24 + 3a + k]          = R_e[3a+k]
 template <typename T, int KIND, bool WANT_K, bool WANT_R>
-__global__ __launch_bounds__(64) void hex8Assemble(const double* __restrict__ X, const double* __restrict__ U,
-                                                   const int32_t* __restrict__ conn,
-                                                   const int32_t* __restrict__ bslot,
-                                                   const uint8_t* __restrict__ fixed, int64_t e0, int64_t count,
-                                                   int maskDbc, double lam_, double mu_, T* __restrict__ K,
-                                                   double* __restrict__ R, unsigned long long* __restrict__ err,
-                                                   double* __restrict__ Kdense, double* __restrict__ Rdense) {
+__global__ __launch_bounds__(64) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
+                                                  const int32_t* __restrict__ conn, int64_t e0, int64_t count,
+                                                  double lam_, double mu_, T* __restrict__ Kebe,
+                                                  double* __restrict__ Rebe, unsigned long long* __restrict__ err) {
   __shared__ Lds<T> s;
   const int lane     = threadIdx.x;
   const int el       = lane >> 3;
@@ -174,13 +175,11 @@ __global__ __launch_bounds__(64) void hex8Assemble(const double* __restrict__ X,
 
   // phase 0: stage node q
   if (valid) {
-    const int32_t nd = conn[e * 8 + q];
-    s.node[el][q]    = nd;
+    const int64_t nd = conn[e * 8 + q];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      s.X[el][q][k]       = T(X[(int64_t)nd * 3 + k]);
-      s.U[el][q][k]       = T(U[(int64_t)nd * 3 + k]);
-      s.fix[el][q * 3 + k] = fixed[(int64_t)nd * 3 + k];
+      s.X[el][q][k] = T(X[nd * 3 + k]);
+      s.U[el][q][k] = T(U[nd * 3 + k]);
     }
   }
   __syncthreads();
@@ -226,21 +225,25 @@ __global__ __launch_bounds__(64) void hex8Assemble(const double* __restrict__ X,
     for (int i = 0; i < 9; ++i) s.F[el][q][i] = F[i / 3][i % 3];
 #pragma unroll
     for (int v = 0; v < 6; ++v) s.S[el][q][v] = Sv[v] * dV;
+    if constexpr (WANT_K) {
 #pragma unroll
-    for (int i = 0; i < 21; ++i) s.C[el][q][i] = Cp[i] * dV;
+      for (int i = 0; i < 21; ++i) s.C[el][q][i] = Cp[i] * dV;
+    }
   }
   __syncthreads();
   if (!valid) return;
 
-  // phase 2: block row a = q
+  // phase 2: node a = q owns upper blocks (a, (a+j) & 7), j = 0..3 (+ j = 4 when a < 4)
   const int a = q;
-  T Kab[8][9];
+  constexpr int NB = 5;
+  const int nb = a < 4 ? 5 : 4;
+  T Kb[NB][9];
   T Ra[3] = {T(0), T(0), T(0)};
   if constexpr (WANT_K) {
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
-      for (int i = 0; i < 9; ++i) Kab[b][i] = T(0);
+      for (int i = 0; i < 9; ++i) Kb[j][i] = T(0);
   }
   for (int g = 0; g < 8; ++g) {
     T F[3][3], Sv[6], ga[3];
@@ -262,9 +265,6 @@ __global__ __launch_bounds__(64) void hex8Assemble(const double* __restrict__ X,
       }
     }
     if constexpr (WANT_K) {
-      T Cp[21];
-#pragma unroll
-      for (int i = 0; i < 21; ++i) Cp[i] = s.C[el][g][i];
       T CBa[6][3];
 #pragma unroll
       for (int v = 0; v < 6; ++v)
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(64) void hex8Assemble(const double* __restrict__ X,
         for (int k = 0; k < 3; ++k) {
           T acc = T(0);
 #pragma unroll
-          for (int w = 0; w < 6; ++w) acc += Cp[sym6(v, w)] * Ba[w][k];
+          for (int w = 0; w < 6; ++w) acc += s.C[el][g][sym6(v, w)] * Ba[w][k];
           CBa[v][k] = acc;
         }
       // S g_a for the geometric stiffness g_a^T S g_b
@@ -280,67 +280,114 @@ __global__ __launch_bounds__(64) void hex8Assemble(const double* __restrict__ X,
       const T Sga1 = Sv[5] * ga[0] + Sv[1] * ga[1] + Sv[3] * ga[2];
       const T Sga2 = Sv[4] * ga[0] + Sv[3] * ga[1] + Sv[2] * ga[2];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        T gb[3];
+      for (int j = 0; j < NB; ++j) {
+        if (j < nb) {
+          const int b = (a + j) & 7;
+          T gb[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) gb[i] = s.G[el][g][b][i];
-        T Bb[6][3];
-        bop(F, gb, Bb);
+          for (int i = 0; i < 3; ++i) gb[i] = s.G[el][g][b][i];
+          T Bb[6][3];
+          bop(F, gb, Bb);
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+          for (int k = 0; k < 3; ++k)
 #pragma unroll
-          for (int l = 0; l < 3; ++l) {
-            T acc = Kab[b][k * 3 + l];
+            for (int l = 0; l < 3; ++l) {
+              T acc = Kb[j][k * 3 + l];
 #pragma unroll
-            for (int v = 0; v < 6; ++v) acc += CBa[v][k] * Bb[v][l];
-            Kab[b][k * 3 + l] = acc;
-          }
-        const T kg = Sga0 * gb[0] + Sga1 * gb[1] + Sga2 * gb[2];
-        Kab[b][0] += kg;
-        Kab[b][4] += kg;
-        Kab[b][8] += kg;
+              for (int v = 0; v < 6; ++v) acc += CBa[v][k] * Bb[v][l];
+              Kb[j][k * 3 + l] = acc;
+            }
+          const T kg = Sga0 * gb[0] + Sga1 * gb[1] + Sga2 * gb[2];
+          Kb[j][0] += kg;
+          Kb[j][4] += kg;
+          Kb[j][8] += kg;
+        }
       }
     }
   }
 
-  // phase 3: scatter (or dense output for one element)
-  if (Kdense) {
-    if constexpr (WANT_K) {
+  // phase 3: element-by-element stores (K_e(a,b) for b >= a, else transposed into (b,a))
+  if constexpr (WANT_K) {
 #pragma unroll
-      for (int b = 0; b < 8; ++b)
+    for (int j = 0; j < NB; ++j) {
+      if (j < nb) {
+        const int b    = (a + j) & 7;
+        const bool tr  = b < a;
+        const int sb   = tr ? upperBlock(b, a) : upperBlock(a, b);
+        T* dst         = Kebe + (e * 36 + sb) * 9;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-          for (int l = 0; l < 3; ++l) Kdense[(a * 3 + k) * 24 + b * 3 + l] = double(Kab[b][k * 3 + l]);
-    }
-    if constexpr (WANT_R) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) Rdense[a * 3 + k] = double(Ra[k]);
-    }
-    return;
-  }
-  if constexpr (WANT_K) {
-    const int32_t* sl = bslot + e * 64 + a * 8;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      T* dst = K + (int64_t)sl[b] * 9;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-          if (maskDbc && (s.fix[el][a * 3 + k] | s.fix[el][b * 3 + l])) continue;
-          unsafeAtomicAdd(dst + k * 3 + l, Kab[b][k * 3 + l]);
-        }
+          for (int l = 0; l < 3; ++l) dst[tr ? l * 3 + k : k * 3 + l] = Kb[j][k * 3 + l];
+      }
     }
   }
   if constexpr (WANT_R) {
-    const int64_t na = s.node[el][a];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      if (maskDbc && s.fix[el][a * 3 + k]) continue;
-      unsafeAtomicAdd(R + na * 3 + k, double(Ra[k]));
-    }
+    for (int k = 0; k < 3; ++k) Rebe[e * 24 + a * 3 + k] = double(Ra[k]);
   }
+}
+
+// K gather: thread per 3x3 block over blocks [p0, p1): sum the listed element blocks (transposed where
+// flagged) in list order, then FULL: rows/columns of constrained DOFs are zero with a unit diagonal
+// (simpleassemblers.inl:152-169).
+template <typename T>
+__global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ kcptr, const int32_t* __restrict__ kclist,
+                                               const int32_t* __restrict__ blockRow,
+                                               const int32_t* __restrict__ colidx,
+                                               const uint8_t* __restrict__ fixed, int maskDbc,
+                                               const T* __restrict__ Kebe, int64_t p0, int64_t p1,
+                                               T* __restrict__ K) {
+  const int64_t p = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= p1) return;
+  const int32_t c0 = kcptr[p], c1 = kcptr[p + 1];
+  T acc[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) acc[i] = T(0);
+  for (int32_t c = c0; c < c1; ++c) {
+    const int32_t code = kclist[c];
+    const T* src       = Kebe + (int64_t)(code >> 1) * 9;
+    T v[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v[i] = src[i];
+    const bool tr = code & 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) acc[k * 3 + l] += tr ? v[l * 3 + k] : v[k * 3 + l];
+  }
+  if (maskDbc) {
+    const int64_t r = (int64_t)blockRow[p] * 3, c = (int64_t)colidx[p] * 3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+        if (fixed[r + k] | fixed[c + l]) acc[k * 3 + l] = (r + k == c + l) ? T(1) : T(0);
+  }
+  T* dst = K + p * 9;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dst[i] = acc[i];
+}
+
+// R gather: thread per DOF over node rows [r0, r1)
+__global__ __launch_bounds__(256) void gatherR(const int32_t* __restrict__ ncptr, const int32_t* __restrict__ nclist,
+                                               const uint8_t* __restrict__ fixed, int maskDbc,
+                                               const double* __restrict__ Rebe, int64_t r0, int64_t r1,
+                                               double* __restrict__ R) {
+  const int64_t d = r0 * 3 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (d >= r1 * 3) return;
+  if (maskDbc && fixed[d]) {
+    R[d] = 0.0;
+    return;
+  }
+  const int64_t node = d / 3;
+  const int k        = (int)(d - node * 3);
+  double acc         = 0.0;
+  for (int32_t c = ncptr[node]; c < ncptr[node + 1]; ++c) {
+    const int32_t code = nclist[c];  // e*8 + a
+    acc += Rebe[(int64_t)(code >> 3) * 24 + (code & 7) * 3 + k];
+  }
+  R[d] = acc;
 }
 
 // Traction faces (traction.hh:106-138): 1-point rule at the face centre, R_a -= N_a t |J_face|.
@@ -388,20 +435,24 @@ __global__ void pointLoadKernel(const int64_t* __restrict__ dof, const double* _
 }
 
 template <typename T>
-__global__ void dbcDiagKernel(const int32_t* __restrict__ fixedList, int64_t n, const int32_t* __restrict__ diag,
-                              T* __restrict__ K) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int32_t dof = fixedList[i];
-  const int32_t node = dof / 3, k = dof % 3;
-  K[(int64_t)diag[node] * 9 + k * 4] = T(1);
-}
-
-template <typename T>
 __global__ void fillKernel(T* __restrict__ p, int64_t n, T v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+// dense 24x24 / 24 output of one element from its staged blocks (calculateMatrix/calculateVector)
+__global__ void unpackElement(const double* __restrict__ Kebe, const double* __restrict__ Rebe, int64_t e,
+                              double* __restrict__ Kd, double* __restrict__ Rd) {
+  const int t = threadIdx.x;  // 576 threads
+  if (t < 576) {
+    const int r = t / 24, c = t % 24;
+    const int a = r / 3, k = r % 3, b = c / 3, l = c % 3;
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const double* blk = Kebe + (e * 36 + upperBlock(lo, hi)) * 9;
+    Kd[t] = a <= b ? blk[k * 3 + l] : blk[l * 3 + k];
+  }
+  if (t < 24) Rd[t] = Rebe[e * 24 + t];
 }
 
 inline unsigned gridFor(int64_t n, int bs) {
@@ -410,18 +461,18 @@ inline unsigned gridFor(int64_t n, int bs) {
 }
 
 template <typename T, int KIND, bool WK, bool WR>
-void launchElem(ik_handle* h, int64_t e0, int64_t count, int mask, T* K, double* R, double* Kd, double* Rd) {
+void launchElem(ik_handle* h, int64_t e0, int64_t count, T* Kebe, double* Rebe) {
   const unsigned grid = (unsigned)((count + EPB - 1) / EPB);
-  hipLaunchKernelGGL((hex8Assemble<T, KIND, WK, WR>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p, h->conn.p,
-                     h->bslot.p, h->fixed.p, e0, count, mask, h->lam, h->mu, K, R, h->errFlag.p, Kd, Rd);
+  hipLaunchKernelGGL((hex8Element<T, KIND, WK, WR>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p, h->conn.p,
+                     e0, count, h->lam, h->mu, Kebe, Rebe, h->errFlag.p);
 }
 
 template <typename T, bool WK, bool WR>
-void launchElemMat(ik_handle* h, int64_t e0, int64_t count, int mask, T* K, double* R, double* Kd, double* Rd) {
+void launchElemMat(ik_handle* h, int64_t e0, int64_t count, T* Kebe, double* Rebe) {
   if (h->material == IK_SVK)
-    launchElem<T, IK_SVK, WK, WR>(h, e0, count, mask, K, R, Kd, Rd);
+    launchElem<T, IK_SVK, WK, WR>(h, e0, count, Kebe, Rebe);
   else
-    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, mask, K, R, Kd, Rd);
+    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, Kebe, Rebe);
 }
 
 }  // namespace
@@ -446,35 +497,63 @@ static void checkErrFlag(ik_handle* h) {
   }
 }
 
+// Element chunks on the main stream; each chunk's gathers on the second stream, released by an event
+// recorded behind the chunk, so gathers (memory-bound) overlap the next chunks (compute-bound).
+template <typename T>
+static void runChunks(ik_handle* h, int dbc, bool wantK, bool wantR, T* Kebe, T* K) {
+  const int mask = dbc == IK_DBC_RAW ? 0 : 1;
+  const size_t nc = h->chunkElem.size() - 1;
+  while (h->chunkEvents.size() < nc) {
+    hipEvent_t ev;
+    IK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    h->chunkEvents.push_back(ev);
+  }
+  for (size_t c = 0; c < nc; ++c) {
+    const int64_t e0 = h->chunkElem[c], e1 = h->chunkElem[c + 1];
+    {
+      ScopedKernelTimer t(h, wantK ? KASM_KR : KASM_R, h->stream);
+      if (wantK && wantR)
+        launchElemMat<T, true, true>(h, e0, e1 - e0, Kebe, h->ebeR.p);
+      else if (wantK)
+        launchElemMat<T, true, false>(h, e0, e1 - e0, Kebe, nullptr);
+      else
+        launchElemMat<double, false, true>(h, e0, e1 - e0, (double*)nullptr, h->ebeR.p);
+    }
+    const int64_t r0 = h->chunkRow[c], r1 = h->chunkRow[c + 1];
+    if (r1 <= r0) continue;
+    IK_HIP(hipEventRecord(h->chunkEvents[c], h->stream));
+    IK_HIP(hipStreamWaitEvent(h->stream2, h->chunkEvents[c], 0));
+    if (wantK) {
+      const int64_t p0 = h->hrowptr[r0], p1 = h->hrowptr[r1];
+      ScopedKernelTimer t(h, KGATHER, h->stream2);
+      hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, h->stream2, h->kcptr.p,
+                         h->kclist.p, h->blockRow.p, h->colidx.p, h->fixed.p, mask, Kebe, p0, p1, K);
+    }
+    if (wantR)
+      hipLaunchKernelGGL(gatherR, dim3((unsigned)(((r1 - r0) * 3 + 255) / 256)), dim3(256), 0, h->stream2,
+                         h->ncptr.p, h->nclist.p, h->fixed.p, mask, h->ebeR.p, r0, r1, h->R.p);
+  }
+  // join: later work on the main stream sees every gathered row
+  IK_HIP(hipEventRecord(h->joinEvent, h->stream2));
+  IK_HIP(hipStreamWaitEvent(h->stream, h->joinEvent, 0));
+}
+
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
   require(h->neas == 0, IK_ENOTIMPL, "EAS assembly is handled by the EAS kernels");
   const int mask = dbc == IK_DBC_RAW ? 0 : 1;
-  if (wantK) {
-    if (precision == IK_FP64) {
-      h->K64.ensure((size_t)h->nnzb * 9);
-      launchZero(h->K64.p, h->nnzb * 9, h->stream);
-    } else {
-      h->K32.ensure((size_t)h->nnzb * 9);
-      launchZeroF(h->K32.p, h->nnzb * 9, h->stream);
-    }
-  }
-  if (wantR) launchZero(h->R.p, h->ndof, h->stream);
+  if (wantR) h->ebeR.ensure((size_t)h->ne * 24);
+  // the mixed form pairs an FP32 tangent with an FP64 residual: two passes
+  const bool fused = wantK && wantR && precision == IK_FP64;
   if (wantK && precision == IK_FP64) {
-    ScopedKernelTimer t(h, wantR ? KASM_KR : KASM_KR);
-    if (wantR)
-      launchElemMat<double, true, true>(h, 0, h->ne, mask, h->K64.p, h->R.p, nullptr, nullptr);
-    else
-      launchElemMat<double, true, false>(h, 0, h->ne, mask, h->K64.p, nullptr, nullptr, nullptr);
-  } else {
-    if (wantK) {
-      ScopedKernelTimer t(h, KASM_KR);
-      launchElemMat<float, true, false>(h, 0, h->ne, mask, h->K32.p, nullptr, nullptr, nullptr);
-    }
-    if (wantR) {
-      ScopedKernelTimer t(h, KASM_R);
-      launchElemMat<double, false, true>(h, 0, h->ne, mask, (double*)nullptr, h->R.p, nullptr, nullptr);
-    }
+    h->K64.ensure((size_t)h->nnzb * 9);
+    h->ebeK64.ensure((size_t)h->ne * 324);
+    runChunks<double>(h, dbc, true, fused, h->ebeK64.p, h->K64.p);
+  } else if (wantK) {
+    h->K32.ensure((size_t)h->nnzb * 9);
+    h->ebeK32.ensure((size_t)h->ne * 324);
+    runChunks<float>(h, dbc, true, false, h->ebeK32.p, h->K32.p);
   }
+  if (wantR && !fused) runChunks<double>(h, dbc, false, true, (double*)nullptr, (double*)nullptr);
   if (wantR && h->nfaces) {
     hipLaunchKernelGGL(tractionKernel, dim3(gridFor(h->nfaces, 128)), dim3(128), 0, h->stream, h->X.p, h->conn.p,
                        h->faceElem.p, h->faceAxis.p, h->faceSide.p, h->nfaces, h->trac[0] * h->lambda,
@@ -484,27 +563,19 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
     hipLaunchKernelGGL(pointLoadKernel, dim3(gridFor(h->npl, 128)), dim3(128), 0, h->stream, h->plDof.p,
                        h->plCoef.p, h->npl, h->lambda, h->R.p);
   }
-  if (wantK && dbc == IK_DBC_FULL) launchDbcDiag(h, precision);
   checkErrFlag(h);
-}
-
-void launchDbcDiag(ik_handle* h, int precision) {
-  if (!h->nfixed) return;
-  if (precision == IK_FP64)
-    hipLaunchKernelGGL(dbcDiagKernel<double>, dim3(gridFor(h->nfixed, 256)), dim3(256), 0, h->stream,
-                       h->fixedList.p, h->nfixed, h->diagBlk.p, h->K64.p);
-  else
-    hipLaunchKernelGGL(dbcDiagKernel<float>, dim3(gridFor(h->nfixed, 256)), dim3(256), 0, h->stream,
-                       h->fixedList.p, h->nfixed, h->diagBlk.p, h->K32.p);
 }
 
 void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re) {
   require(e >= 0 && e < h->ne, IK_EINVAL, "element index out of range");
+  h->ebeK64.ensure((size_t)h->ne * 324);
+  h->ebeR.ensure((size_t)h->ne * 24);
+  launchElemMat<double, true, true>(h, e, 1, h->ebeK64.p, h->ebeR.p);
+  checkErrFlag(h);
   DevBuf<double> kd, rd;
   kd.ensure(576);
   rd.ensure(24);
-  launchElemMat<double, true, true>(h, e, 1, 0, (double*)nullptr, nullptr, kd.p, rd.p);
-  checkErrFlag(h);
+  hipLaunchKernelGGL(unpackElement, dim3(1), dim3(576), 0, h->stream, h->ebeK64.p, h->ebeR.p, e, kd.p, rd.p);
   if (Ke) IK_HIP(hipMemcpyAsync(Ke, kd.p, 576 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   if (Re) IK_HIP(hipMemcpyAsync(Re, rd.p, 24 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   IK_HIP(hipStreamSynchronize(h->stream));

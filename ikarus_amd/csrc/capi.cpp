@@ -156,6 +156,8 @@ int ik_create(ik_handle** out, const ik_problem_desc* d) {
     require(d->material == IK_NEOHOOKE || d->material == IK_SVK, IK_EINVAL, "unknown material");
     require(d->num_nodes * 3 < (int64_t)INT32_MAX, IK_EINVAL, "too many DOFs for int32 indices");
     IK_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    IK_HIP(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    IK_HIP(hipEventCreateWithFlags(&h->joinEvent, hipEventDisableTiming));
     IK_HIP(hipEventCreate(&h->ev0));
     IK_HIP(hipEventCreate(&h->ev1));
     h->nn       = d->num_nodes;
@@ -229,9 +231,13 @@ void ik_destroy(ik_handle* h) {
   if (h->hred) (void)hipHostFree(h->hred);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
-  hipStream_t s = h->stream;
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+  for (auto ev : h->chunkEvents) (void)hipEventDestroy(ev);
+  if (h->joinEvent) (void)hipEventDestroy(h->joinEvent);
+  hipStream_t s = h->stream, s2 = h->stream2;
   delete h;
   if (s) (void)hipStreamDestroy(s);
+  if (s2) (void)hipStreamDestroy(s2);
 }
 
 const char* ik_last_error(const ik_handle* h) { return h ? h->err.c_str() : "null handle"; }
@@ -260,8 +266,8 @@ int ik_vector(ik_handle* h, int dbc, double* R, int where) {
   return guard(h, [&] {
     requireBound(h);
     require(dbc >= 0 && dbc <= 2, IK_EINVAL, "bad DBC option");
-    require(R != nullptr, IK_EINVAL, "null output");
     assembleR(h, dbc == IK_DBC_FULL ? IK_DBC_FULL : IK_DBC_RAW);
+    if (!R) return;  // assemble into the handle's residual only
     if (dbc != IK_DBC_REDUCED) {
       copyOut(h, R, h->R.p, h->ndof, where);
       return;

@@ -65,7 +65,7 @@ struct PcgScalars {
   double bb;        // ||b||^2
 };
 
-enum KernelId { KASM_KR = 0, KASM_R = 1, KSPMV = 2, KMLP = 3, KCOUNT = 4 };
+enum KernelId { KASM_KR = 0, KASM_R = 1, KSPMV = 2, KMLP = 3, KGATHER = 4, KCOUNT = 5 };
 
 // Reduction partials: one double per block, final fold in fixed order (deterministic).
 constexpr int kReduceBlocks = 1024;
@@ -75,7 +75,10 @@ constexpr int kReduceBlocks = 1024;
 struct ik_handle {
   std::string err;
   int64_t bad_elem = -1;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // all work is ordered on this stream
+  hipStream_t stream2 = nullptr;  // assembly gathers, joined back into `stream` by events
+  std::vector<hipEvent_t> chunkEvents;
+  hipEvent_t joinEvent = nullptr;
 
   // problem
   int64_t nn = 0, ne = 0, ndof = 0;
@@ -101,12 +104,20 @@ struct ik_handle {
   std::vector<int64_t> hplDof;
   std::vector<double> hplCoef;
 
-  // 3x3 block CSR pattern (block row = node) and element block-slot maps
+  // 3x3 block CSR pattern (block row = node)
   bool patternBuilt = false;
   std::vector<int32_t> hrowptr, hcolidx, hdiag;
   int64_t nnzb = 0;
-  ik::DevBuf<int32_t> rowptr, colidx, diagBlk, bslot;
-  // element coloring-free scatter uses atomics; K storage
+  ik::DevBuf<int32_t> rowptr, colidx, diagBlk, blockRow;
+  // gather lists: for every block, the element blocks that sum into it (code = ((e*36+sb)<<1)|transposed),
+  // ordered by element; for every node, its (element, corner) incidences (code = e*8+a)
+  ik::DevBuf<int32_t> kcptr, kclist, ncptr, nclist;
+  // element chunks and the block rows that are complete after each chunk
+  std::vector<int64_t> chunkElem, chunkRow;
+  // element-by-element staging of K_e (36 upper 3x3 blocks) and R_e (24)
+  ik::DevBuf<double> ebeK64, ebeR;
+  ik::DevBuf<float> ebeK32;
+  // K storage
   ik::DevBuf<double> K64;
   ik::DevBuf<float> K32;
   int kPrecision = -1, kDbc = -1;
@@ -138,8 +149,8 @@ struct ik_handle {
 
   // timing
   bool timing = false;
-  int64_t tLaunches[ik::KCOUNT] = {0, 0, 0, 0};
-  double tMs[ik::KCOUNT] = {0, 0, 0, 0};
+  int64_t tLaunches[ik::KCOUNT] = {};
+  double tMs[ik::KCOUNT] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -150,7 +161,6 @@ void buildPattern(ik_handle* h);
 void launchZero(double* p, int64_t n, hipStream_t s);
 void launchZeroF(float* p, int64_t n, hipStream_t s);
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR);
-void launchDbcDiag(ik_handle* h, int precision);
 void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re);
 void easUpdate(ik_handle* h, const double* duDev);
 // sparse.hip
@@ -169,12 +179,13 @@ void mlpPredict(ik_handle* h, const double* uPrev, const double* uPrev2, double 
 struct ScopedKernelTimer {
   ik_handle* h;
   int id;
-  ScopedKernelTimer(ik_handle* hh, int kid) : h(hh), id(kid) {
-    if (h->timing) IK_HIP(hipEventRecord(h->ev0, h->stream));
+  hipStream_t s;
+  ScopedKernelTimer(ik_handle* hh, int kid, hipStream_t ss = nullptr) : h(hh), id(kid), s(ss ? ss : hh->stream) {
+    if (h->timing) IK_HIP(hipEventRecord(h->ev0, s));
   }
   ~ScopedKernelTimer() {
     if (!h->timing) return;
-    (void)hipEventRecord(h->ev1, h->stream);
+    (void)hipEventRecord(h->ev1, s);
     (void)hipEventSynchronize(h->ev1);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);

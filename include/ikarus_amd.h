@@ -86,7 +86,8 @@ int ik_set_requirement(ik_handle* h, const double* u, double lambda, int where);
 int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where);
 
 /* VectorAssembler::vector(req, forces, dbc) (interface.hh:355-365; simpleassemblers.inl:60-118).
- * R has size() entries (RAW/FULL) or reducedSize() entries (REDUCED). */
+ * R has size() entries (RAW/FULL) or reducedSize() entries (REDUCED); R = NULL assembles into the
+ * handle's device residual without copying it out. */
 int ik_vector(ik_handle* h, int dbc, double* R, int where);
 
 /* MatrixAssembler::matrix(req, stiffness, dbc) (interface.hh:430-440; simpleassemblers.inl:121-204).
@@ -183,7 +184,7 @@ int ik_mlp_set_load(ik_handle* h, const double traction[3], double char_length);
 
 /* ---- timing of the hot kernels on the handle's stream (HIP events) ----------------------------- */
 enum { IK_KERNEL_ASSEMBLE_KR = 0, IK_KERNEL_ASSEMBLE_R = 1, IK_KERNEL_SPMV = 2, IK_KERNEL_MLP = 3,
-       IK_KERNEL_COUNT = 4 };
+       IK_KERNEL_GATHER_K = 4, IK_KERNEL_COUNT = 5 };
 int ik_timing_enable(ik_handle* h, int on);
 int ik_timing_read(ik_handle* h, int kernel, int64_t* launches, double* total_ms);
 int ik_timing_reset(ik_handle* h);

@@ -544,32 +544,44 @@ void elementCompute(const Problem& P, int e, const double* uFull, double lambda,
     if (wantK || neas) {
       double Sm[D][D];
       stressMatrix<D>(Sv, Sm);
-      for (int a = 0; a < nn; ++a) {
-        for (int b = 0; b < nn; ++b) {
-          if (!wantK) break;
-          double Cloc[ns * ns];
-          const double* Cuse = Cv;
-          if (P.faithfulCost) {  // reference cost structure: tangent per pair (nonlinearelastic.hh:318)
-            double Stmp[ns];
-            materialD<D>(P.mat, Ev, Stmp, Cloc, nullptr);
-            Cuse = Cloc;
-          }
-          // material part B_a^T C B_b (nonlinearelastic.hh:313-322)
-          for (int k = 0; k < D; ++k)
+      if (wantK) {
+        // C*B_b once per Gauss point (or per node pair in the reference-cost mode, nonlinearelastic.hh:318)
+        double CB[nn][ns * D];
+        for (int b = 0; b < nn; ++b)
+          for (int v = 0; v < ns; ++v)
             for (int l = 0; l < D; ++l) {
-              double s = 0.0;
-              for (int v = 0; v < ns; ++v) {
-                double cb = 0.0;
-                for (int w = 0; w < ns; ++w) cb += Cuse[v * ns + w] * B[b][w * D + l];
-                s += B[a][v * D + k] * cb;
-              }
-              out.K[(a * D + k) * nd + (b * D + l)] += s * dV;
+              double cb = 0.0;
+              for (int w = 0; w < ns; ++w) cb += Cv[v * ns + w] * B[b][w * D + l];
+              CB[b][v * D + l] = cb;
             }
-          // geometric part g_a^T S g_b delta_kl (nonlinearelastic.hh:294-302)
-          double kg = 0.0;
-          for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) kg += gp.g[a][i] * Sm[i][j] * gp.g[b][j];
-          for (int k = 0; k < D; ++k) out.K[(a * D + k) * nd + (b * D + k)] += kg * dV;
+        for (int a = 0; a < nn; ++a) {
+          for (int b = 0; b < nn; ++b) {
+            const double* CBb = CB[b];
+            double CBloc[ns * D];
+            if (P.faithfulCost) {
+              double Cloc[ns * ns], Stmp[ns];
+              materialD<D>(P.mat, Ev, Stmp, Cloc, nullptr);
+              for (int v = 0; v < ns; ++v)
+                for (int l = 0; l < D; ++l) {
+                  double cb = 0.0;
+                  for (int w = 0; w < ns; ++w) cb += Cloc[v * ns + w] * B[b][w * D + l];
+                  CBloc[v * D + l] = cb;
+                }
+              CBb = CBloc;
+            }
+            // material part B_a^T C B_b (nonlinearelastic.hh:313-322)
+            for (int k = 0; k < D; ++k)
+              for (int l = 0; l < D; ++l) {
+                double s = 0.0;
+                for (int v = 0; v < ns; ++v) s += B[a][v * D + k] * CBb[v * D + l];
+                out.K[(a * D + k) * nd + (b * D + l)] += s * dV;
+              }
+            // geometric part g_a^T S g_b delta_kl (nonlinearelastic.hh:294-302)
+            double kg = 0.0;
+            for (int i = 0; i < D; ++i)
+              for (int j = 0; j < D; ++j) kg += gp.g[a][i] * Sm[i][j] * gp.g[b][j];
+            for (int k = 0; k < D; ++k) out.K[(a * D + k) * nd + (b * D + k)] += kg * dV;
+          }
         }
       }
       if (neas) {
