@@ -512,7 +512,9 @@ static void runChunks(ik_handle* h, int dbc, bool wantK, bool wantR, T* Kebe, T*
     const int64_t e0 = h->chunkElem[c], e1 = h->chunkElem[c + 1];
     {
       ScopedKernelTimer t(h, wantK ? KASM_KR : KASM_R, h->stream);
-      if (wantK && wantR)
+      if (h->neas)
+        easStage(h, e0, e1 - e0, wantK);
+      else if (wantK && wantR)
         launchElemMat<T, true, true>(h, e0, e1 - e0, Kebe, h->ebeR.p);
       else if (wantK)
         launchElemMat<T, true, false>(h, e0, e1 - e0, Kebe, nullptr);
@@ -539,9 +541,13 @@ static void runChunks(ik_handle* h, int dbc, bool wantK, bool wantR, T* Kebe, T*
 }
 
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
-  require(h->neas == 0, IK_ENOTIMPL, "EAS assembly is handled by the EAS kernels");
   const int mask = dbc == IK_DBC_RAW ? 0 : 1;
-  if (wantR) h->ebeR.ensure((size_t)h->ne * 24);
+  h->ebeR.ensure((size_t)h->ne * 24);  // EAS elements always produce R_e
+  if (h->neas) {
+    require(precision == IK_FP64, IK_ENOTIMPL, "the EAS element is FP64 only");
+    // the EAS kernel computes the condensed R_e with K_e; both gathers run when K is wanted
+    wantR = true;
+  }
   // the mixed form pairs an FP32 tangent with an FP64 residual: two passes
   const bool fused = wantK && wantR && precision == IK_FP64;
   if (wantK && precision == IK_FP64) {
@@ -554,6 +560,7 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
     runChunks<float>(h, dbc, true, false, h->ebeK32.p, h->K32.p);
   }
   if (wantR && !fused) runChunks<double>(h, dbc, false, true, (double*)nullptr, (double*)nullptr);
+  if (h->neas) h->easCacheValid = true;
   if (wantR && h->nfaces) {
     hipLaunchKernelGGL(tractionKernel, dim3(gridFor(h->nfaces, 128)), dim3(128), 0, h->stream, h->X.p, h->conn.p,
                        h->faceElem.p, h->faceAxis.p, h->faceSide.p, h->nfaces, h->trac[0] * h->lambda,
@@ -570,7 +577,10 @@ void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re) {
   require(e >= 0 && e < h->ne, IK_EINVAL, "element index out of range");
   h->ebeK64.ensure((size_t)h->ne * 324);
   h->ebeR.ensure((size_t)h->ne * 24);
-  launchElemMat<double, true, true>(h, e, 1, h->ebeK64.p, h->ebeR.p);
+  if (h->neas)
+    easStage(h, e, 1, true);
+  else
+    launchElemMat<double, true, true>(h, e, 1, h->ebeK64.p, h->ebeR.p);
   checkErrFlag(h);
   DevBuf<double> kd, rd;
   kd.ensure(576);

@@ -132,7 +132,8 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     info->correction_norm = std::sqrt(ik::dot(h, h->du.p, h->du.p, n));
     if (h->neas) ik::easUpdate(h, h->du.p);  // CORRECTION_UPDATED (newtonraphson.hh:230)
     ik::axpy(h, 1.0, h->du.p, h->u.p, n);
-    h->kValid = false;
+    h->kValid        = false;
+    h->easCacheValid = false;
     assembleR(h, IK_DBC_FULL);
     rNorm               = normR(h);
     info->residual_norm = rNorm;
@@ -248,9 +249,10 @@ int64_t ik_reduced_size(const ik_handle* h) { return h ? h->ndof - h->nfixed : -
 int ik_set_requirement(ik_handle* h, const double* u, double lambda, int where) {
   return guard(h, [&] {
     if (u) copyIn(h, h->u.p, u, h->ndof, where);
-    h->lambda = lambda;
-    h->bound  = true;
-    h->kValid = false;
+    h->lambda        = lambda;
+    h->bound         = true;
+    h->kValid        = false;
+    h->easCacheValid = false;
     IK_HIP(hipStreamSynchronize(h->stream));
   });
 }
@@ -395,6 +397,7 @@ int ik_update_state(ik_handle* h, const double* du, int where) {
     if (!h->neas) return;
     h->du.ensure(h->ndof);
     copyIn(h, h->du.p, du, h->ndof, where);
+    if (!h->easCacheValid) assembleR(h, IK_DBC_RAW);  // D, L, Rt at the bound state
     ik::easUpdate(h, h->du.p);
     IK_HIP(hipStreamSynchronize(h->stream));
   });
@@ -411,6 +414,8 @@ int ik_set_alpha(ik_handle* h, const double* alpha, int where) {
   return guard(h, [&] {
     require(h->neas > 0, IK_EINVAL, "no EAS modes");
     copyIn(h, h->alpha.p, alpha, h->ne * h->neas, where);
+    h->easCacheValid = false;
+    h->kValid        = false;
     IK_HIP(hipStreamSynchronize(h->stream));
   });
 }

@@ -128,8 +128,9 @@ struct ik_handle {
   double lambda = 0;
   ik::DevBuf<double> u, R, du, rhs, tmp;
 
-  // EAS
-  ik::DevBuf<double> alpha;
+  // EAS: internal variables and, per element, Y = D^{-1} L (21x24) and y = D^{-1} Rt of the last staged state
+  ik::DevBuf<double> alpha, easCache;
+  bool easCacheValid = false;
 
   // error flag (lowest element index with det C <= 0)
   ik::DevBuf<unsigned long long> errFlag;
@@ -163,6 +164,7 @@ void launchZeroF(float* p, int64_t n, hipStream_t s);
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR);
 void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re);
 void easUpdate(ik_handle* h, const double* duDev);
+void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK);
 // sparse.hip
 void spmv64(ik_handle* h, const double* x, double* y);
 void spmv32(ik_handle* h, const float* x, float* y);

@@ -15,8 +15,7 @@ namespace ik {
 namespace {
 
 constexpr int SPMV_BS   = 256;            // threads per block
-constexpr int GROUP     = 8;              // lanes per block row
-constexpr int ROWS_PER_BLOCK = SPMV_BS / GROUP;
+constexpr int ROWS_PER_BLOCK = (SPMV_BS / 64) * 7;  // 7 block rows per wavefront
 
 __device__ inline double waveSum(double v) {
 #pragma unroll
@@ -40,8 +39,11 @@ __device__ inline double blockSum(double v) {
   return s;
 }
 
-// y = K x over block rows; optional fused partial dot d = sum_i x_i y_i per block.
-// Group of 8 lanes walks the contiguous value run of one block row with stride 8 (coalesced).
+// y = K x over 3x3 block rows; optional fused partial dot d = sum_i x_i y_i per block.
+// Lane (r, m) of a wavefront: row r = lane / 9 (7 rows per wavefront, lane 63 idle), entry m = 3k + l of
+// every 3x3 block in the row. Each load instruction reads 7 contiguous 72-byte block runs; the three
+// partial sums of row k are combined with two lane shifts.
+constexpr int ROWS_PER_WAVE = 7;
 template <typename TV, typename TX, bool DOT>
 __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict__ rowptr,
                                                       const int32_t* __restrict__ colidx,
@@ -50,35 +52,33 @@ __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict_
                                                       double* __restrict__ partial,
                                                       const PcgScalars* __restrict__ sc) {
   if (sc && sc->done) return;
-  const int g        = threadIdx.x & (GROUP - 1);
-  const int64_t row  = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / GROUP);
-  TX a0 = 0, a1 = 0, a2 = 0;
-  if (row < nrows) {
-    const int64_t rb = rowptr[row], re = rowptr[row + 1];
-    const int64_t L  = (re - rb) * 9;
-    const TV* V      = val + rb * 9;
-    for (int64_t idx = g; idx < L; idx += GROUP) {
-      const int64_t blk = idx / 9;
-      const int ent     = (int)(idx - blk * 9);
-      const int k = ent / 3, l = ent - 3 * (ent / 3);
-      const TX prod     = (TX)V[idx] * x[(int64_t)colidx[rb + blk] * 3 + l];
-      a0 += (k == 0) ? prod : TX(0);
-      a1 += (k == 1) ? prod : TX(0);
-      a2 += (k == 2) ? prod : TX(0);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rl   = lane / 9;
+  const int m    = lane - rl * 9;
+  const int k = m / 3, l = m - 3 * (m / 3);
+  const int64_t row = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * ROWS_PER_WAVE + rl;
+  const bool active = rl < ROWS_PER_WAVE && row < nrows;
+  TX acc = TX(0);
+  if (active) {
+    const int32_t rb = rowptr[row], re = rowptr[row + 1];
+    const TV* V       = val + (int64_t)rb * 9 + m;
+    const int32_t* C  = colidx + rb;
+    const int n       = re - rb;
+    int j             = 0;
+    for (; j + 2 <= n; j += 2) {
+      const int32_t c0 = C[j], c1 = C[j + 1];
+      const TV v0 = V[(int64_t)j * 9], v1 = V[(int64_t)(j + 1) * 9];
+      acc += (TX)v0 * x[(int64_t)c0 * 3 + l];
+      acc += (TX)v1 * x[(int64_t)c1 * 3 + l];
     }
+    if (j < n) acc += (TX)V[(int64_t)j * 9] * x[(int64_t)C[j] * 3 + l];
   }
-#pragma unroll
-  for (int o = GROUP / 2; o > 0; o >>= 1) {
-    a0 += __shfl_xor(a0, o, GROUP);
-    a1 += __shfl_xor(a1, o, GROUP);
-    a2 += __shfl_xor(a2, o, GROUP);
-  }
+  const TX a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
   double d = 0.0;
-  if (row < nrows && g == 0) {
-    y[row * 3 + 0] = a0;
-    y[row * 3 + 1] = a1;
-    y[row * 3 + 2] = a2;
-    if (DOT) d = (double)x[row * 3] * a0 + (double)x[row * 3 + 1] * a1 + (double)x[row * 3 + 2] * a2;
+  if (active && l == 0) {
+    const TX yk = acc + a1 + a2;
+    y[row * 3 + k] = yk;
+    if (DOT) d = (double)x[row * 3 + k] * (double)yk;
   }
   if (DOT) {
     const double s = blockSum<SPMV_BS>(d);
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict_
   }
 }
 
-// r = b - K x with FP64 accumulation (K may be FP32); used by iterative refinement
+// r = b - K x with FP64 accumulation (K may be FP32); used by iterative refinement. Same lane mapping.
 template <typename TV>
 __global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restrict__ rowptr,
                                                           const int32_t* __restrict__ colidx,
@@ -94,36 +94,26 @@ __global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restr
                                                           const double* __restrict__ x,
                                                           const double* __restrict__ b, double* __restrict__ r,
                                                           int64_t nrows, double* __restrict__ partial) {
-  const int g       = threadIdx.x & (GROUP - 1);
-  const int64_t row = (int64_t)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / GROUP);
-  double a0 = 0, a1 = 0, a2 = 0;
-  if (row < nrows) {
-    const int64_t rb = rowptr[row], re = rowptr[row + 1];
-    const int64_t L  = (re - rb) * 9;
-    const TV* V      = val + rb * 9;
-    for (int64_t idx = g; idx < L; idx += GROUP) {
-      const int64_t blk = idx / 9;
-      const int ent     = (int)(idx - blk * 9);
-      const int k = ent / 3, l = ent - 3 * (ent / 3);
-      const double prod = (double)V[idx] * x[(int64_t)colidx[rb + blk] * 3 + l];
-      a0 += (k == 0) ? prod : 0.0;
-      a1 += (k == 1) ? prod : 0.0;
-      a2 += (k == 2) ? prod : 0.0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rl   = lane / 9;
+  const int m    = lane - rl * 9;
+  const int k = m / 3, l = m - 3 * (m / 3);
+  const int64_t row = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * ROWS_PER_WAVE + rl;
+  const bool active = rl < ROWS_PER_WAVE && row < nrows;
+  double acc = 0.0;
+  if (active) {
+    const int32_t rb = rowptr[row], re = rowptr[row + 1];
+    const TV* V       = val + (int64_t)rb * 9 + m;
+    for (int32_t j = rb; j < re; ++j) {
+      acc += (double)V[(int64_t)(j - rb) * 9] * x[(int64_t)colidx[j] * 3 + l];
     }
   }
-#pragma unroll
-  for (int o = GROUP / 2; o > 0; o >>= 1) {
-    a0 += __shfl_xor(a0, o, GROUP);
-    a1 += __shfl_xor(a1, o, GROUP);
-    a2 += __shfl_xor(a2, o, GROUP);
-  }
+  const double a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
   double d = 0.0;
-  if (row < nrows && g == 0) {
-    const double r0 = b[row * 3] - a0, r1 = b[row * 3 + 1] - a1, r2 = b[row * 3 + 2] - a2;
-    r[row * 3]     = r0;
-    r[row * 3 + 1] = r1;
-    r[row * 3 + 2] = r2;
-    d              = r0 * r0 + r1 * r1 + r2 * r2;
+  if (active && l == 0) {
+    const double rk = b[row * 3 + k] - (acc + a1 + a2);
+    r[row * 3 + k]  = rk;
+    d               = rk * rk;
   }
   const double s = blockSum<SPMV_BS>(d);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;

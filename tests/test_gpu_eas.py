@@ -1,0 +1,97 @@
+"""EAS-21 (enhanced Green-Lagrange) hex8 on the GPU against the oracle and the reference's golden values."""
+import numpy as np
+import pytest
+
+import ikarus_amd as ia
+import pyoracle as po
+from ikarus_amd import _capi as capi
+from ikarus_amd import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_for(p):
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, p.eas, 2)
+    o.set_fixed(p.fixed)
+    if len(p.face_elem):
+        o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction)
+    for d, c in zip(p.point_dof, p.point_coef):
+        o.add_point_load(int(d), float(c))
+    return o
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_eas_element_matrix_and_vector(material):
+    """Condensed K_e, R_e of EnhancedAssumedStrains (enhancedassumedstrains.hh:258-348) at alpha != 0."""
+    p = pb.hex_block((3, 3, 2), material=material, jitter=0.08, seed=13, eas=21)
+    print(f"case: eas element material={material}", flush=True)
+    dev = ia.Device(p)
+    o = _oracle_for(p)
+    rng = np.random.default_rng(2)
+    u = 0.02 * rng.standard_normal(p.ndof)
+    # a large enhanced strain makes det C <= 0 in element 7: both sides must refuse it
+    alpha = 1e-4 * rng.standard_normal((p.ne, 21))
+    o.set_alpha(alpha)
+    dev.set_alpha(alpha)
+    dev.set_requirement(u, 0.0)
+    if material == capi.NEOHOOKE:  # SVK has no det C check (svk.hh), Neo-Hooke aborts (neohooke.hh:90-93)
+        with pytest.raises(RuntimeError):
+            o.element(7, u, 0.0)
+        with pytest.raises(capi.IkError) as ei:
+            dev.element(7)
+        assert ei.value.code == capi.IK_ENEGDET and ei.value.bad_element == 7
+    alpha *= 1e-2
+    o.set_alpha(alpha)
+    dev.set_alpha(alpha)
+    dev.set_requirement(u, 0.0)
+    for e in (0, 7, 17):
+        K, R = dev.element(e)
+        Ko, Ro, *_ = o.element(e, u, 0.0)
+        assert _rel(K, Ko) < 1e-10, e
+        assert _rel(R, Ro) < 1e-10, e
+        assert np.array_equal(K, K.T)  # mirrored from the upper triangle
+
+
+def test_eas_assembly_and_alpha_update():
+    p = pb.hex_block((4, 3, 3), jitter=0.05, seed=17, eas=21)
+    print("case: eas assembly + update", flush=True)
+    dev = ia.Device(p)
+    o = _oracle_for(p)
+    rng = np.random.default_rng(3)
+    u = 0.01 * rng.standard_normal(p.ndof) * (1 - p.fixed)
+    alpha = 1e-6 * rng.standard_normal((p.ne, 21))
+    o.set_alpha(alpha)
+    dev.set_alpha(alpha)
+    dev.set_requirement(u, 0.8)
+    dev.matrix(capi.DBC_FULL)
+    outer, inner, vals = dev.csc()
+    oo, io = o.pattern()
+    assert np.array_equal(outer, oo) and np.array_equal(inner, io)
+    assert _rel(vals, o.matrix(u, 0.8, po.FULL)) < 1e-10
+    assert _rel(dev.vector(), o.vector(u, 0.8, po.FULL)) < 1e-10
+    du = 1e-3 * rng.standard_normal(p.ndof) * (1 - p.fixed)
+    dev.update_state(du)
+    o.update_state(u, 0.8, du)
+    assert _rel(dev.alpha(), o.alpha()) < 1e-10
+
+
+# tests/src/testcantileverbeamEAS.cpp:61-67 with tests/src/testcantileverbeam.hh:83-198
+@pytest.mark.parametrize("material,maxu", [(capi.NEOHOOKE, 4.781820664768682), (capi.SVK, 4.7692391315649365)])
+def test_cantilever_eas21_golden_on_gpu(material, maxu):
+    p = pb.cantilever3d(material=material)
+    print(f"case: cantilever golden material={material}", flush=True)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.LCConfig()
+    cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 20, 0.0, 1.0, capi.PRED_CONSTANT
+    s = ia.nr_settings(tol=1e-10, linear=ia.solver_opts(rtol=1e-15, max_iter=4000))
+    info, iters, res = dev.load_control(cfg, s)
+    u, lam = dev.get_requirement()
+    assert info.success
+    assert info.total_iterations == 80
+    assert abs(np.max(np.abs(u)) - maxu) <= 1e-10 * maxu
+    assert abs(lam - 1.0) < 1e-12
