@@ -37,11 +37,14 @@ __global__ void gatherKernel(const double* __restrict__ src, const int32_t* __re
                              double* __restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[idx[i]];
 }
-// u_{n+1} = 2 u_n - u_{n-1}; uPrev <- u_n (secant predictor)
-__global__ void extrapolateKernel(double* __restrict__ u, double* __restrict__ uPrev, int64_t n) {
+
+// duN1 <- duN ; duN <- u - uPrev ; uPrev <- u
+__global__ void historyKernel(const double* __restrict__ u, double* __restrict__ uPrev, double* __restrict__ duN,
+                              double* __restrict__ duN1, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const double cur = u[i];
-    u[i]             = 2.0 * cur - uPrev[i];
+    duN1[i]          = duN[i];
+    duN[i]           = cur - uPrev[i];
     uPrev[i]         = cur;
   }
 }
@@ -195,7 +198,7 @@ int ik_create(ik_handle** out, const ik_problem_desc* d) {
       h->faceAxis.upload(d->face_axis, h->nfaces, h->stream);
       h->faceSide.upload(d->face_side, h->nfaces, h->stream);
     }
-    for (int k = 0; k < 3; ++k) h->trac[k] = d->traction[k];
+    for (int k = 0; k < 3; ++k) h->trac[k] = h->mlpTraction[k] = d->traction[k];
     h->npl = d->num_point_loads;
     if (h->npl) {
       require(d->point_dof && d->point_coef, IK_EINVAL, "point load arrays missing");
@@ -233,6 +236,7 @@ void ik_destroy(ik_handle* h) {
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+  ik::mlpDestroy(h->mlp);
   for (auto ev : h->chunkEvents) (void)hipEventDestroy(ev);
   if (h->joinEvent) (void)hipEventDestroy(h->joinEvent);
   hipStream_t s = h->stream, s2 = h->stream2;
@@ -455,19 +459,27 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
     info->total_iterations += ni.iterations;
     info->linear_iterations += ni.linear_iterations;
     if (!ni.success) return;
-    ik::DevBuf<double> uPrev;
-    if (c->predictor == IK_PRED_LINEAR) {
+    require(c->predictor == IK_PRED_CONSTANT || c->predictor == IK_PRED_LINEAR || c->predictor == IK_PRED_MLP,
+            IK_EINVAL, "unknown predictor");
+    // increment history for the extrapolating predictors: duN = u_n - u_{n-1}, duN1 = u_{n-1} - u_{n-2}
+    ik::DevBuf<double> uPrev, duN, duN1;
+    if (c->predictor != IK_PRED_CONSTANT) {
       uPrev.ensure(h->ndof);
+      duN.ensure(h->ndof);
+      duN1.ensure(h->ndof);
       IK_HIP(hipMemcpyAsync(uPrev.p, h->u.p, h->ndof * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    } else {
-      require(c->predictor == IK_PRED_CONSTANT, IK_ENOTIMPL, "predictor not available in this build");
+      ik::launchZero(duN.p, h->ndof, h->stream);
+      ik::launchZero(duN1.p, h->ndof, h->stream);
     }
     for (int ls = 0; ls < c->load_steps; ++ls) {
-      if (c->predictor == IK_PRED_LINEAR && ls > 0)
-        hipLaunchKernelGGL(extrapolateKernel, dim3(grid(h->ndof)), dim3(256), 0, h->stream, h->u.p, uPrev.p, h->ndof);
-      else if (c->predictor == IK_PRED_LINEAR)
-        IK_HIP(hipMemcpyAsync(uPrev.p, h->u.p, h->ndof * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      if (c->predictor == IK_PRED_LINEAR && ls > 0) {
+        ik::axpy(h, 1.0, duN.p, h->u.p, h->ndof);  // u += u_n - u_{n-1} (ArcLength secant semantics)
+      } else if (c->predictor == IK_PRED_MLP) {
+        ik::mlpPredict(h, duN.p, duN1.p, h->lambda + dl, dl, ls > 0 ? dl : 0.0, ls > 1 ? dl : 0.0);
+      }
       h->lambda += dl;  // predictor: lambda += dlambda (loadcontrol.inl:55-57)
+      h->kValid        = false;
+      h->easCacheValid = false;
       newtonSolve(h, s, dl, &ni);
       if (iterations) iterations[ls + 1] = ni.iterations;
       if (residuals) residuals[ls + 1] = ni.residual_norm;
@@ -475,20 +487,21 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
       info->total_iterations += ni.iterations;
       info->linear_iterations += ni.linear_iterations;
       if (!ni.success) return;
+      if (c->predictor != IK_PRED_CONSTANT)
+        hipLaunchKernelGGL(historyKernel, dim3(grid(h->ndof)), dim3(256), 0, h->stream, h->u.p, uPrev.p, duN.p,
+                           duN1.p, h->ndof);
     }
     info->success = 1;
   });
 }
 
-int64_t ik_mlp_num_params(void) { return 14 * 128 + 128 + 2 * (128 * 128 + 128) + 3 * 128 + 3; }
+int64_t ik_mlp_num_params(void) { return ik::mlpNumParams(); }
 
 int ik_mlp_set_params(ik_handle* h, const float* params, int where) {
   return guard(h, [&] {
-    const int64_t np = ik_mlp_num_params();
-    h->mlpParams.ensure(np);
-    IK_HIP(hipMemcpyAsync(h->mlpParams.p, params, np * sizeof(float),
-                          where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream));
-    IK_HIP(hipStreamSynchronize(h->stream));
+    require(params != nullptr, IK_EINVAL, "null parameters");
+    if (!h->mlp) h->mlp = ik::mlpCreate(h->stream);
+    ik::mlpSetParams(h->mlp, params, where, h->stream);
     h->mlpLoaded = true;
   });
 }
@@ -496,14 +509,13 @@ int ik_mlp_set_params(ik_handle* h, const float* params, int where) {
 int ik_mlp_get_params(ik_handle* h, float* params, int where) {
   return guard(h, [&] {
     require(h->mlpLoaded, IK_EINVAL, "no MLP parameters set");
-    IK_HIP(hipMemcpyAsync(params, h->mlpParams.p, ik_mlp_num_params() * sizeof(float),
-                          where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
-    IK_HIP(hipStreamSynchronize(h->stream));
+    ik::mlpGetParams(h->mlp, params, where, h->stream);
   });
 }
 
 int ik_mlp_set_load(ik_handle* h, const double traction[3], double charLen) {
   return guard(h, [&] {
+    require(charLen > 0, IK_EINVAL, "characteristic length must be positive");
     for (int k = 0; k < 3; ++k) h->mlpTraction[k] = traction[k];
     h->mlpCharLen = charLen;
   });

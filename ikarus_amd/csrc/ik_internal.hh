@@ -11,6 +11,8 @@
 
 namespace ik {
 
+struct Mlp;
+
 struct Error {
   int code;
   std::string msg;
@@ -143,7 +145,7 @@ struct ik_handle {
   double* hred = nullptr;         // pinned host scalar for reductions
 
   // learned predictor
-  ik::DevBuf<float> mlpParams;
+  ik::Mlp* mlp = nullptr;
   bool mlpLoaded = false;
   double mlpTraction[3] = {0, 0, 0};
   double mlpCharLen = 1.0;
@@ -174,8 +176,13 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
 void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info);
 void residualBlock64(ik_handle* h, const double* b, const double* x, double* r);  // r = b - K x (FP64 K)
 // mlp.hip
-void mlpPredict(ik_handle* h, const double* uPrev, const double* uPrev2, double lamNext, double dlam,
-                double* uOut);
+Mlp* mlpCreate(hipStream_t s);
+void mlpDestroy(Mlp* m);
+int64_t mlpNumParams();
+void mlpSetParams(Mlp* m, const float* p, int where, hipStream_t s);
+void mlpGetParams(Mlp* m, float* p, int where, hipStream_t s);
+void mlpPredict(ik_handle* h, const double* duN, const double* duN1, double lamNext, double dlam, double dlamPrev,
+                double dlamPrev2);
 
 // timing helpers (events on the handle's stream around one launch)
 struct ScopedKernelTimer {

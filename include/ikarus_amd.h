@@ -174,13 +174,32 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
                     int32_t* iterations, double* residuals);
 
 /* ---- learned per-node predictor (no reference counterpart) ------------------------------------ */
-/* MLP 14 -> 128 -> 128 -> 128 -> 3, SiLU, BF16 weights on MFMA with FP32 accumulation. Parameters are
- * FP32 master copies laid out W1[128x14],b1,W2[128x128],b2,W3[128x128],b3,W4[3x128],b4. */
+/* MLP 14 -> 128 -> 128 -> 128 -> 3, SiLU, BF16 operands on MFMA with FP32 accumulation. Parameters are
+ * FP32 master copies laid out W1[128x14], b1[128], W2[128x128], b2, W3[128x128], b3, W4[3x128], b4[3]
+ * (row-major, W maps inputs to outputs). Features per node for the load step n -> n+1:
+ *   [X/L (3), t/40 (3), lambda_{n+1}, 40 dlambda, v_n/0.1 (3), v_{n-1}/0.1 (3)], v = du/dlambda;
+ * output y: v_{n+1} = v_n + 0.1 y and u_pred = u_n + dlambda v_{n+1} on free DOFs. */
 int64_t ik_mlp_num_params(void);
+/* parameters used by IK_PRED_MLP inside ik_load_control */
 int ik_mlp_set_params(ik_handle* h, const float* params, int where);
 int ik_mlp_get_params(ik_handle* h, float* params, int where);
-/* load-path descriptor for the features: traction direction*magnitude, current and next lambda */
+/* traction per unit load factor and length scale L used by the features (defaults: the problem's
+ * traction, L = 1) */
 int ik_mlp_set_load(ik_handle* h, const double traction[3], double char_length);
+
+/* Training context: forward, mean-squared-error gradient and Adam steps with the same kernels. */
+typedef struct ik_mlp ik_mlp;
+int ik_mlp_create(ik_mlp** m, uint64_t seed);  /* He-uniform init from SplitMix64(seed) */
+void ik_mlp_destroy(ik_mlp* m);
+const char* ik_mlp_last_error(const ik_mlp* m);
+int ik_mlp_params(ik_mlp* m, float* params, int where);
+int ik_mlp_load(ik_mlp* m, const float* params, int where);
+/* x: [n x 14] features, y: [n x 3] outputs (row-major FP32) */
+int ik_mlp_forward(ik_mlp* m, const float* x, int64_t n, float* y, int where);
+/* loss = mean((y - t)^2) over n x 3 entries; grad (nullable): d loss / d params */
+int ik_mlp_loss_grad(ik_mlp* m, const float* x, const float* t, int64_t n, double* loss, float* grad, int where);
+/* loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr */
+int ik_mlp_train_step(ik_mlp* m, const float* x, const float* t, int64_t n, float lr, double* loss, int where);
 
 /* ---- timing of the hot kernels on the handle's stream (HIP events) ----------------------------- */
 enum { IK_KERNEL_ASSEMBLE_KR = 0, IK_KERNEL_ASSEMBLE_R = 1, IK_KERNEL_SPMV = 2, IK_KERNEL_MLP = 3,

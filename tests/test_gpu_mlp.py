@@ -1,0 +1,132 @@
+"""GPU MLP kernels (BF16 MFMA, FP32 accumulation) against the numpy oracle, and the predictor in LoadControl."""
+import numpy as np
+import pytest
+
+import ikarus_amd as ia
+import mlp_oracle as mo
+from ikarus_amd import _capi as capi
+from ikarus_amd import learned
+from ikarus_amd import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, 14)).astype(np.float32)
+    t = (0.3 * np.sin(x[:, :3]) + 0.1 * x[:, 3:6] * x[:, 6:7]).astype(np.float32)
+    return x, t
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n", [1, 37, 1000])
+def test_forward_matches_oracle(n):
+    print(f"case: mlp forward n={n}", flush=True)
+    m = learned.MLP(seed=3)
+    p = m.params()
+    x, _ = _data(n, 1)
+    y = m.forward(x)
+    yo, _, _ = mo.forward(p, x, bf16=True)
+    assert _rel(y, yo) < 2e-3
+    y64, _, _ = mo.forward(p.astype(np.float64), x.astype(np.float64), bf16=False)
+    assert _rel(y, y64) < 5e-2  # BF16 operands vs exact FP64
+
+
+@pytest.mark.parametrize("n", [64, 3000])
+def test_gradient_matches_oracle(n):
+    print(f"case: mlp gradient n={n}", flush=True)
+    m = learned.MLP(seed=4)
+    p = m.params()
+    x, t = _data(n, 2)
+    loss, g = m.loss_grad(x, t)
+    lo, go = mo.loss_grad(p, x, t, bf16=True)
+    assert abs(loss - lo) <= 1e-3 * lo
+    for name, sl in _slices():
+        assert _rel(g[sl], go[sl]) < 1e-2, name
+    _, g64 = mo.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), bf16=False)
+    assert _rel(g, g64) < 5e-2
+
+
+def _slices():
+    o = 0
+    for name, shp in mo.SHAPES:
+        k = int(np.prod(shp))
+        yield name, slice(o, o + k)
+        o += k
+
+
+def test_adam_step_matches_oracle_and_training_reduces_loss():
+    print("case: mlp training", flush=True)
+    m = learned.MLP(seed=5)
+    p0 = m.params().astype(np.float64)
+    x, t = _data(4096, 3)
+    lo, go = mo.loss_grad(p0.astype(np.float32), x, t, bf16=True)
+    l1 = m.train_step(x, t, 1e-3)
+    p1 = m.params()
+    po1, _, _ = mo.adam(p0, go, np.zeros_like(p0), np.zeros_like(p0), 1, lr=1e-3)
+    # the first Adam step moves every parameter by about lr in the direction of -sign(g)
+    moved = np.abs(p1 - p0)
+    big = np.abs(go) > 1e-3 * np.abs(go).max()
+    assert np.allclose(moved[big], 1e-3, rtol=0.05)
+    assert np.mean(np.sign(p1 - p0)[big] == np.sign(po1 - p0)[big]) > 0.98
+    for _ in range(300):
+        last = m.train_step(x, t, 2e-3)
+    assert last < 0.2 * l1
+
+
+def test_zero_network_predictor_equals_secant_predictor():
+    """With W4 = b4 = 0 the MLP predicts v_{n+1} = v_n, i.e. u_n + du_n: the secant predictor."""
+    print("case: mlp predictor vs secant", flush=True)
+    p = pb.hex_block((4, 4, 4), seed=8)
+    params = learned.MLP(seed=1).params()
+    d = mo.unpack(params.astype(np.float64))
+    d["W4"][:] = 0
+    d["b4"][:] = 0
+    params = mo.pack(d).astype(np.float32)
+    res = {}
+    for pred in (capi.PRED_LINEAR, capi.PRED_MLP):
+        dev = ia.Device(p)
+        if pred == capi.PRED_MLP:
+            dev.check(capi.lib().ik_mlp_set_params(dev.h, capi.ptr(params), capi.HOST))
+        dev.set_requirement(np.zeros(p.ndof), 0.0)
+        cfg = capi.LCConfig()
+        cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 6, 0.0, 1.0, pred
+        info, iters, r = dev.load_control(cfg, ia.nr_settings(tol=1e-9, linear=ia.solver_opts(rtol=1e-12)))
+        u, _ = dev.get_requirement()
+        res[pred] = (list(iters), u)
+    assert res[capi.PRED_LINEAR][0] == res[capi.PRED_MLP][0]
+    assert _rel(res[capi.PRED_MLP][1], res[capi.PRED_LINEAR][1]) < 1e-12
+
+
+def test_trained_predictor_reduces_newton_iterations():
+    """Train on solver paths at 6^3 and evaluate on an unseen seed at 8^3: the learned predictor needs no
+    more Newton iterations than the secant predictor and fewer than the constant one."""
+    print("case: mlp train + eval small", flush=True)
+    rng = np.random.default_rng(0)
+    xs, ts = [], []
+    for s in range(12):
+        p = pb.hex_block((6, 6, 6), seed=300 + s)
+        hist = learned.solve_path(p, 10)
+        x, t = learned.samples_from_path(p, hist, 128, rng)
+        xs.append(x)
+        ts.append(t)
+    m, losses = learned.train(np.concatenate(xs), np.concatenate(ts), epochs=60, batch=1024, lr=3e-3)
+    assert losses[-1] < 0.5 * losses[0]
+    params = m.params()
+    p = pb.hex_block((8, 8, 8), seed=999)
+    tot = {}
+    for pred in (capi.PRED_CONSTANT, capi.PRED_LINEAR, capi.PRED_MLP):
+        dev = ia.Device(p)
+        if pred == capi.PRED_MLP:
+            dev.check(capi.lib().ik_mlp_set_params(dev.h, capi.ptr(params), capi.HOST))
+        dev.set_requirement(np.zeros(p.ndof), 0.0)
+        cfg = capi.LCConfig()
+        cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 10, 0.0, 1.0, pred
+        info, iters, res = dev.load_control(cfg, ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-12)))
+        assert info.success and res[-1] <= 1e-8
+        tot[pred] = info.total_iterations
+    print("newton iterations constant/secant/mlp:", tot, flush=True)
+    assert tot[capi.PRED_MLP] <= tot[capi.PRED_LINEAR] < tot[capi.PRED_CONSTANT]
