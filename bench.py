@@ -61,19 +61,32 @@ def cpu_baseline(seconds):
                        f"(tangent per node pair), serial, {dt:.1f} s")
 
 
-def newton_path(p, mode, steps=5):
+NEWTON_MODES = {
+    # plain FP64 form: FP64 tangent, Jacobi-PCG to 1e-10 per Newton iteration, constant predictor (LoadControl)
+    "fp64": dict(precision=capi.FP64, kind=capi.PCG_JACOBI_FP64, rtol=1e-10, forcing=0.0, pred=capi.PRED_CONSTANT),
+    # same precision, inexact Newton + secant predictor (isolates the algorithmic part of the gain)
+    "fp64_inexact_secant": dict(precision=capi.FP64, kind=capi.PCG_JACOBI_FP64, rtol=1e-10, forcing=0.9,
+                                pred=capi.PRED_LINEAR),
+    # mixed form: FP32 tangent + FP32 block-Jacobi PCG with FP64 refinement, inexact Newton, secant predictor
+    "mixed_secant": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
+                         pred=capi.PRED_LINEAR),
+    # mixed form + learned predictor
+    "mixed_mlp": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
+                      pred=capi.PRED_MLP),
+}
+
+
+def newton_path(p, mode, steps, mlp_params=None):
+    m = NEWTON_MODES[mode]
     dev = ia.Device(p)
+    if m["pred"] == capi.PRED_MLP:
+        dev.check(capi.lib().ik_mlp_set_params(dev.h, capi.ptr(mlp_params), capi.HOST))
     dev.set_requirement(np.zeros(p.ndof), 0.0)
     cfg = capi.LCConfig()
-    cfg.load_steps, cfg.tbegin, cfg.tend = steps, 0.0, 1.0
-    if mode == "fp64":
-        cfg.predictor = capi.PRED_CONSTANT
-        s = ia.nr_settings(tol=1e-8, precision=capi.FP64, linear=ia.solver_opts(capi.PCG_JACOBI_FP64, rtol=1e-10))
-    else:
-        cfg.predictor = capi.PRED_LINEAR
-        s = ia.nr_settings(tol=1e-8, precision=capi.FP32,
-                           linear=ia.solver_opts(capi.PCG_BJACOBI_FP32_IR, rtol=1e-6, inner_rtol=1e-3, max_refine=20),
-                           forcing=1e-2)
+    cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = steps, 0.0, 1.0, m["pred"]
+    s = ia.nr_settings(tol=1e-8, precision=m["precision"],
+                       linear=ia.solver_opts(m["kind"], rtol=m["rtol"], inner_rtol=1e-3, max_refine=30),
+                       forcing=m["forcing"])
     dev.synchronize()
     t0 = time.perf_counter()
     info, iters, res = dev.load_control(cfg, s)
@@ -81,8 +94,9 @@ def newton_path(p, mode, steps=5):
     dt = time.perf_counter() - t0
     u, _ = dev.get_requirement()
     dev.close()
-    return dict(seconds=round(dt, 4), success=bool(info.success), newton_iterations=[int(i) for i in iters],
-                linear_iterations=int(info.linear_iterations), final_residual=float(res[info.solves - 1])), u
+    return dict(seconds=round(dt, 4), success=bool(info.success), newton_iterations=int(info.total_iterations),
+                per_step=[int(i) for i in iters], linear_iterations=int(info.linear_iterations),
+                final_residual=float(res[info.solves - 1])), u
 
 
 def main():
@@ -93,6 +107,9 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="elements per direction")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--newton-steps", type=int, default=40, help="load steps of the Newton comparison")
+    ap.add_argument("--newton-seed", type=int, default=2000, help="unseen seed of the Newton comparison")
+    ap.add_argument("--mlp", default=os.path.join(ROOT, "ikarus_amd", "data", "predictor_mlp_32cubed.f32"))
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -182,15 +199,26 @@ def main():
         "residual_kernel_ms": round(msr / max(nr, 1), 4),
     }
     if not args.no_newton:
-        log("[bench] newton fp64 ...")
-        r64, u64 = newton_path(p, "fp64")
-        log(f"[bench] newton fp64 {r64}")
-        log("[bench] newton mixed ...")
-        rmx, umx = newton_path(p, "mixed")
-        log(f"[bench] newton mixed {rmx}")
-        out["newton"] = {"load_steps": 5, "fp64": r64, "mixed": rmx,
-                         "speedup_mixed_vs_fp64": round(r64["seconds"] / rmx["seconds"], 3),
-                         "rel_diff_u": float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))}
+        pn = pb.hex_block((n, n, n), seed=args.newton_seed)
+        mlp = np.fromfile(args.mlp, dtype=np.float32) if os.path.exists(args.mlp) else None
+        res, us = {}, {}
+        for mode in NEWTON_MODES:
+            if mode == "mixed_mlp" and mlp is None:
+                continue
+            log(f"[bench] newton {mode} ...")
+            res[mode], us[mode] = newton_path(pn, mode, args.newton_steps, mlp)
+            log(f"[bench] newton {mode}: {res[mode]['seconds']} s, {res[mode]['newton_iterations']} Newton, "
+                f"{res[mode]['linear_iterations']} PCG iterations")
+        best = "mixed_mlp" if "mixed_mlp" in res else "mixed_secant"
+        ref = us["fp64"]
+        out["newton"] = {
+            "workload": f"hex8 Neo-Hooke {n}^3, {args.newton_steps} load steps to ||R||<=1e-8, unseen seed "
+                        f"{args.newton_seed}, traction {[round(t, 3) for t in pn.traction]}",
+            "modes": res,
+            "speedup_vs_fp64": {k: round(res["fp64"]["seconds"] / v["seconds"], 3) for k, v in res.items()},
+            "rel_diff_u_vs_fp64": {k: float(np.linalg.norm(us[k] - ref) / np.linalg.norm(ref)) for k in res},
+            "headline": f"{best} vs fp64",
+        }
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

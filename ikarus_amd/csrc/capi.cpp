@@ -119,7 +119,8 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
   info->success = 1;
   assembleR(h, IK_DBC_FULL);
   double rNorm        = normR(h);
-  const double r0     = rNorm;
+  double rPrev        = rNorm;
+  double etaPrev      = 0.0;
   info->residual_norm = rNorm;
   int iter            = 0;
   h->kValid           = false;
@@ -127,7 +128,22 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     assembleK(h, IK_DBC_FULL, s->precision);
     // K x = R ; correction = -x (solve(correction, -rx), newtonraphson.hh:222-223)
     double rtol = s->linear.rtol;
-    if (s->forcing > 0.0 && r0 > 0.0) rtol = std::max(s->linear.rtol, std::min(0.5, s->forcing * rNorm / r0));
+    if (s->forcing > 0.0) {
+      // inexact Newton, Eisenstat-Walker choice 2 (gamma = forcing, eta_max = 0.1) with the safeguard
+      // eta >= gamma eta_prev^2 and a floor: no need to solve below half the absolute Newton tolerance
+      constexpr double etaMax = 0.1;
+      double eta = etaMax;
+      if (iter > 0) {
+        eta = s->forcing * (rNorm / rPrev) * (rNorm / rPrev);
+        const double sg = s->forcing * etaPrev * etaPrev;
+        if (sg > 0.1) eta = std::max(eta, sg);
+      }
+      eta     = std::min(eta, etaMax);
+      eta     = std::max(eta, 0.5 * s->tol / std::max(rNorm, 1e-300));
+      rtol    = std::max(eta, s->linear.rtol);
+      etaPrev = rtol;
+    }
+    rPrev = rNorm;
     ik_solver_info li{};
     linearSolve(h, h->R.p, h->du.p, &s->linear, rtol, &li);
     info->linear_iterations += li.iterations;

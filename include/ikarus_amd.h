@@ -141,8 +141,9 @@ typedef struct {
   int32_t min_iter;
   int32_t precision;     /* tangent precision IK_FP64 | IK_FP32                             */
   ik_solver_opts linear;
-  double forcing;        /* > 0: inexact Newton, linear rtol = min(linear.rtol,
-                            forcing * ||R_k|| / ||R_0||) ; 0: fixed linear.rtol          */
+  double forcing;        /* > 0: inexact Newton, Eisenstat-Walker choice 2 with gamma =
+                            forcing, eta_max = 0.1, floor max(linear.rtol, tol/(2||R_k||));
+                            0: every linear solve to linear.rtol                        */
 } ik_nr_settings;
 
 typedef struct {

@@ -95,3 +95,17 @@ def test_cantilever_eas21_golden_on_gpu(material, maxu):
     assert info.total_iterations == 80
     assert abs(np.max(np.abs(u)) - maxu) <= 1e-10 * maxu
     assert abs(lam - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("material", ["nh", "svk"])
+def test_cpp_mirror_api_cantilever(material):
+    """The same golden test through the header-only C++ mirror (include/ikarus_amd/ikarus.hh)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp", "_build",
+                       "cantilever_eas")
+    assert os.path.exists(exe), "C++ test program not built (run __graft_entry__.build())"
+    print(f"case: C++ cantilever {material}", flush=True)
+    r = subprocess.run([exe, material], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
