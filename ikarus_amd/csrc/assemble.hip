@@ -6,13 +6,24 @@
 // the Traction skill (mechanics/loads/traction.hh:106-138) and the SparseFlatAssembler scatter with its
 // Dirichlet treatment (ikarus/assembler/simpleassemblers.inl:60-169).
 //
-// Element kernel (one 64-lane wavefront = 8 elements, 8 lanes per element):
-//   phase 0  lane (el, q) stages node q of element el (X, u) in LDS;
-//   phase 1  lane (el, q) evaluates Gauss point q: J, J^-1, dN/dX of all 8 nodes, F, S*dV, C*dV (LDS);
-//   phase 2  lane (el, a) accumulates 4-5 of the 36 upper 3x3 blocks of K_e (blocks (a, a+j mod 8),
-//            j = 0..3, plus j = 4 on lanes 0-3, so every lane has the same work) and R_e[a] over the
-//            8 Gauss points; C*B_a is formed once per Gauss point and reused for every block;
-//   phase 3  the blocks and R_e are stored element-by-element with plain stores.
+// Tangent form. With g_a = dN_a/dX, C^-1 = Ci and the per-node vectors p_a = F Ci g_a, the Voigt product
+// B_a^T CC B_b + (g_a^T S g_b) I of both laws collapses (minor symmetries of CC) to
+//   K_ab[k][l] = dV [ w1 p_ak p_bl + w2 p_al p_bk + w2 (g_a . Ci g_b) Phi_kl + (g_a . S g_b) d_kl ],
+//   Phi = F Ci F^T,  Neo-Hooke: w1 = lambda, w2 = mu - lambda ln J;  SVK: Ci -> I, w1 = lambda, w2 = mu,
+// which is the reference's B^T CC B + kg (nonlinearelastic.hh:313-322, 294-302) summed in another order.
+// The two rank-1 parts are batched 24x24 contractions over the 8 Gauss points,
+//   W1 = sum_g w1 dV p_g p_g^T,  W2 = sum_g w2 dV p_g p_g^T,  K_ab[k][l] = W1[3a+k][3b+l] + W2[3a+l][3b+k] + ...,
+// and run on the FP64 matrix cores (v_mfma_f64_4x4x4: the four 4x4 blocks of one instruction are four
+// elements; 21 upper tiles x 2 k-steps per matrix).
+//
+// Element kernel (one 64-lane wavefront = 4 elements):
+//   phase 0  stage X, u of the 4 x 8 nodes in LDS;
+//   phase 1  lane pair (el, gp) evaluates one Gauss point, each lane half the nodes: J, g_a, F (partial
+//            sums combined across the pair with DPP), the material, p_a -> LDS; R_e is reduced over the
+//            Gauss points with DPP row rotations;
+//   phase 2  W1, W2 on MFMA, written into the LDS element blocks (W2 transposed within each 3x3 block);
+//   phase 3  lane (el, a, half) adds the Phi and g^T S g terms of its node pairs;
+//   phase 4  the 4 x 36 upper 3x3 blocks go to global memory with 16-byte coalesced stores.
 // Gather kernels then sum, for every 3x3 block of the CSR pattern, the element blocks listed for it
 // (a fixed order, so the result is bitwise reproducible), apply the FULL Dirichlet treatment on the fly
 // and write each block once. Elements are processed in chunks; each chunk's gather runs right behind
@@ -22,351 +33,489 @@
 namespace ik {
 namespace {
 
-constexpr int EPB = 8;  // elements per 64-lane block
+constexpr int EPW = 4;  // elements per 64-lane wavefront
 
-// packed upper-triangle index of a symmetric 6x6 (row-major)
-__host__ __device__ constexpr int sym6(int p, int q) {
-  return p <= q ? p * 6 - (p * (p - 1)) / 2 + (q - p) : q * 6 - (q * (q - 1)) / 2 + (p - q);
-}
 // Voigt pairs [11,22,33,23,13,12] (ikarus/utils/tensorutils.hh:181-202)
 __host__ __device__ constexpr int vI(int v) { return v < 3 ? v : (v == 3 ? 1 : 0); }
 __host__ __device__ constexpr int vJ(int v) { return v < 3 ? v : (v == 5 ? 1 : 2); }
+// Voigt index of the symmetric pair (i, j)
+__host__ __device__ constexpr int voigt(int i, int j) { return i == j ? i : 6 - i - j; }
 
-template <typename T>
-__device__ inline void gaussPoint(int q, T xi[3]) {
+__device__ inline void gaussPoint(int q, double xi[3]) {
   // 2-point Gauss-Legendre on [0,1]: 1/2 -+ 1/(2 sqrt 3), x-index fastest
-  const T d = T(0.28867513459481288225);
+  const double d = 0.28867513459481288225;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) xi[k] = ((q >> k) & 1) ? T(0.5) + d : T(0.5) - d;
+  for (int k = 0; k < 3; ++k) xi[k] = ((q >> k) & 1) ? 0.5 + d : 0.5 - d;
 }
 
 // dN_a/dxi for the trilinear corner a = (a&1, a>>1&1, a>>2&1) (Dune reference cube)
-template <typename T>
-__device__ inline void dShape(int a, const T xi[3], T dN[3]) {
-  T f[3], s[3];
+__device__ inline void dShape(int a, const double xi[3], double dN[3]) {
+  double f[3], s[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const bool hi = (a >> k) & 1;
-    f[k]          = hi ? xi[k] : T(1) - xi[k];
-    s[k]          = hi ? T(1) : T(-1);
+    f[k]          = hi ? xi[k] : 1.0 - xi[k];
+    s[k]          = hi ? 1.0 : -1.0;
   }
   dN[0] = s[0] * f[1] * f[2];
   dN[1] = f[0] * s[1] * f[2];
   dN[2] = f[0] * f[1] * s[2];
 }
 
-template <typename T>
-__device__ inline T det3(const T A[3][3]) {
+__device__ inline double det3(const double A[3][3]) {
   return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
          A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
 }
 
-template <typename T>
-__device__ inline void inv3(const T A[3][3], T d, T I[3][3]) {
-  const T id = T(1) / d;
-  I[0][0]    = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) * id;
-  I[0][1]    = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
-  I[0][2]    = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
-  I[1][0]    = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) * id;
-  I[1][1]    = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
-  I[1][2]    = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
-  I[2][0]    = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) * id;
-  I[2][1]    = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
-  I[2][2]    = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+__device__ inline void inv3(const double A[3][3], double d, double I[3][3]) {
+  const double id = 1.0 / d;
+  I[0][0]         = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) * id;
+  I[0][1]         = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+  I[0][2]         = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+  I[1][0]         = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) * id;
+  I[1][1]         = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+  I[1][2]         = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+  I[2][0]         = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) * id;
+  I[2][1]         = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+  I[2][2]         = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
 }
 
-// PK2 stress S (Voigt, no shear factor) and packed Voigt tangent from F.
+// PK2 stress S (Voigt), Ci = C^-1 (Voigt-packed, Neo-Hooke only) and the weight w2 of the tangent form.
 // Returns false when det C <= 0 (the reference aborts, materialhelpers.hh:120-126).
-template <typename T, int KIND>
-__device__ inline bool material(T lam, T mu, const T F[3][3], T S[6], T Cp[21]) {
-  T C[3][3];
+template <int KIND>
+__device__ inline bool material(double lam, double mu, const double F[3][3], double S[6], double Ci[6],
+                                double& w2) {
+  double C[6];
 #pragma unroll
-  for (int I = 0; I < 3; ++I)
-#pragma unroll
-    for (int J = 0; J < 3; ++J) C[I][J] = F[0][I] * F[0][J] + F[1][I] * F[1][J] + F[2][I] * F[2][J];
+  for (int v = 0; v < 6; ++v) C[v] = F[0][vI(v)] * F[0][vJ(v)] + F[1][vI(v)] * F[1][vJ(v)] + F[2][vI(v)] * F[2][vJ(v)];
   if constexpr (KIND == IK_SVK) {
     // E = (C - I)/2; S = lam tr(E) I + 2 mu E (svk.hh:100-140)
-    const T E0 = T(0.5) * (C[0][0] - T(1)), E1 = T(0.5) * (C[1][1] - T(1)), E2 = T(0.5) * (C[2][2] - T(1));
-    const T tr = E0 + E1 + E2;
-    S[0]       = lam * tr + T(2) * mu * E0;
-    S[1]       = lam * tr + T(2) * mu * E1;
-    S[2]       = lam * tr + T(2) * mu * E2;
-    S[3]       = mu * C[1][2];  // 2 mu E12 = mu C12
-    S[4]       = mu * C[0][2];
-    S[5]       = mu * C[0][1];
+    const double E0 = 0.5 * (C[0] - 1.0), E1 = 0.5 * (C[1] - 1.0), E2 = 0.5 * (C[2] - 1.0);
+    const double tr = E0 + E1 + E2;
+    S[0]            = lam * tr + 2.0 * mu * E0;
+    S[1]            = lam * tr + 2.0 * mu * E1;
+    S[2]            = lam * tr + 2.0 * mu * E2;
+    S[3]            = mu * C[3];  // 2 mu E23 = mu C23
+    S[4]            = mu * C[4];
+    S[5]            = mu * C[5];
 #pragma unroll
-    for (int i = 0; i < 21; ++i) Cp[i] = T(0);
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int q = p; q < 3; ++q) Cp[sym6(p, q)] = lam + (p == q ? T(2) * mu : T(0));
-#pragma unroll
-    for (int p = 3; p < 6; ++p) Cp[sym6(p, p)] = mu;
+    for (int v = 0; v < 6; ++v) Ci[v] = v < 3 ? 1.0 : 0.0;
+    w2 = mu;
     return true;
   } else {
-    // Neo-Hooke (neohooke.hh:79-142): S = mu (I - C^-1) + lam lnJ C^-1,
-    // C_IJKL = lam Ci_IJ Ci_KL + (mu - lam lnJ)(Ci_IK Ci_JL + Ci_IL Ci_JK)
-    const T detC = det3(C);
-    if (!(detC > T(0))) return false;
-    T Ci[3][3];
-    inv3(C, detC, Ci);
-    const T lnJ = log(sqrt(detC));
-    const T c2  = mu - lam * lnJ;
+    // Neo-Hooke (neohooke.hh:79-142): S = mu (I - C^-1) + lam lnJ C^-1
+    const double detC = C[0] * (C[1] * C[2] - C[3] * C[3]) - C[5] * (C[5] * C[2] - C[3] * C[4]) +
+                        C[4] * (C[5] * C[3] - C[1] * C[4]);
+    if (!(detC > 0.0)) return false;
+    const double id = 1.0 / detC;
+    Ci[0]           = (C[1] * C[2] - C[3] * C[3]) * id;
+    Ci[1]           = (C[0] * C[2] - C[4] * C[4]) * id;
+    Ci[2]           = (C[0] * C[1] - C[5] * C[5]) * id;
+    Ci[3]           = (C[4] * C[5] - C[0] * C[3]) * id;
+    Ci[4]           = (C[5] * C[3] - C[1] * C[4]) * id;
+    Ci[5]           = (C[4] * C[3] - C[2] * C[5]) * id;
+    const double lnJ = log(sqrt(detC));
 #pragma unroll
-    for (int v = 0; v < 6; ++v) {
-      const T ci = Ci[vI(v)][vJ(v)];
-      S[v]       = mu * ((v < 3 ? T(1) : T(0)) - ci) + lam * lnJ * ci;
-    }
-#pragma unroll
-    for (int p = 0; p < 6; ++p)
-#pragma unroll
-      for (int q = p; q < 6; ++q) {
-        const int I = vI(p), J = vJ(p), K = vI(q), L = vJ(q);
-        Cp[sym6(p, q)] = lam * Ci[I][J] * Ci[K][L] + c2 * (Ci[I][K] * Ci[J][L] + Ci[I][L] * Ci[J][K]);
-      }
+    for (int v = 0; v < 6; ++v) S[v] = mu * ((v < 3 ? 1.0 : 0.0) - Ci[v]) + lam * lnJ * Ci[v];
+    w2 = mu - lam * lnJ;
     return true;
   }
 }
 
-// B_a (6x3): column k = [F_k0 g0, F_k1 g1, F_k2 g2, F_k1 g2 + F_k2 g1, F_k0 g2 + F_k2 g0, F_k0 g1 + F_k1 g0]
-// (derivative of the Green-Lagrange strain w.r.t. the nodal displacement, Voigt with engineering shear)
-template <typename T>
-__device__ inline void bop(const T F[3][3], const T g[3], T B[6][3]) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    B[0][k] = F[k][0] * g[0];
-    B[1][k] = F[k][1] * g[1];
-    B[2][k] = F[k][2] * g[2];
-    B[3][k] = F[k][1] * g[2] + F[k][2] * g[1];
-    B[4][k] = F[k][0] * g[2] + F[k][2] * g[0];
-    B[5][k] = F[k][0] * g[1] + F[k][1] * g[0];
-  }
+// symmetric (Voigt-packed) matrix times vector
+__device__ inline void symMul(const double M[6], const double x[3], double y[3]) {
+  y[0] = M[0] * x[0] + M[5] * x[1] + M[4] * x[2];
+  y[1] = M[5] * x[0] + M[1] * x[1] + M[3] * x[2];
+  y[2] = M[4] * x[0] + M[3] * x[1] + M[2] * x[2];
 }
 
-template <typename T>
-struct Lds {
-  T X[EPB][8][3];
-  T U[EPB][8][3];
-  T G[EPB][8][8][3];  // [element][gauss point][node][dir]
-  T F[EPB][8][9];
-  T S[EPB][8][6];     // S * dV
-  T C[EPB][8][21];    // C * dV (packed)
-};
+// DPP lane moves within a row of 16 lanes (64-bit value as two 32-bit moves)
+template <int CTRL>
+__device__ inline double dpp(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+constexpr int DPP_XOR1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;   // quad_perm [2,3,0,1]
 
 // upper-block index of (a, b), a <= b: row-major over the upper triangle of the 8x8 node pairs
 __host__ __device__ constexpr int upperBlock(int a, int b) { return 8 * a - (a * (a - 1)) / 2 + (b - a); }
 
-// One launch covers elements [e0, e0 + count). Outputs per element e (absolute index):
-//   Kebe[(e*36 + sb)*9 + 3k + l] = K_e[3a+k][3b+l] for the upper block sb = upperBlock(a, b)
-//   Rebe[e*24 + 3a + k]          = R_e[3a+k]
-template <typename T, int KIND, bool WANT_K, bool WANT_R>
-__global__ __launch_bounds__(64) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
-                                                  const int32_t* __restrict__ conn, int64_t e0, int64_t count,
-                                                  double lam_, double mu_, T* __restrict__ Kebe,
-                                                  double* __restrict__ Rebe, unsigned long long* __restrict__ err) {
-  __shared__ Lds<T> s;
-  const int lane     = threadIdx.x;
-  const int el       = lane >> 3;
-  const int q        = lane & 7;
-  const int64_t eloc = (int64_t)blockIdx.x * EPB + el;
-  const bool valid   = eloc < count;
-  const int64_t e    = e0 + eloc;
-  const T lam = T(lam_), mu = T(mu_);
-
-  // phase 0: stage node q
-  if (valid) {
-    const int64_t nd = conn[e * 8 + q];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      s.X[el][q][k] = T(X[nd * 3 + k]);
-      s.U[el][q][k] = T(U[nd * 3 + k]);
+// copy n staged values (n a multiple of 4) to global memory with 16-byte stores
+template <typename TK>
+__device__ inline void storeBlocks(const double* src, TK* dst, int n, int lane) {
+  if constexpr (sizeof(TK) == 8) {
+    const double2* s = reinterpret_cast<const double2*>(src);
+    double2* d       = reinterpret_cast<double2*>(dst);
+    for (int j = lane; j < n / 2; j += 64) d[j] = s[j];
+  } else {
+    const double4* s = reinterpret_cast<const double4*>(src);
+    float4* d        = reinterpret_cast<float4*>(dst);
+    for (int j = lane; j < n / 4; j += 64) {
+      const double4 v = s[j];
+      d[j]            = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
     }
-  }
-  __syncthreads();
-
-  // phase 1: Gauss point q
-  if (valid) {
-    T xi[3];
-    gaussPoint(q, xi);
-    T dN[8][3];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) dShape(a, xi, dN[a]);
-    T J[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        T acc = T(0);
-#pragma unroll
-        for (int a = 0; a < 8; ++a) acc += s.X[el][a][i] * dN[a][j];
-        J[i][j] = acc;
-      }
-    const T detJ = det3(J);
-    T Ji[3][3];
-    inv3(J, detJ, Ji);
-    T F[3][3] = {{T(1), T(0), T(0)}, {T(0), T(1), T(0)}, {T(0), T(0), T(1)}};
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      T g[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) g[i] = Ji[0][i] * dN[a][0] + Ji[1][i] * dN[a][1] + Ji[2][i] * dN[a][2];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        s.G[el][q][a][i] = g[i];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) F[i][j] += s.U[el][a][i] * g[j];
-      }
-    }
-    T Sv[6], Cp[21];
-    const bool ok = material<T, KIND>(lam, mu, F, Sv, Cp);
-    if (!ok) atomicMin(err, (unsigned long long)e);
-    const T dV = fabs(detJ) * T(0.125);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) s.F[el][q][i] = F[i / 3][i % 3];
-#pragma unroll
-    for (int v = 0; v < 6; ++v) s.S[el][q][v] = Sv[v] * dV;
-    if constexpr (WANT_K) {
-#pragma unroll
-      for (int i = 0; i < 21; ++i) s.C[el][q][i] = Cp[i] * dV;
-    }
-  }
-  __syncthreads();
-  if (!valid) return;
-
-  // phase 2: node a = q owns upper blocks (a, (a+j) & 7), j = 0..3 (+ j = 4 when a < 4)
-  const int a = q;
-  constexpr int NB = 5;
-  const int nb = a < 4 ? 5 : 4;
-  T Kb[NB][9];
-  T Ra[3] = {T(0), T(0), T(0)};
-  if constexpr (WANT_K) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-#pragma unroll
-      for (int i = 0; i < 9; ++i) Kb[j][i] = T(0);
-  }
-  for (int g = 0; g < 8; ++g) {
-    T F[3][3], Sv[6], ga[3];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = s.F[el][g][i];
-#pragma unroll
-    for (int v = 0; v < 6; ++v) Sv[v] = s.S[el][g][v];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ga[i] = s.G[el][g][a][i];
-    T Ba[6][3];
-    bop(F, ga, Ba);
-    if constexpr (WANT_R) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        T acc = T(0);
-#pragma unroll
-        for (int v = 0; v < 6; ++v) acc += Ba[v][k] * Sv[v];
-        Ra[k] += acc;
-      }
-    }
-    if constexpr (WANT_K) {
-      T CBa[6][3];
-#pragma unroll
-      for (int v = 0; v < 6; ++v)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          T acc = T(0);
-#pragma unroll
-          for (int w = 0; w < 6; ++w) acc += s.C[el][g][sym6(v, w)] * Ba[w][k];
-          CBa[v][k] = acc;
-        }
-      // S g_a for the geometric stiffness g_a^T S g_b
-      const T Sga0 = Sv[0] * ga[0] + Sv[5] * ga[1] + Sv[4] * ga[2];
-      const T Sga1 = Sv[5] * ga[0] + Sv[1] * ga[1] + Sv[3] * ga[2];
-      const T Sga2 = Sv[4] * ga[0] + Sv[3] * ga[1] + Sv[2] * ga[2];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        if (j < nb) {
-          const int b = (a + j) & 7;
-          T gb[3];
-#pragma unroll
-          for (int i = 0; i < 3; ++i) gb[i] = s.G[el][g][b][i];
-          T Bb[6][3];
-          bop(F, gb, Bb);
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) {
-              T acc = Kb[j][k * 3 + l];
-#pragma unroll
-              for (int v = 0; v < 6; ++v) acc += CBa[v][k] * Bb[v][l];
-              Kb[j][k * 3 + l] = acc;
-            }
-          const T kg = Sga0 * gb[0] + Sga1 * gb[1] + Sga2 * gb[2];
-          Kb[j][0] += kg;
-          Kb[j][4] += kg;
-          Kb[j][8] += kg;
-        }
-      }
-    }
-  }
-
-  // phase 3: element-by-element stores (K_e(a,b) for b >= a, else transposed into (b,a))
-  if constexpr (WANT_K) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      if (j < nb) {
-        const int b    = (a + j) & 7;
-        const bool tr  = b < a;
-        const int sb   = tr ? upperBlock(b, a) : upperBlock(a, b);
-        T* dst         = Kebe + (e * 36 + sb) * 9;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int l = 0; l < 3; ++l) dst[tr ? l * 3 + k : k * 3 + l] = Kb[j][k * 3 + l];
-      }
-    }
-  }
-  if constexpr (WANT_R) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) Rebe[e * 24 + a * 3 + k] = double(Ra[k]);
   }
 }
 
-// K gather: thread per 3x3 block over blocks [p0, p1): sum the listed element blocks (transposed where
-// flagged) in list order, then FULL: rows/columns of constrained DOFs are zero with a unit diagonal
-// (simpleassemblers.inl:152-169).
+// symmetric-pair index of (k, l) = its Voigt index
+constexpr int NSYM = 6;
+
+// One launch covers elements [e0, e0 + count), EPW per 64-lane workgroup. Outputs per element e:
+//   dest == nullptr: Kebe[(e*36 + sb)*9 + 3k + l] = K_e[3a+k][3b+l] for the upper block sb = upperBlock(a, b)
+//   otherwise block sb goes to slot dest[e*36+sb] >> 1 of the gather order (transposed when bit 0 is set)
+//   Rebe[e*24 + 3a + k]          = R_e[3a+k]
+// The element values are computed in FP64; TK = float rounds the blocks once at the store.
+//
+// Lane = 16 kk + 4 bk + i: element bk, Gauss points kk and kk + 4, nodes i and i + 4. In the
+// component-major ordering (row 8k + a) of K_e, the 8x8 node matrix K^{kl}[a][b] = K_ab[k][l] is
+//   K^{kl} = sum_g [ w1 p_k p_l^T + w2 p_l p_k^T ] + sum_{g,I} g_I (Psi_kl Ci g + d_kl S g)_I^T,
+// (p_k = (p_ak)_a, g_I = (g_aI)_a over the nodes) so with the contraction index (gauss point, term) every
+// operand of v_mfma_f64_4x4x4 is a value the lane computed itself: A[a][k] at lane 16 k + 4 bk + a,
+// B[k][b] at lane 16 k + 4 bk + b, D[a][b] at lane 16 a + 4 bk + b. Tiles (alpha, beta) of K^{kl} cover
+// nodes 4 alpha..+3 x 4 beta..+3; (0,1) for all kl, (0,0) and (1,1) for k <= l (the rest follow from
+// K_ab[k][l] = K_ba[l][k]): 21 tiles x (2 gauss blocks x (2 W terms + 3 I)) contraction steps.
+template <int KIND, typename TK, bool WANT_K, bool WANT_R>
+__global__ __launch_bounds__(64, 2) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
+                                                     const int32_t* __restrict__ conn, int64_t e0, int64_t count,
+                                                     double lam, double mu, const int32_t* __restrict__ dest,
+                                                     TK* __restrict__ Kebe, double* __restrict__ Rebe,
+                                                     unsigned long long* __restrict__ err) {
+  const int lane     = threadIdx.x;
+  const int kk       = lane >> 4, bk = (lane >> 2) & 3, i = lane & 3;
+  const int64_t base = (int64_t)blockIdx.x * EPW;
+  const int nvalid   = (int)(count - base < EPW ? count - base : EPW);
+  const bool valid   = bk < nvalid;
+  // lanes of a missing tail element compute the last valid element; only their stores are masked
+  const int64_t e = e0 + base + (valid ? bk : nvalid - 1);
+
+  // staging slots of the wavefront's element blocks, fetched early so the stores do not wait on them
+  __shared__ int32_t codes[EPW * 36];
+  if constexpr (WANT_K) {
+    if (dest != nullptr)
+      for (int j = lane; j < nvalid * 36; j += 64) codes[j] = dest[(e0 + base) * 36 + j];
+  }
+
+  double x[2][3], u[2][3];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t nd = conn[e * 8 + i + 4 * h];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      x[h][k] = X[nd * 3 + k];
+      u[h][k] = U[nd * 3 + k];
+    }
+  }
+
+  // accumulators: K01[3k+l] tile (0,1) of K^{kl}; K00/K11[v] tiles (0,0)/(1,1) of K^{kl}, k <= l, v = voigt(k,l);
+  // G01[v - 3] the Psi term of tile (0,1) for k < l (shared by K^{kl} and K^{lk})
+  double K01[9], K00[NSYM], K11[NSYM], G01[3];
+  if constexpr (WANT_K) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) K01[t] = 0.0;
+#pragma unroll
+    for (int t = 0; t < NSYM; ++t) K00[t] = K11[t] = 0.0;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) G01[t] = 0.0;
+  }
+  double rr[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+
+#pragma unroll
+  for (int sblk = 0; sblk < 2; ++sblk) {
+    const int gp = 4 * sblk + kk;
+    double xi[3];
+    gaussPoint(gp, xi);
+    double dN[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) dShape(i + 4 * h, xi, dN[h]);
+    // J = sum_a x_a dN_a^T over the quad's 8 nodes (2 per lane, combined with DPP)
+    double J[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double v = x[0][p] * dN[0][q] + x[1][p] * dN[1][q];
+        v += dpp<DPP_XOR1>(v);
+        v += dpp<DPP_XOR2>(v);
+        J[p][q] = v;
+      }
+    const double detJ = det3(J);
+    double Ji[3][3];
+    inv3(J, detJ, Ji);
+    double g[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int p = 0; p < 3; ++p) g[h][p] = Ji[0][p] * dN[h][0] + Ji[1][p] * dN[h][1] + Ji[2][p] * dN[h][2];
+    double F[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double v = u[0][p] * g[0][q] + u[1][p] * g[1][q];
+        v += dpp<DPP_XOR1>(v);
+        v += dpp<DPP_XOR2>(v);
+        F[p][q] = (p == q ? 1.0 : 0.0) + v;
+      }
+    double Sv[6], Ci[6], w2;
+    const bool ok = material<KIND>(lam, mu, F, Sv, Ci, w2);
+    if (!ok && valid && i == 0) atomicMin(err, (unsigned long long)e);
+    const double dV = fabs(detJ) * 0.125;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) Sv[v] *= dV;
+    double sg[2][3];  // S dV g
+#pragma unroll
+    for (int h = 0; h < 2; ++h) symMul(Sv, g[h], sg[h]);
+    if constexpr (WANT_R) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rr[h][k] += F[k][0] * sg[h][0] + F[k][1] * sg[h][1] + F[k][2] * sg[h][2];
+    }
+    if constexpr (WANT_K) {
+      // M = F Ci, p = M g, Psi = w2 dV M F^T, cg = Ci g
+      double M[3][3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double row[3] = {F[k][0], F[k][1], F[k][2]};
+        symMul(Ci, row, M[k]);
+      }
+      const double w1d = lam * dV, w2d = w2 * dV;
+      double P[2][3], cg[2][3], psi[6];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) P[h][k] = M[k][0] * g[h][0] + M[k][1] * g[h][1] + M[k][2] * g[h][2];
+        symMul(Ci, g[h], cg[h]);
+      }
+#pragma unroll
+      for (int v = 0; v < 6; ++v) {
+        const int k = vI(v), l = vJ(v);
+        psi[v]      = w2d * (M[k][0] * F[l][0] + M[k][1] * F[l][1] + M[k][2] * F[l][2]);
+      }
+      // W terms: A = p_{alpha,k} (or l), B = w p_{beta,l} (or k)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          K01[3 * k + l] = __builtin_amdgcn_mfma_f64_4x4x4f64(P[0][k], w1d * P[1][l], K01[3 * k + l], 0, 0, 0);
+          K01[3 * k + l] = __builtin_amdgcn_mfma_f64_4x4x4f64(P[0][l], w2d * P[1][k], K01[3 * k + l], 0, 0, 0);
+          if (k <= l) {
+            const int v = voigt(k, l);
+            K00[v]      = __builtin_amdgcn_mfma_f64_4x4x4f64(P[0][k], w1d * P[0][l], K00[v], 0, 0, 0);
+            K00[v]      = __builtin_amdgcn_mfma_f64_4x4x4f64(P[0][l], w2d * P[0][k], K00[v], 0, 0, 0);
+            K11[v]      = __builtin_amdgcn_mfma_f64_4x4x4f64(P[1][k], w1d * P[1][l], K11[v], 0, 0, 0);
+            K11[v]      = __builtin_amdgcn_mfma_f64_4x4x4f64(P[1][l], w2d * P[1][k], K11[v], 0, 0, 0);
+          }
+        }
+      // Psi and g^T S g terms: A = g_{alpha,I}, B = Psi_v cg_{beta,I} + d_kl sg_{beta,I}
+#pragma unroll
+      for (int v = 0; v < 6; ++v) {
+        const bool dg = v < 3;
+#pragma unroll
+        for (int I = 0; I < 3; ++I) {
+          const double b0 = dg ? psi[v] * cg[0][I] + sg[0][I] : psi[v] * cg[0][I];
+          const double b1 = dg ? psi[v] * cg[1][I] + sg[1][I] : psi[v] * cg[1][I];
+          if (dg)
+            K01[4 * v] = __builtin_amdgcn_mfma_f64_4x4x4f64(g[0][I], b1, K01[4 * v], 0, 0, 0);
+          else
+            G01[v - 3] = __builtin_amdgcn_mfma_f64_4x4x4f64(g[0][I], b1, G01[v - 3], 0, 0, 0);
+          K00[v] = __builtin_amdgcn_mfma_f64_4x4x4f64(g[0][I], b0, K00[v], 0, 0, 0);
+          K11[v] = __builtin_amdgcn_mfma_f64_4x4x4f64(g[1][I], b1, K11[v], 0, 0, 0);
+        }
+      }
+    }
+  }
+
+  if constexpr (WANT_R) {
+    // sum over the four Gauss-point pairs kk (lanes 16 apart)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double v = rr[h][k];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        rr[h][k] = v;
+      }
+    if (kk == 0 && valid) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Rebe[e * 24 + 3 * (i + 4 * h) + k] = rr[h][k];
+    }
+  }
+  if constexpr (WANT_K) {
+#pragma unroll
+    for (int v = 3; v < 6; ++v) {
+      const int k = vI(v), l = vJ(v);
+      K01[3 * k + l] += G01[v - 3];
+      K01[3 * l + k] += G01[v - 3];
+    }
+    // result lane = 16 r + 4 bk + c holds rows a = r (+4), columns b = c (+4); every upper-block entry is
+    // written once into LDS, then the 4 elements leave with 16-byte coalesced stores
+    __shared__ __align__(16) double ebe[EPW][324];
+    {
+      const int r = kk, c = i;
+      double* out = ebe[bk];
+      {
+        double* blk = out + upperBlock(r, c + 4) * 9;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) blk[t] = K01[t];
+      }
+#pragma unroll
+      for (int al = 0; al < 2; ++al) {
+        const int a = r + 4 * al, b = c + 4 * al;
+#pragma unroll
+        for (int v = 0; v < 6; ++v) {
+          const int k = vI(v), l = vJ(v);
+          const double val = al ? K11[v] : K00[v];
+          if (r < c) {
+            out[upperBlock(a, b) * 9 + 3 * k + l] = val;
+          } else if (r == c) {
+            out[upperBlock(a, a) * 9 + 3 * k + l] = val;
+            if (k != l) out[upperBlock(a, a) * 9 + 3 * l + k] = val;
+          } else if (k != l) {
+            out[upperBlock(b, a) * 9 + 3 * l + k] = val;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (dest == nullptr) {
+      storeBlocks<TK>(&ebe[0][0], Kebe + (e0 + base) * 324, nvalid * 324, lane);
+    } else if (lane < 63) {
+      // staging order: 9 lanes per block, 7 blocks per pass, each block to its slot (transposed if flagged)
+      const int t = lane % 9, k = t / 3, l = t - 3 * (t / 3);
+      for (int j = lane / 9; j < nvalid * 36; j += 7) {
+        const int el = j / 36, sb = j - 36 * (j / 36);
+        const int32_t code = codes[j];
+        Kebe[(int64_t)(code >> 1) * 9 + t] = (TK)ebe[el][sb * 9 + ((code & 1) ? 3 * l + k : t)];
+      }
+    }
+  }
+}
+
+// Residual-only element kernel (R_e = sum_gp F (S dV) g_a): lane (el, gp) = 8 elements x 8 Gauss points per
+// wavefront, nodes staged in LDS; lane (el, a) then sums node a's contributions over the Gauss points in a
+// fixed order.
+template <int KIND>
+__global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X, const double* __restrict__ U,
+                                                   const int32_t* __restrict__ conn, int64_t e0, int64_t count,
+                                                   double lam, double mu, double* __restrict__ Rebe,
+                                                   unsigned long long* __restrict__ err) {
+  __shared__ double XU[8][8][6];
+  __shared__ double part[8][8][25];  // [element][gauss point][3a+k] (+1 pad)
+  const int lane     = threadIdx.x;
+  const int el       = lane >> 3, q = lane & 7;
+  const int64_t base = (int64_t)blockIdx.x * 8;
+  const int nvalid   = (int)(count - base < 8 ? count - base : 8);
+  const bool valid   = el < nvalid;
+  const int64_t e    = e0 + base + (valid ? el : nvalid - 1);
+  {
+    const int64_t nd = conn[e * 8 + q];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      XU[el][q][k]     = X[nd * 3 + k];
+      XU[el][q][3 + k] = U[nd * 3 + k];
+    }
+  }
+  __syncthreads();
+  double xi[3];
+  gaussPoint(q, xi);
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    double dN[3];
+    dShape(a, xi, dN);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) J[i][j] += XU[el][a][i] * dN[j];
+  }
+  const double detJ = det3(J);
+  double Ji[3][3];
+  inv3(J, detJ, Ji);
+  double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    double dN[3], g[3];
+    dShape(a, xi, dN);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g[i] = Ji[0][i] * dN[0] + Ji[1][i] * dN[1] + Ji[2][i] * dN[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[i][j] += XU[el][a][3 + i] * g[j];
+  }
+  double Sv[6], Ci[6], w2;
+  if (!material<KIND>(lam, mu, F, Sv, Ci, w2) && valid) atomicMin(err, (unsigned long long)e);
+  const double dV = fabs(detJ) * 0.125;
+#pragma unroll
+  for (int v = 0; v < 6; ++v) Sv[v] *= dV;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    double dN[3], g[3], sg[3];
+    dShape(a, xi, dN);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g[i] = Ji[0][i] * dN[0] + Ji[1][i] * dN[1] + Ji[2][i] * dN[2];
+    symMul(Sv, g, sg);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) part[el][q][3 * a + k] = F[k][0] * sg[0] + F[k][1] * sg[1] + F[k][2] * sg[2];
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double acc = 0.0;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) acc += part[el][g][3 * q + k];
+      Rebe[e * 24 + 3 * q + k] = acc;
+    }
+  }
+}
+
+// K gather: thread per (upper block, entry) over the upper blocks [q0, q1) (in row order). The staged
+// blocks of upper block q occupy the contiguous slots [segptr[q], segptr[q+1]) (ordered by element), so
+// consecutive threads read consecutive values. The entry's sum is treated with FULL (rows/columns of
+// constrained DOFs zero, unit diagonal; simpleassemblers.inl:152-169) and written to the block and,
+// transposed, to its mirror block below the diagonal (K and the Dirichlet treatment are symmetric).
 template <typename T>
-__global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ kcptr, const int32_t* __restrict__ kclist,
-                                               const int32_t* __restrict__ blockRow,
+__global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segptr, const int32_t* __restrict__ upBlk,
+                                               const int32_t* __restrict__ upT, const int32_t* __restrict__ blockRow,
                                                const int32_t* __restrict__ colidx,
                                                const uint8_t* __restrict__ fixed, int maskDbc,
-                                               const T* __restrict__ Kebe, int64_t p0, int64_t p1,
+                                               const T* __restrict__ Kst, int64_t q0, int64_t q1,
                                                T* __restrict__ K) {
-  const int64_t p = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= p1) return;
-  const int32_t c0 = kcptr[p], c1 = kcptr[p + 1];
-  T acc[9];
+  const uint32_t loc = blockIdx.x * 256u + threadIdx.x;  // (q - q0) * 9 + t
+  const uint32_t ql   = loc / 9u;
+  if ((int64_t)ql >= q1 - q0) return;
+  const int64_t q  = q0 + ql;
+  const int t      = (int)(loc - ql * 9u);
+  const int k      = t / 3, l = t - 3 * (t / 3);
+  int32_t c        = segptr[q];
+  const int32_t c1 = segptr[q + 1];
+  T acc            = T(0);
+  // segments hold one block per element sharing the two nodes (8 at most on hexahedral meshes whose
+  // nodes have 8 elements); the loads of a group of 8 are issued together, then summed in order
+  for (; c < c1; c += 8) {
+    T v[8];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) acc[i] = T(0);
-  for (int32_t c = c0; c < c1; ++c) {
-    const int32_t code = kclist[c];
-    const T* src       = Kebe + (int64_t)(code >> 1) * 9;
-    T v[9];
+    for (int j = 0; j < 8; ++j) v[j] = c + j < c1 ? Kst[(int64_t)(c + j) * 9 + t] : T(0);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) v[i] = src[i];
-    const bool tr = code & 1;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int l = 0; l < 3; ++l) acc[k * 3 + l] += tr ? v[l * 3 + k] : v[k * 3 + l];
+    for (int j = 0; j < 8; ++j)
+      if (c + j < c1) acc += v[j];
   }
+  const int32_t p = upBlk[q];
   if (maskDbc) {
-    const int64_t r = (int64_t)blockRow[p] * 3, c = (int64_t)colidx[p] * 3;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int l = 0; l < 3; ++l)
-        if (fixed[r + k] | fixed[c + l]) acc[k * 3 + l] = (r + k == c + l) ? T(1) : T(0);
+    const int64_t r = (int64_t)blockRow[p] * 3 + k, cc = (int64_t)colidx[p] * 3 + l;
+    if (fixed[r] | fixed[cc]) acc = (r == cc) ? T(1) : T(0);
   }
-  T* dst = K + p * 9;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) dst[i] = acc[i];
+  K[(int64_t)p * 9 + t] = acc;
+  const int32_t pt      = upT[q];
+  if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = acc;
 }
 
 // R gather: thread per DOF over node rows [r0, r1)
@@ -461,18 +610,24 @@ inline unsigned gridFor(int64_t n, int bs) {
 }
 
 template <typename T, int KIND, bool WK, bool WR>
-void launchElem(ik_handle* h, int64_t e0, int64_t count, T* Kebe, double* Rebe) {
-  const unsigned grid = (unsigned)((count + EPB - 1) / EPB);
-  hipLaunchKernelGGL((hex8Element<T, KIND, WK, WR>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p, h->conn.p,
-                     e0, count, h->lam, h->mu, Kebe, Rebe, h->errFlag.p);
+void launchElem(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, T* Kebe, double* Rebe) {
+  if constexpr (!WK) {
+    const unsigned grid = (unsigned)((count + 7) / 8);
+    hipLaunchKernelGGL((hex8Residual<KIND>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p, h->conn.p, e0,
+                       count, h->lam, h->mu, Rebe, h->errFlag.p);
+  } else {
+    const unsigned grid = (unsigned)((count + EPW - 1) / EPW);
+    hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p,
+                       h->conn.p, e0, count, h->lam, h->mu, dest, Kebe, Rebe, h->errFlag.p);
+  }
 }
 
 template <typename T, bool WK, bool WR>
-void launchElemMat(ik_handle* h, int64_t e0, int64_t count, T* Kebe, double* Rebe) {
+void launchElemMat(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, T* Kebe, double* Rebe) {
   if (h->material == IK_SVK)
-    launchElem<T, IK_SVK, WK, WR>(h, e0, count, Kebe, Rebe);
+    launchElem<T, IK_SVK, WK, WR>(h, e0, count, dest, Kebe, Rebe);
   else
-    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, Kebe, Rebe);
+    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, dest, Kebe, Rebe);
 }
 
 }  // namespace
@@ -513,23 +668,24 @@ static void runChunks(ik_handle* h, int dbc, bool wantK, bool wantR, T* Kebe, T*
     {
       ScopedKernelTimer t(h, wantK ? KASM_KR : KASM_R, h->stream);
       if (h->neas)
-        easStage(h, e0, e1 - e0, wantK);
+        easStage(h, e0, e1 - e0, wantK, h->bdest.p);
       else if (wantK && wantR)
-        launchElemMat<T, true, true>(h, e0, e1 - e0, Kebe, h->ebeR.p);
+        launchElemMat<T, true, true>(h, e0, e1 - e0, h->bdest.p, Kebe, h->ebeR.p);
       else if (wantK)
-        launchElemMat<T, true, false>(h, e0, e1 - e0, Kebe, nullptr);
+        launchElemMat<T, true, false>(h, e0, e1 - e0, h->bdest.p, Kebe, nullptr);
       else
-        launchElemMat<double, false, true>(h, e0, e1 - e0, (double*)nullptr, h->ebeR.p);
+        launchElemMat<double, false, true>(h, e0, e1 - e0, nullptr, (double*)nullptr, h->ebeR.p);
     }
     const int64_t r0 = h->chunkRow[c], r1 = h->chunkRow[c + 1];
     if (r1 <= r0) continue;
     IK_HIP(hipEventRecord(h->chunkEvents[c], h->stream));
     IK_HIP(hipStreamWaitEvent(h->stream2, h->chunkEvents[c], 0));
     if (wantK) {
-      const int64_t p0 = h->hrowptr[r0], p1 = h->hrowptr[r1];
+      const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
       ScopedKernelTimer t(h, KGATHER, h->stream2);
-      hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, h->stream2, h->kcptr.p,
-                         h->kclist.p, h->blockRow.p, h->colidx.p, h->fixed.p, mask, Kebe, p0, p1, K);
+      hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)(((q1 - q0) * 9 + 255) / 256)), dim3(256), 0, h->stream2,
+                         h->segptr.p, h->upBlk.p, h->upT.p, h->blockRow.p, h->colidx.p, h->fixed.p, mask, Kebe, q0,
+                         q1, K);
     }
     if (wantR)
       hipLaunchKernelGGL(gatherR, dim3((unsigned)(((r1 - r0) * 3 + 255) / 256)), dim3(256), 0, h->stream2,
@@ -559,7 +715,18 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
     h->ebeK32.ensure((size_t)h->ne * 324);
     runChunks<float>(h, dbc, true, false, h->ebeK32.p, h->K32.p);
   }
-  if (wantR && !fused) runChunks<double>(h, dbc, false, true, (double*)nullptr, (double*)nullptr);
+  if (wantR && !fused) {
+    // residual alone: one element launch over the mesh, then one gather (a few MB; no staging overlap needed)
+    {
+      ScopedKernelTimer t(h, KASM_R, h->stream);
+      if (h->neas)
+        easStage(h, 0, h->ne, false, nullptr);
+      else
+        launchElemMat<double, false, true>(h, 0, h->ne, nullptr, (double*)nullptr, h->ebeR.p);
+    }
+    hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
+                       h->nclist.p, h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
+  }
   if (h->neas) h->easCacheValid = true;
   if (wantR && h->nfaces) {
     hipLaunchKernelGGL(tractionKernel, dim3(gridFor(h->nfaces, 128)), dim3(128), 0, h->stream, h->X.p, h->conn.p,
@@ -578,9 +745,9 @@ void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re) {
   h->ebeK64.ensure((size_t)h->ne * 324);
   h->ebeR.ensure((size_t)h->ne * 24);
   if (h->neas)
-    easStage(h, e, 1, true);
+    easStage(h, e, 1, true, nullptr);
   else
-    launchElemMat<double, true, true>(h, e, 1, h->ebeK64.p, h->ebeR.p);
+    launchElemMat<double, true, true>(h, e, 1, nullptr, h->ebeK64.p, h->ebeR.p);
   checkErrFlag(h);
   DevBuf<double> kd, rd;
   kd.ensure(576);

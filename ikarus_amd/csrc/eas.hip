@@ -159,7 +159,8 @@ template <int KIND, bool WANT_K>
 __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, const double* __restrict__ U,
                                                  const int32_t* __restrict__ conn,
                                                  const double* __restrict__ alpha, int64_t e0, double lam,
-                                                 double mu, double* __restrict__ Kebe, double* __restrict__ Rebe,
+                                                 double mu, const int32_t* __restrict__ dest,
+                                                 double* __restrict__ Kebe, double* __restrict__ Rebe,
                                                  double* __restrict__ cache, unsigned long long* __restrict__ err) {
   __shared__ EasLds s;
   const int t     = threadIdx.x;
@@ -458,7 +459,14 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
         out[6] = out[2];
         out[7] = out[5];
       }
-      for (int i = 0; i < 9; ++i) dst[i] = out[i];
+      if (dest) {
+        const int32_t code = dest[e * 36 + t];
+        dst                = Kebe + (int64_t)(code >> 1) * 9;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) dst[3 * k + l] = (code & 1) ? out[3 * l + k] : out[3 * k + l];
+      } else {
+        for (int i = 0; i < 9; ++i) dst[i] = out[i];
+      }
     }
   }
   if (t >= 40 && t < 64) {
@@ -488,24 +496,25 @@ __global__ __launch_bounds__(256) void easUpdateKernel(const double* __restrict_
 }
 
 template <bool WK>
-void launchEas(ik_handle* h, int64_t e0, int64_t count, double* Kebe) {
+void launchEas(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, double* Kebe) {
   if (h->material == IK_SVK)
     hipLaunchKernelGGL((easElement<IK_SVK, WK>), dim3((unsigned)count), dim3(64), 0, h->stream, h->X.p, h->u.p,
-                       h->conn.p, h->alpha.p, e0, h->lam, h->mu, Kebe, h->ebeR.p, h->easCache.p, h->errFlag.p);
+                       h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, Kebe, h->ebeR.p, h->easCache.p,
+                       h->errFlag.p);
   else
     hipLaunchKernelGGL((easElement<IK_NEOHOOKE, WK>), dim3((unsigned)count), dim3(64), 0, h->stream, h->X.p,
-                       h->u.p, h->conn.p, h->alpha.p, e0, h->lam, h->mu, Kebe, h->ebeR.p, h->easCache.p,
+                       h->u.p, h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, Kebe, h->ebeR.p, h->easCache.p,
                        h->errFlag.p);
 }
 
 }  // namespace
 
-void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK) {
+void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t* dest) {
   h->easCache.ensure((size_t)h->ne * (NE * 24 + NE));
   if (wantK)
-    launchEas<true>(h, e0, count, h->ebeK64.p);
+    launchEas<true>(h, e0, count, dest, h->ebeK64.p);
   else
-    launchEas<false>(h, e0, count, nullptr);
+    launchEas<false>(h, e0, count, nullptr, nullptr);
 }
 
 void easUpdate(ik_handle* h, const double* duDev) {

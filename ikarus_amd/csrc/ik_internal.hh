@@ -111,9 +111,14 @@ struct ik_handle {
   std::vector<int32_t> hrowptr, hcolidx, hdiag;
   int64_t nnzb = 0;
   ik::DevBuf<int32_t> rowptr, colidx, diagBlk, blockRow;
-  // gather lists: for every block, the element blocks that sum into it (code = ((e*36+sb)<<1)|transposed),
-  // ordered by element; for every node, its (element, corner) incidences (code = e*8+a)
-  ik::DevBuf<int32_t> kcptr, kclist, ncptr, nclist;
+  // upper blocks (column >= row) in row order: their block index and the index of their transpose;
+  // huptr[r] = first upper block of row r
+  std::vector<int64_t> huptr;
+  ik::DevBuf<int32_t> upBlk, upT;
+  // staging order of the element blocks: upper block by upper block, each segment ordered by element.
+  // bdest[e*36+sb] = (slot << 1) | transposed, segptr[q] = first slot of upper block q; for every node,
+  // its (element, corner) incidences (code = e*8+a), ordered by element
+  ik::DevBuf<int32_t> bdest, segptr, ncptr, nclist;
   // element chunks and the block rows that are complete after each chunk
   std::vector<int64_t> chunkElem, chunkRow;
   // element-by-element staging of K_e (36 upper 3x3 blocks) and R_e (24)
@@ -166,7 +171,8 @@ void launchZeroF(float* p, int64_t n, hipStream_t s);
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR);
 void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re);
 void easUpdate(ik_handle* h, const double* duDev);
-void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK);
+// element blocks go to staging slots dest[e*36+sb] (nullptr: element-by-element order)
+void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t* dest);
 // sparse.hip
 void spmv64(ik_handle* h, const double* x, double* y);
 void spmv32(ik_handle* h, const float* x, float* y);

@@ -4,8 +4,9 @@
 // and keeps per element (3*8)^2 positions into the CSC value array (simpleassemblers.inl:207-222,
 // 254-265; eigensparseaddon.hh:19-29). For FlatInterleaved DOFs (dof = 3*node + k, fehelper.hh:146-152)
 // that scalar pattern is exactly the node-adjacency pattern expanded by 3x3 blocks, so the device keeps
-// the block form: block row = node, sorted block columns, and 64 int32 block slots per hex8 element
-// (slot[a*8+b] = position of block (node_a, node_b)). The scalar CSC is re-derived from it on demand.
+// the block form: block row = node, sorted block columns, and per hex8 element 36 int32 staging slots
+// (one per upper element block, pointing into the gather order of its K block). The scalar CSC is
+// re-derived from it on demand.
 #include <algorithm>
 
 #include "ik_internal.hh"
@@ -53,34 +54,54 @@ void buildPattern(ik_handle* h) {
     const auto b = h->hcolidx.begin() + h->hrowptr[i], en = h->hcolidx.begin() + h->hrowptr[i + 1];
     h->hdiag[i] = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
   }
-  // Gather lists. Element e stores its 36 upper blocks (a <= b) at sb(a,b) = 8a - a(a-1)/2 + (b-a);
-  // block (node_a, node_b) receives +block sb(a,b) when a <= b and +block(sb(b,a))^T otherwise.
-  require((int64_t)ne * 72 < (int64_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 gather codes");
+  // Element e computes its 36 upper blocks (a <= b) sb(a,b) = 8a - a(a-1)/2 + (b-a). Block (node_a, node_b)
+  // is an upper block of K when node_a <= node_b and receives the element block as it is; otherwise the
+  // upper block (node_b, node_a) receives its transpose. Staging slots are grouped by upper block and
+  // ordered by element inside each group, so the gather reads contiguous segments and sums them in a
+  // fixed order.
+  require((int64_t)ne * 36 < (int64_t)INT32_MAX / 2, IK_EINVAL, "mesh too large for int32 staging slots");
+  std::vector<int32_t> slots((size_t)ne * 36);  // block index p of each element block
   std::vector<int32_t> kcnt(h->nnzb + 1, 0);
-  std::vector<int32_t> slots((size_t)ne * 64);
   for (int64_t e = 0; e < ne; ++e)
-    for (int a = 0; a < 8; ++a) {
-      const int32_t r = conn[e * 8 + a];
-      const auto b = h->hcolidx.begin() + h->hrowptr[r], en = h->hcolidx.begin() + h->hrowptr[r + 1];
-      for (int c = 0; c < 8; ++c) {
-        const auto it = std::lower_bound(b, en, conn[e * 8 + c]);
-        const int32_t p = (int32_t)(it - h->hcolidx.begin());
-        slots[e * 64 + a * 8 + c] = p;
+    for (int a = 0; a < 8; ++a)
+      for (int c = a; c < 8; ++c) {
+        const int32_t na = conn[e * 8 + a], nc = conn[e * 8 + c];
+        const int32_t r = na < nc ? na : nc, col = na < nc ? nc : na;
+        const auto b = h->hcolidx.begin() + h->hrowptr[r], en = h->hcolidx.begin() + h->hrowptr[r + 1];
+        const int32_t p = (int32_t)(std::lower_bound(b, en, col) - h->hcolidx.begin());
+        slots[e * 36 + 8 * a - (a * (a - 1)) / 2 + (c - a)] = p;
         kcnt[p + 1]++;
       }
+  // upper blocks and their transposes
+  h->huptr.assign(nn + 1, 0);
+  for (int64_t i = 0; i < nn; ++i) h->huptr[i + 1] = h->huptr[i] + (h->hrowptr[i + 1] - h->hdiag[i]);
+  std::vector<int32_t> up(h->huptr[nn]), upT(h->huptr[nn]);
+  for (int64_t i = 0; i < nn; ++i)
+    for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
+      const int32_t c = h->hcolidx[p];
+      const auto b = h->hcolidx.begin() + h->hrowptr[c], en = h->hcolidx.begin() + h->hrowptr[c + 1];
+      const int64_t q = h->huptr[i] + (p - h->hdiag[i]);
+      up[q]           = p;
+      upT[q]          = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
     }
-  for (int64_t p = 0; p < h->nnzb; ++p) kcnt[p + 1] += kcnt[p];
-  std::vector<int32_t> klist(kcnt[h->nnzb]);
+  // segments in upper-block order, then each element block's slot (elements visited in increasing order)
+  std::vector<int32_t> seg(h->huptr[nn] + 1, 0);
+  for (int64_t q = 0; q < h->huptr[nn]; ++q) seg[q + 1] = seg[q] + kcnt[up[q] + 1];
+  std::vector<int32_t> qOf(h->nnzb, -1);
+  for (int64_t q = 0; q < h->huptr[nn]; ++q) qOf[up[q]] = (int32_t)q;
+  std::vector<int32_t> dest((size_t)ne * 36);
   {
-    std::vector<int32_t> pos(kcnt.begin(), kcnt.end() - 1);
-    for (int64_t e = 0; e < ne; ++e)  // increasing e: each list is ordered by element
+    std::vector<int32_t> pos(seg.begin(), seg.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
       for (int a = 0; a < 8; ++a)
-        for (int c = 0; c < 8; ++c) {
-          const int lo = a < c ? a : c, hi = a < c ? c : a;
-          const int sb = 8 * lo - (lo * (lo - 1)) / 2 + (hi - lo);
-          const int32_t code = (int32_t)(((e * 36 + sb) << 1) | (a > c ? 1 : 0));
-          klist[pos[slots[e * 64 + a * 8 + c]]++] = code;
+        for (int c = a; c < 8; ++c) {
+          const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
+          const int32_t q = qOf[slots[e * 36 + sb]];
+          const bool tr   = conn[e * 8 + a] > conn[e * 8 + c];
+          dest[e * 36 + sb] = (pos[q]++ << 1) | (tr ? 1 : 0);
         }
+    for (int64_t q = 0; q < h->huptr[nn]; ++q)
+      require(pos[q] == seg[q + 1], IK_EINVAL, "internal error: staging segments do not tile the slots");
   }
   std::vector<int32_t> brow(h->nnzb);
   for (int64_t i = 0; i < nn; ++i)
@@ -122,8 +143,10 @@ void buildPattern(ik_handle* h) {
   h->colidx.upload(h->hcolidx.data(), h->hcolidx.size(), h->stream);
   h->diagBlk.upload(h->hdiag.data(), h->hdiag.size(), h->stream);
   h->blockRow.upload(brow.data(), brow.size(), h->stream);
-  h->kcptr.upload(kcnt.data(), kcnt.size(), h->stream);
-  h->kclist.upload(klist.data(), klist.size(), h->stream);
+  h->upBlk.upload(up.data(), up.size(), h->stream);
+  h->upT.upload(upT.data(), upT.size(), h->stream);
+  h->bdest.upload(dest.data(), dest.size(), h->stream);
+  h->segptr.upload(seg.data(), seg.size(), h->stream);
   h->ncptr.upload(ncnt.data(), ncnt.size(), h->stream);
   h->nclist.upload(nlist.data(), nlist.size(), h->stream);
   IK_HIP(hipStreamSynchronize(h->stream));
