@@ -3,13 +3,14 @@
 Workload (default): 3D hex8 Neo-Hooke block, 64x64x64 elements (823,875 DOFs), unit cube, E=1000, nu=0.3,
 seeded +-5% interior jitter (seed 0), bottom face clamped, seeded top-face traction, FP64.
 
-One step = one full Newton-iteration assembly of the hot path: tangent K (FP64, FULL Dirichlet
-treatment, scattered into the 3x3-block CSR pattern) plus residual R, inputs resident in HBM.
+One step = one Newton iteration's assembly of the hot path at a fixed state: residual R, then tangent K
+(FP64, FULL Dirichlet treatment, summed into the 3x3-block CSR pattern), inputs resident in HBM.
 value = elements assembled per second (Melem/s) over the timed steps.
 
-Also reported (same run): the dominant kernel's roofline figures (HIP events on the library stream),
-the CPU baseline (oracle restatement timed on a bounded sample on the host cores), SpMV bandwidth,
-and the Newton load path (5 load steps to ||R|| <= 1e-8) in the plain FP64 form and the mixed form.
+Also reported (same run): the element kernel's roofline figures (HIP events on the library stream; HBM
+traffic per launch from the committed rocprofv3 counter summary), the gather and SpMV bandwidths, the CPU
+baseline (oracle restatement timed on a bounded sample on the host cores) and the Newton load path
+(40 load steps to ||R|| <= 1e-8) in the plain FP64 form and the mixed / learned forms.
 """
 import argparse
 import json
@@ -99,6 +100,71 @@ def newton_path(p, mode, steps, mlp_params=None):
                 final_residual=float(res[info.solves - 1])), u
 
 
+def assembly_step(dev):
+    """One Newton iteration's assembly at a fixed state: R (FULL) then K (FULL, FP64), the reference order
+    (newtonraphson.hh:206-223)."""
+    dev.check(capi.lib().ik_vector(dev.h, capi.DBC_FULL, None, capi.DEVICE))
+    dev.check(capi.lib().ik_matrix(dev.h, capi.DBC_FULL, capi.FP64))
+
+
+def kernel_pass(dev, p, steps):
+    """Per-kernel launch durations from HIP events on the library stream: in timing mode the element chunks of
+    an assembly run back to back between one pair of events, then the gathers between another, so these are
+    the kernels' stand-alone durations (the timed steps above overlap them on two streams)."""
+    dev.timing(True)
+    dev.timing_reset()
+    for _ in range(steps):
+        assembly_step(dev)
+    out = {}
+    for name, kid in (("element", capi.KERNEL_ASSEMBLE_KR), ("gather", capi.KERNEL_GATHER_K),
+                      ("residual", capi.KERNEL_ASSEMBLE_R)):
+        n, ms = dev.timing_read(kid)
+        out[name] = (n, ms)
+    dev.timing(False)
+    return out
+
+
+def spmv_pass(dev, p, precision, iters=50):
+    """Average SpMV launch inside the PCG of the given precision (FULL K assembled first)."""
+    dev.check(capi.lib().ik_matrix(dev.h, capi.DBC_FULL, precision))
+    x = np.random.default_rng(1).standard_normal(p.ndof)
+    kind = capi.PCG_JACOBI_FP64 if precision == capi.FP64 else capi.PCG_BJACOBI_FP32_IR
+    opts = ia.solver_opts(kind, rtol=1e-30, max_iter=iters, inner_rtol=1e-30, max_refine=1)
+    dev.timing(True)
+    dev.timing_reset()
+    dev.solve(x, opts)
+    n, ms = dev.timing_read(capi.KERNEL_SPMV)
+    dev.timing(False)
+    return n, ms
+
+
+def copy_calibration(nbytes=512 << 20, reps=2):
+    """Device-to-device copies of known size (the runtime's blit kernel), the reference for the
+    FETCH_SIZE / WRITE_SIZE counter scale."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def read_traffic(path):
+    """HBM bytes per launch of the element kernel from the committed counter summary (None if absent)."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return int(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +177,9 @@ def main():
     ap.add_argument("--newton-seed", type=int, default=2000, help="unseen seed of the Newton comparison")
     ap.add_argument("--mlp", default=os.path.join(ROOT, "ikarus_amd", "data", "predictor_mlp_32cubed.f32"))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="only the copy calibration and the per-kernel passes (the command profiled by rocprofv3)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "pmc_hex8_assembly_64cubed.txt"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding design is "
@@ -123,53 +192,62 @@ def main():
     dev = ia.Device(p)
     u = 0.005 * np.random.default_rng(0).standard_normal(p.ndof) * (1 - p.fixed)
     dev.set_requirement(u, 1.0)
-    def step():
-        dev.check(capi.lib().ik_matrix(dev.h, capi.DBC_FULL, capi.FP64))
-        dev.check(capi.lib().ik_vector(dev.h, capi.DBC_FULL, None, capi.DEVICE))
+    nbr, nnzb = np_i64(), np_i64()
+    dev.check(capi.lib().ik_matrix_bsr(dev.h, capi.ptr(nbr), capi.ptr(nnzb), None, None, None, None))
+    nnzb = int(nnzb[0]) if int(nnzb[0]) > 0 else None
+
+    if args.profile_pass:
+        copy_calibration()
+        log("[bench] copy calibration done")
+        kernel_pass(dev, p, args.warmup)  # warm up in the same (timed, back-to-back) launch pattern
+        kp = kernel_pass(dev, p, args.steps)
+        s64 = spmv_pass(dev, p, capi.FP64)
+        s32 = spmv_pass(dev, p, capi.FP32)
+        dev.close()
+        print(json.dumps({"profile_pass": True, "steps": args.steps,
+                          "ms_per_launch": {k: round(v[1] / max(v[0], 1), 4) for k, v in kp.items()},
+                          "launches": {k: v[0] for k, v in kp.items()},
+                          "spmv_ms": {"fp64": round(s64[1] / max(s64[0], 1), 4),
+                                      "fp32": round(s32[1] / max(s32[0], 1), 4)}}), flush=True)
+        return
 
     for _ in range(args.warmup):
-        step()
+        assembly_step(dev)
     dev.synchronize()
     log("[bench] warmup done")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        assembly_step(dev)
     dev.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = dt / args.steps * 1e3
     value = p.ne * args.steps / dt / 1e6
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} Melem/s")
 
-    # dominant kernel: tangent assembly, timed with HIP events on the library's stream
-    dev.timing(True)
-    dev.timing_reset()
-    for _ in range(args.steps):
-        step()
-    nk, msk = dev.timing_read(capi.KERNEL_ASSEMBLE_KR)
-    nr, msr = dev.timing_read(capi.KERNEL_ASSEMBLE_R)
-    dev.timing(False)
-    k_ms = msk / max(nk, 1)                      # average element-kernel launch (one element chunk)
-    elems_per_launch = p.ne * args.steps / max(nk, 1)
+    kp = kernel_pass(dev, p, args.steps)
+    ne_, k_ms = kp["element"][0], kp["element"][1] / max(kp["element"][0], 1)
+    elems_per_launch = p.ne * args.steps / max(ne_, 1)
     achieved = FLOP_PER_ELEM_KR * elems_per_launch / (k_ms * 1e-3) / 1e12
-    ng, msg = dev.timing_read(capi.KERNEL_GATHER_K)
-    # SpMV bandwidth: algorithmic bytes 8*nnz + 4*nnzb + 4*(nn+1) + 16*ndof
-    nbr, nnzb = np_i64(), np_i64()
-    dev.check(capi.lib().ik_matrix_bsr(dev.h, capi.ptr(nbr), capi.ptr(nnzb), None, None, None, None))
-    nnzb = int(nnzb[0])
-    x = np.random.default_rng(1).standard_normal(p.ndof)
-    dev.timing(True)
-    dev.timing_reset()
-    opts = ia.solver_opts(capi.PCG_JACOBI_FP64, rtol=1e-30, max_iter=50)
-    dev.solve(x, opts)
-    ns, mss = dev.timing_read(capi.KERNEL_SPMV)
-    dev.timing(False)
-    spmv_ms = mss / max(ns, 1)
-    spmv_bytes = 8 * 9 * nnzb + 4 * nnzb + 4 * (p.X.shape[0] + 1) + 16 * p.ndof
-    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    ng, g_ms_tot = kp["gather"]
+    g_ms = g_ms_tot / max(ng, 1)
+    r_ms = kp["residual"][1] / max(kp["residual"][0], 1)
+    nnzb = nnzb or 0
+    nup = (nnzb + p.X.shape[0]) // 2
+    # gather (scatter into the CSR): staged element blocks read, K written, per-upper-block indices
+    gather_bytes = (2592 * p.ne + 72 * nnzb + 20 * nup) / max(ng / args.steps, 1)
+    gather_gbs = gather_bytes / (g_ms * 1e-3) / 1e9
+    s64n, s64ms = spmv_pass(dev, p, capi.FP64)
+    s32n, s32ms = spmv_pass(dev, p, capi.FP32)
+    spmv64_ms, spmv32_ms = s64ms / max(s64n, 1), s32ms / max(s32n, 1)
+    spmv64_bytes = 8 * 9 * nnzb + 4 * nnzb + 4 * (p.X.shape[0] + 1) + 16 * p.ndof
+    spmv32_bytes = 4 * 9 * nnzb + 4 * nnzb + 4 * (p.X.shape[0] + 1) + 8 * p.ndof
+    spmv64_gbs = spmv64_bytes / (spmv64_ms * 1e-3) / 1e9
+    spmv32_gbs = spmv32_bytes / (spmv32_ms * 1e-3) / 1e9
     dev.close()
-    log(f"[bench] element kernel {k_ms:.3f} ms/launch x {nk / args.steps:.0f} launches ({achieved:.1f} TFLOP/s "
-        f"algorithmic), K gather {msg / max(ng, 1):.3f} ms/launch, R kernel {msr / max(nr, 1):.3f} ms/launch, "
-        f"SpMV {spmv_ms:.3f} ms ({spmv_gbs:.0f} GB/s)")
+    log(f"[bench] element kernel {k_ms:.4f} ms/launch x {ne_ / args.steps:.0f} launches ({achieved:.1f} TFLOP/s "
+        f"algorithmic), gather {g_ms:.4f} ms/launch ({gather_gbs:.0f} GB/s), residual {r_ms:.4f} ms, "
+        f"SpMV fp64 {spmv64_ms:.4f} ms ({spmv64_gbs:.0f} GB/s), fp32 {spmv32_ms:.4f} ms ({spmv32_gbs:.0f} GB/s)")
+    traffic = read_traffic(args.traffic)
 
     out = {
         "metric": "Newton solve time to ||R||<=1e-8 (mixed+learned vs FP64); assembly Melem/s",
@@ -184,19 +262,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"hex8 Neo-Hooke K+R assembly, {n}x{n}x{n} elements, unit cube, 5% jitter seed 0, "
-                               "FULL Dirichlet, 3x3-block CSR", "elements": p.ne, "dofs": p.ndof,
-                   "block_nnz": nnzb, "parallelism": "single GPU"},
+        "config": {"workload": f"hex8 Neo-Hooke R+K assembly (one Newton iteration), {n}x{n}x{n} elements, unit "
+                               "cube, 5% jitter seed 0, FULL Dirichlet, 3x3-block CSR", "elements": p.ne,
+                   "dofs": p.ndof, "block_nnz": nnzb, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel": "hex8Element<double,NeoHooke,K>", "kernel_ms_per_launch": round(k_ms, 4),
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "hex8Element<NeoHooke,double,K>", "kernel_ms_per_launch": round(k_ms, 4),
                      "elements_per_launch": int(elems_per_launch), "flop_per_elem": FLOP_PER_ELEM_KR},
         "assembly_breakdown_ms_per_step": {
-            "element": round(msk / args.steps, 4), "gather_K": round(msg / args.steps, 4),
-            "residual": round(msr / args.steps, 4)},
-        "spmv": {"ms": round(spmv_ms, 4), "GBs": round(spmv_gbs, 1), "frac_hbm": round(spmv_gbs / PEAK_HBM_GBS, 4),
-                 "bytes": spmv_bytes},
-        "residual_kernel_ms": round(msr / max(nr, 1), 4),
+            "element_K": round(kp["element"][1] / args.steps, 4), "gather_K": round(g_ms_tot / args.steps, 4),
+            "residual": round(kp["residual"][1] / args.steps, 4)},
+        "gather": {"ms_per_launch": round(g_ms, 4), "GBs": round(gather_gbs, 1),
+                   "frac_hbm": round(gather_gbs / PEAK_HBM_GBS, 4), "bytes_per_launch": int(gather_bytes)},
+        "spmv": {"fp64": {"ms": round(spmv64_ms, 4), "GBs": round(spmv64_gbs, 1),
+                          "frac_hbm": round(spmv64_gbs / PEAK_HBM_GBS, 4), "bytes": spmv64_bytes},
+                 "fp32": {"ms": round(spmv32_ms, 4), "GBs": round(spmv32_gbs, 1),
+                          "frac_hbm": round(spmv32_gbs / PEAK_HBM_GBS, 4), "bytes": spmv32_bytes}},
     }
     if not args.no_newton:
         pn = pb.hex_block((n, n, n), seed=args.newton_seed)

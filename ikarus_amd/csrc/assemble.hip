@@ -178,7 +178,7 @@ constexpr int NSYM = 6;
 // nodes 4 alpha..+3 x 4 beta..+3; (0,1) for all kl, (0,0) and (1,1) for k <= l (the rest follow from
 // K_ab[k][l] = K_ba[l][k]): 21 tiles x (2 gauss blocks x (2 W terms + 3 I)) contraction steps.
 template <int KIND, typename TK, bool WANT_K, bool WANT_R>
-__global__ __launch_bounds__(64, 2) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
+__global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
                                                      const int32_t* __restrict__ conn, int64_t e0, int64_t count,
                                                      double lam, double mu, const int32_t* __restrict__ dest,
                                                      TK* __restrict__ Kebe, double* __restrict__ Rebe,
@@ -652,41 +652,59 @@ static void checkErrFlag(ik_handle* h) {
   }
 }
 
-// Element chunks on the main stream; each chunk's gathers on the second stream, released by an event
-// recorded behind the chunk, so gathers (memory-bound) overlap the next chunks (compute-bound).
+static void launchChunk(ik_handle* h, int64_t e0, int64_t e1, bool wantR, void* Kebe, bool fp32) {
+  if (h->neas)
+    easStage(h, e0, e1 - e0, true, h->bdest.p);
+  else if (fp32)
+    launchElemMat<float, true, false>(h, e0, e1 - e0, h->bdest.p, (float*)Kebe, nullptr);
+  else if (wantR)
+    launchElemMat<double, true, true>(h, e0, e1 - e0, h->bdest.p, (double*)Kebe, h->ebeR.p);
+  else
+    launchElemMat<double, true, false>(h, e0, e1 - e0, h->bdest.p, (double*)Kebe, nullptr);
+}
+
 template <typename T>
-static void runChunks(ik_handle* h, int dbc, bool wantK, bool wantR, T* Kebe, T* K) {
+static void launchGatherK(ik_handle* h, int64_t r0, int64_t r1, int mask, const T* Kebe, T* K, hipStream_t s) {
+  const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
+  hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)(((q1 - q0) * 9 + 255) / 256)), dim3(256), 0, s, h->segptr.p,
+                     h->upBlk.p, h->upT.p, h->blockRow.p, h->colidx.p, h->fixed.p, mask, Kebe, q0, q1, K);
+}
+
+// Tangent assembly. Element chunks run on the main stream; each chunk's gathers run on the second stream,
+// released by an event recorded behind the chunk, so the gathers of one chunk overlap the element work of
+// the next while its staged blocks are still cached. In timing mode the chunks and then the gathers run
+// back to back on the main stream, each group between one pair of events, giving stand-alone durations.
+template <typename T>
+static void runChunks(ik_handle* h, int dbc, bool wantR, T* Kebe, T* K) {
   const int mask = dbc == IK_DBC_RAW ? 0 : 1;
+  const bool fp32 = sizeof(T) == 4;
   const size_t nc = h->chunkElem.size() - 1;
+  if (h->timing) {
+    {
+      ScopedKernelTimer t(h, KASM_KR, h->stream, (int64_t)nc);
+      for (size_t c = 0; c < nc; ++c) launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], wantR, Kebe, fp32);
+    }
+    {
+      ScopedKernelTimer t(h, KGATHER, h->stream, (int64_t)nc);
+      for (size_t c = 0; c < nc; ++c) launchGatherK<T>(h, h->chunkRow[c], h->chunkRow[c + 1], mask, Kebe, K, h->stream);
+    }
+    if (wantR)
+      hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
+                         h->nclist.p, h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
+    return;
+  }
   while (h->chunkEvents.size() < nc) {
     hipEvent_t ev;
     IK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     h->chunkEvents.push_back(ev);
   }
   for (size_t c = 0; c < nc; ++c) {
-    const int64_t e0 = h->chunkElem[c], e1 = h->chunkElem[c + 1];
-    {
-      ScopedKernelTimer t(h, wantK ? KASM_KR : KASM_R, h->stream);
-      if (h->neas)
-        easStage(h, e0, e1 - e0, wantK, h->bdest.p);
-      else if (wantK && wantR)
-        launchElemMat<T, true, true>(h, e0, e1 - e0, h->bdest.p, Kebe, h->ebeR.p);
-      else if (wantK)
-        launchElemMat<T, true, false>(h, e0, e1 - e0, h->bdest.p, Kebe, nullptr);
-      else
-        launchElemMat<double, false, true>(h, e0, e1 - e0, nullptr, (double*)nullptr, h->ebeR.p);
-    }
+    launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], wantR, Kebe, fp32);
     const int64_t r0 = h->chunkRow[c], r1 = h->chunkRow[c + 1];
     if (r1 <= r0) continue;
     IK_HIP(hipEventRecord(h->chunkEvents[c], h->stream));
     IK_HIP(hipStreamWaitEvent(h->stream2, h->chunkEvents[c], 0));
-    if (wantK) {
-      const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
-      ScopedKernelTimer t(h, KGATHER, h->stream2);
-      hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)(((q1 - q0) * 9 + 255) / 256)), dim3(256), 0, h->stream2,
-                         h->segptr.p, h->upBlk.p, h->upT.p, h->blockRow.p, h->colidx.p, h->fixed.p, mask, Kebe, q0,
-                         q1, K);
-    }
+    launchGatherK<T>(h, r0, r1, mask, Kebe, K, h->stream2);
     if (wantR)
       hipLaunchKernelGGL(gatherR, dim3((unsigned)(((r1 - r0) * 3 + 255) / 256)), dim3(256), 0, h->stream2,
                          h->ncptr.p, h->nclist.p, h->fixed.p, mask, h->ebeR.p, r0, r1, h->R.p);
@@ -709,11 +727,11 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
   if (wantK && precision == IK_FP64) {
     h->K64.ensure((size_t)h->nnzb * 9);
     h->ebeK64.ensure((size_t)h->ne * 324);
-    runChunks<double>(h, dbc, true, fused, h->ebeK64.p, h->K64.p);
+    runChunks<double>(h, dbc, fused, h->ebeK64.p, h->K64.p);
   } else if (wantK) {
     h->K32.ensure((size_t)h->nnzb * 9);
     h->ebeK32.ensure((size_t)h->ne * 324);
-    runChunks<float>(h, dbc, true, false, h->ebeK32.p, h->K32.p);
+    runChunks<float>(h, dbc, false, h->ebeK32.p, h->K32.p);
   }
   if (wantR && !fused) {
     // residual alone: one element launch over the mesh, then one gather (a few MB; no staging overlap needed)

@@ -191,11 +191,14 @@ void mlpPredict(ik_handle* h, const double* duN, const double* duN1, double lamN
                 double dlamPrev2);
 
 // timing helpers (events on the handle's stream around one launch)
+// HIP events around `launches` back-to-back launches of one kernel on stream s (timing mode only)
 struct ScopedKernelTimer {
   ik_handle* h;
   int id;
   hipStream_t s;
-  ScopedKernelTimer(ik_handle* hh, int kid, hipStream_t ss = nullptr) : h(hh), id(kid), s(ss ? ss : hh->stream) {
+  int64_t launches;
+  ScopedKernelTimer(ik_handle* hh, int kid, hipStream_t ss = nullptr, int64_t n = 1)
+      : h(hh), id(kid), s(ss ? ss : hh->stream), launches(n) {
     if (h->timing) IK_HIP(hipEventRecord(h->ev0, s));
   }
   ~ScopedKernelTimer() {
@@ -205,7 +208,7 @@ struct ScopedKernelTimer {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
     h->tMs[id] += ms;
-    h->tLaunches[id] += 1;
+    h->tLaunches[id] += launches;
   }
 };
 }  // namespace ik
