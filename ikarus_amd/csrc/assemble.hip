@@ -181,8 +181,8 @@ template <int KIND, typename TK, bool WANT_K, bool WANT_R>
 __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
                                                      const int32_t* __restrict__ conn, int64_t e0, int64_t count,
                                                      double lam, double mu, const int32_t* __restrict__ dest,
-                                                     TK* __restrict__ Kebe, double* __restrict__ Rebe,
-                                                     unsigned long long* __restrict__ err) {
+                                                     const int32_t* __restrict__ rdest, TK* __restrict__ Kebe,
+                                                     double* __restrict__ Rebe, unsigned long long* __restrict__ err) {
   const int lane     = threadIdx.x;
   const int kk       = lane >> 4, bk = (lane >> 2) & 3, i = lane & 3;
   const int64_t base = (int64_t)blockIdx.x * EPW;
@@ -344,7 +344,8 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) Rebe[e * 24 + 3 * (i + 4 * h) + k] = rr[h][k];
+        for (int k = 0; k < 3; ++k)
+          Rebe[(rdest ? (int64_t)rdest[e * 8 + i + 4 * h] * 3 : e * 24 + 3 * (i + 4 * h)) + k] = rr[h][k];
     }
   }
   if constexpr (WANT_K) {
@@ -404,8 +405,8 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
 template <int KIND>
 __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X, const double* __restrict__ U,
                                                    const int32_t* __restrict__ conn, int64_t e0, int64_t count,
-                                                   double lam, double mu, double* __restrict__ Rebe,
-                                                   unsigned long long* __restrict__ err) {
+                                                   double lam, double mu, const int32_t* __restrict__ rdest,
+                                                   double* __restrict__ Rebe, unsigned long long* __restrict__ err) {
   __shared__ double XU[8][8][6];
   __shared__ double part[8][8][25];  // [element][gauss point][3a+k] (+1 pad)
   const int lane     = threadIdx.x;
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
       double acc = 0.0;
 #pragma unroll
       for (int g = 0; g < 8; ++g) acc += part[el][g][3 * q + k];
-      Rebe[e * 24 + 3 * q + k] = acc;
+      Rebe[(rdest ? (int64_t)rdest[e * 8 + q] * 3 : e * 24 + 3 * q) + k] = acc;
     }
   }
 }
@@ -518,10 +519,10 @@ __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segpt
   if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = acc;
 }
 
-// R gather: thread per DOF over node rows [r0, r1)
-__global__ __launch_bounds__(256) void gatherR(const int32_t* __restrict__ ncptr, const int32_t* __restrict__ nclist,
-                                               const uint8_t* __restrict__ fixed, int maskDbc,
-                                               const double* __restrict__ Rebe, int64_t r0, int64_t r1,
+// R gather: thread per DOF over node rows [r0, r1); the node's staged corner contributions occupy the
+// contiguous slots [ncptr[n], ncptr[n+1]) (ordered by element) and are summed in that order.
+__global__ __launch_bounds__(256) void gatherR(const int32_t* __restrict__ ncptr, const uint8_t* __restrict__ fixed,
+                                               int maskDbc, const double* __restrict__ Rst, int64_t r0, int64_t r1,
                                                double* __restrict__ R) {
   const int64_t d = r0 * 3 + (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (d >= r1 * 3) return;
@@ -531,10 +532,15 @@ __global__ __launch_bounds__(256) void gatherR(const int32_t* __restrict__ ncptr
   }
   const int64_t node = d / 3;
   const int k        = (int)(d - node * 3);
+  const int32_t c1   = ncptr[node + 1];
   double acc         = 0.0;
-  for (int32_t c = ncptr[node]; c < ncptr[node + 1]; ++c) {
-    const int32_t code = nclist[c];  // e*8 + a
-    acc += Rebe[(int64_t)(code >> 3) * 24 + (code & 7) * 3 + k];
+  for (int32_t c = ncptr[node]; c < c1; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = c + j < c1 ? Rst[(int64_t)(c + j) * 3 + k] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (c + j < c1) acc += v[j];
   }
   R[d] = acc;
 }
@@ -610,24 +616,26 @@ inline unsigned gridFor(int64_t n, int bs) {
 }
 
 template <typename T, int KIND, bool WK, bool WR>
-void launchElem(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, T* Kebe, double* Rebe) {
+void launchElem(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, T* Kebe,
+                double* Rebe) {
   if constexpr (!WK) {
     const unsigned grid = (unsigned)((count + 7) / 8);
     hipLaunchKernelGGL((hex8Residual<KIND>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p, h->conn.p, e0,
-                       count, h->lam, h->mu, Rebe, h->errFlag.p);
+                       count, h->lam, h->mu, rdest, Rebe, h->errFlag.p);
   } else {
     const unsigned grid = (unsigned)((count + EPW - 1) / EPW);
     hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p,
-                       h->conn.p, e0, count, h->lam, h->mu, dest, Kebe, Rebe, h->errFlag.p);
+                       h->conn.p, e0, count, h->lam, h->mu, dest, rdest, Kebe, Rebe, h->errFlag.p);
   }
 }
 
 template <typename T, bool WK, bool WR>
-void launchElemMat(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, T* Kebe, double* Rebe) {
+void launchElemMat(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, T* Kebe,
+                   double* Rebe) {
   if (h->material == IK_SVK)
-    launchElem<T, IK_SVK, WK, WR>(h, e0, count, dest, Kebe, Rebe);
+    launchElem<T, IK_SVK, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe);
   else
-    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, dest, Kebe, Rebe);
+    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe);
 }
 
 }  // namespace
@@ -654,13 +662,13 @@ static void checkErrFlag(ik_handle* h) {
 
 static void launchChunk(ik_handle* h, int64_t e0, int64_t e1, bool wantR, void* Kebe, bool fp32) {
   if (h->neas)
-    easStage(h, e0, e1 - e0, true, h->bdest.p);
+    easStage(h, e0, e1 - e0, true, h->bdest.p, h->rdest.p);
   else if (fp32)
-    launchElemMat<float, true, false>(h, e0, e1 - e0, h->bdest.p, (float*)Kebe, nullptr);
+    launchElemMat<float, true, false>(h, e0, e1 - e0, h->bdest.p, nullptr, (float*)Kebe, nullptr);
   else if (wantR)
-    launchElemMat<double, true, true>(h, e0, e1 - e0, h->bdest.p, (double*)Kebe, h->ebeR.p);
+    launchElemMat<double, true, true>(h, e0, e1 - e0, h->bdest.p, h->rdest.p, (double*)Kebe, h->ebeR.p);
   else
-    launchElemMat<double, true, false>(h, e0, e1 - e0, h->bdest.p, (double*)Kebe, nullptr);
+    launchElemMat<double, true, false>(h, e0, e1 - e0, h->bdest.p, nullptr, (double*)Kebe, nullptr);
 }
 
 template <typename T>
@@ -690,7 +698,7 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, T* Kebe, T* K) {
     }
     if (wantR)
       hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
-                         h->nclist.p, h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
+                         h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
     return;
   }
   while (h->chunkEvents.size() < nc) {
@@ -707,7 +715,7 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, T* Kebe, T* K) {
     launchGatherK<T>(h, r0, r1, mask, Kebe, K, h->stream2);
     if (wantR)
       hipLaunchKernelGGL(gatherR, dim3((unsigned)(((r1 - r0) * 3 + 255) / 256)), dim3(256), 0, h->stream2,
-                         h->ncptr.p, h->nclist.p, h->fixed.p, mask, h->ebeR.p, r0, r1, h->R.p);
+                         h->ncptr.p, h->fixed.p, mask, h->ebeR.p, r0, r1, h->R.p);
   }
   // join: later work on the main stream sees every gathered row
   IK_HIP(hipEventRecord(h->joinEvent, h->stream2));
@@ -738,12 +746,12 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
     {
       ScopedKernelTimer t(h, KASM_R, h->stream);
       if (h->neas)
-        easStage(h, 0, h->ne, false, nullptr);
+        easStage(h, 0, h->ne, false, nullptr, h->rdest.p);
       else
-        launchElemMat<double, false, true>(h, 0, h->ne, nullptr, (double*)nullptr, h->ebeR.p);
+        launchElemMat<double, false, true>(h, 0, h->ne, nullptr, h->rdest.p, (double*)nullptr, h->ebeR.p);
     }
     hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
-                       h->nclist.p, h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
+                       h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
   }
   if (h->neas) h->easCacheValid = true;
   if (wantR && h->nfaces) {
@@ -763,9 +771,9 @@ void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re) {
   h->ebeK64.ensure((size_t)h->ne * 324);
   h->ebeR.ensure((size_t)h->ne * 24);
   if (h->neas)
-    easStage(h, e, 1, true, nullptr);
+    easStage(h, e, 1, true, nullptr, nullptr);
   else
-    launchElemMat<double, true, true>(h, e, 1, nullptr, h->ebeK64.p, h->ebeR.p);
+    launchElemMat<double, true, true>(h, e, 1, nullptr, nullptr, h->ebeK64.p, h->ebeR.p);
   checkErrFlag(h);
   DevBuf<double> kd, rd;
   kd.ensure(576);

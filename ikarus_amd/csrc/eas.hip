@@ -160,7 +160,8 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
                                                  const int32_t* __restrict__ conn,
                                                  const double* __restrict__ alpha, int64_t e0, double lam,
                                                  double mu, const int32_t* __restrict__ dest,
-                                                 double* __restrict__ Kebe, double* __restrict__ Rebe,
+                                                 const int32_t* __restrict__ rdest, double* __restrict__ Kebe,
+                                                 double* __restrict__ Rebe,
                                                  double* __restrict__ cache, unsigned long long* __restrict__ err) {
   __shared__ EasLds s;
   const int t     = threadIdx.x;
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     const int r = t - 40;
     double acc  = s.Ru[r];
     for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * s.Y[i][24];
-    Rebe[e * 24 + r] = acc;
+    Rebe[rdest ? (int64_t)rdest[e * 8 + r / 3] * 3 + r % 3 : e * 24 + r] = acc;
   }
 }
 
@@ -496,25 +497,25 @@ __global__ __launch_bounds__(256) void easUpdateKernel(const double* __restrict_
 }
 
 template <bool WK>
-void launchEas(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, double* Kebe) {
+void launchEas(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, double* Kebe) {
   if (h->material == IK_SVK)
     hipLaunchKernelGGL((easElement<IK_SVK, WK>), dim3((unsigned)count), dim3(64), 0, h->stream, h->X.p, h->u.p,
-                       h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, Kebe, h->ebeR.p, h->easCache.p,
+                       h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, rdest, Kebe, h->ebeR.p, h->easCache.p,
                        h->errFlag.p);
   else
     hipLaunchKernelGGL((easElement<IK_NEOHOOKE, WK>), dim3((unsigned)count), dim3(64), 0, h->stream, h->X.p,
-                       h->u.p, h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, Kebe, h->ebeR.p, h->easCache.p,
+                       h->u.p, h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, rdest, Kebe, h->ebeR.p, h->easCache.p,
                        h->errFlag.p);
 }
 
 }  // namespace
 
-void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t* dest) {
+void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t* dest, const int32_t* rdest) {
   h->easCache.ensure((size_t)h->ne * (NE * 24 + NE));
   if (wantK)
-    launchEas<true>(h, e0, count, dest, h->ebeK64.p);
+    launchEas<true>(h, e0, count, dest, rdest, h->ebeK64.p);
   else
-    launchEas<false>(h, e0, count, nullptr, nullptr);
+    launchEas<false>(h, e0, count, nullptr, rdest, nullptr);
 }
 
 void easUpdate(ik_handle* h, const double* duDev) {

@@ -116,9 +116,10 @@ struct ik_handle {
   std::vector<int64_t> huptr;
   ik::DevBuf<int32_t> upBlk, upT;
   // staging order of the element blocks: upper block by upper block, each segment ordered by element.
-  // bdest[e*36+sb] = (slot << 1) | transposed, segptr[q] = first slot of upper block q; for every node,
-  // its (element, corner) incidences (code = e*8+a), ordered by element
-  ik::DevBuf<int32_t> bdest, segptr, ncptr, nclist;
+  // bdest[e*36+sb] = (slot << 1) | transposed, segptr[q] = first slot of upper block q. Residual staging:
+  // node by node, each node's (element, corner) incidences ordered by element; rdest[e*8+a] = slot,
+  // ncptr[n] = first slot of node n
+  ik::DevBuf<int32_t> bdest, segptr, ncptr, rdest;
   // element chunks and the block rows that are complete after each chunk
   std::vector<int64_t> chunkElem, chunkRow;
   // element-by-element staging of K_e (36 upper 3x3 blocks) and R_e (24)
@@ -171,8 +172,8 @@ void launchZeroF(float* p, int64_t n, hipStream_t s);
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR);
 void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re);
 void easUpdate(ik_handle* h, const double* duDev);
-// element blocks go to staging slots dest[e*36+sb] (nullptr: element-by-element order)
-void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t* dest);
+// element blocks go to staging slots dest[e*36+sb], corner residuals to rdest[e*8+a] (nullptr: element order)
+void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t* dest, const int32_t* rdest);
 // sparse.hip
 void spmv64(ik_handle* h, const double* x, double* y);
 void spmv32(ik_handle* h, const float* x, float* y);

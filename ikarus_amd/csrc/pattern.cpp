@@ -147,8 +147,11 @@ void buildPattern(ik_handle* h) {
   h->upT.upload(upT.data(), upT.size(), h->stream);
   h->bdest.upload(dest.data(), dest.size(), h->stream);
   h->segptr.upload(seg.data(), seg.size(), h->stream);
+  // residual staging slot of every element corner: its position in the node's incidence list
+  std::vector<int32_t> rdest((size_t)ne * 8);
+  for (size_t j = 0; j < nlist.size(); ++j) rdest[nlist[j]] = (int32_t)j;
   h->ncptr.upload(ncnt.data(), ncnt.size(), h->stream);
-  h->nclist.upload(nlist.data(), nlist.size(), h->stream);
+  h->rdest.upload(rdest.data(), rdest.size(), h->stream);
   IK_HIP(hipStreamSynchronize(h->stream));
   h->patternBuilt = true;
 }

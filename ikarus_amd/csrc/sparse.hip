@@ -41,9 +41,33 @@ __device__ inline double blockSum(double v) {
 
 // y = K x over 3x3 block rows; optional fused partial dot d = sum_i x_i y_i per block.
 // Lane (r, m) of a wavefront: row r = lane / 9 (7 rows per wavefront, lane 63 idle), entry m = 3k + l of
-// every 3x3 block in the row. Each load instruction reads 7 contiguous 72-byte block runs; the three
-// partial sums of row k are combined with two lane shifts.
+// every 3x3 block in the row. Each load instruction reads 7 contiguous 72-byte block runs; the loads of
+// 9 blocks (a third of an interior 27-block row) are issued before their products are summed, which keeps
+// enough bytes in flight per lane to stream K near the HBM rate; the three partial sums of row k are
+// combined with two lane shifts.
 constexpr int ROWS_PER_WAVE = 7;
+constexpr int SPMV_UNROLL   = 9;
+
+// acc += sum over the row's blocks of V[j*9] * x[C[j]*3 + l] (in block order)
+template <typename TV, typename TX, typename TA>
+__device__ inline TA rowDot(const TV* __restrict__ V, const int32_t* __restrict__ C, int n, const TX* __restrict__ x,
+                            int l) {
+  TA acc = TA(0);
+  int j  = 0;
+  for (; j + SPMV_UNROLL <= n; j += SPMV_UNROLL) {
+    int32_t c[SPMV_UNROLL];
+    TV v[SPMV_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SPMV_UNROLL; ++u) {
+      c[u] = C[j + u];
+      v[u] = V[(int64_t)(j + u) * 9];
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_UNROLL; ++u) acc += (TA)v[u] * (TA)x[(int64_t)c[u] * 3 + l];
+  }
+  for (; j < n; ++j) acc += (TA)V[(int64_t)j * 9] * (TA)x[(int64_t)C[j] * 3 + l];
+  return acc;
+}
 template <typename TV, typename TX, bool DOT>
 __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict__ rowptr,
                                                       const int32_t* __restrict__ colidx,
@@ -61,17 +85,7 @@ __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict_
   TX acc = TX(0);
   if (active) {
     const int32_t rb = rowptr[row], re = rowptr[row + 1];
-    const TV* V       = val + (int64_t)rb * 9 + m;
-    const int32_t* C  = colidx + rb;
-    const int n       = re - rb;
-    int j             = 0;
-    for (; j + 2 <= n; j += 2) {
-      const int32_t c0 = C[j], c1 = C[j + 1];
-      const TV v0 = V[(int64_t)j * 9], v1 = V[(int64_t)(j + 1) * 9];
-      acc += (TX)v0 * x[(int64_t)c0 * 3 + l];
-      acc += (TX)v1 * x[(int64_t)c1 * 3 + l];
-    }
-    if (j < n) acc += (TX)V[(int64_t)j * 9] * x[(int64_t)C[j] * 3 + l];
+    acc = rowDot<TV, TX, TX>(val + (int64_t)rb * 9 + m, colidx + rb, re - rb, x, l);
   }
   const TX a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
   double d = 0.0;
@@ -103,10 +117,7 @@ __global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restr
   double acc = 0.0;
   if (active) {
     const int32_t rb = rowptr[row], re = rowptr[row + 1];
-    const TV* V       = val + (int64_t)rb * 9 + m;
-    for (int32_t j = rb; j < re; ++j) {
-      acc += (double)V[(int64_t)(j - rb) * 9] * x[(int64_t)colidx[j] * 3 + l];
-    }
+    acc = rowDot<TV, double, double>(val + (int64_t)rb * 9 + m, colidx + rb, re - rb, x, l);
   }
   const double a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
   double d = 0.0;
