@@ -197,6 +197,7 @@ __global__ __launch_bounds__(1024) void pcgInitFold(const double* __restrict__ p
     sc->bb        = rr;
     sc->rr        = rr;
     sc->rz        = rz;
+    sc->rz_new    = rz;
     const double thr = tol2 * rr;
     sc->threshold = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
     sc->iters     = 0;
@@ -212,6 +213,7 @@ __global__ __launch_bounds__(1024) void pcgAlphaFold(const double* __restrict__ 
   for (int i = threadIdx.x; i < nb; i += 1024) s += partial[i];
   s = blockSum<1024>(s);
   if (threadIdx.x == 0) {
+    sc->rz    = sc->rz_new;  // r.z of the current iterate (set by the previous p update)
     sc->pq    = s;
     sc->alpha = sc->rz / s;
   }
@@ -259,36 +261,45 @@ __global__ __launch_bounds__(256) void pcgUpdateKernel(T* __restrict__ x, T* __r
   }
 }
 
-__global__ __launch_bounds__(1024) void pcgBetaFold(const double* __restrict__ partial, int nb,
-                                                    PcgScalars* __restrict__ sc, int maxIter) {
+// Convergence test, beta and p = M^-1 r + beta p in one launch. Every block folds the update kernel's
+// partials (rr, rz) itself in the same fixed order, so all blocks agree bitwise; block 0 records the
+// scalars. beta = rz_new / rz uses the r.z of the previous iterate, which only pcgAlphaFold writes.
+template <typename T>
+__global__ __launch_bounds__(256) void pcgBetaPKernel(T* __restrict__ p, const T* __restrict__ r,
+                                                      const T* __restrict__ pre, int blockPre, int64_t n,
+                                                      const double* __restrict__ partial, int nb,
+                                                      PcgScalars* __restrict__ sc, int maxIter) {
   if (sc->done) return;
+  __shared__ double bc[2];
   double rr = 0.0, rz = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 1024) {
+  for (int i = threadIdx.x; i < nb; i += 256) {
     rr += partial[i];
     rz += partial[nb + i];
   }
-  rr = blockSum<1024>(rr);
-  rz = blockSum<1024>(rz);
+  rr = blockSum<256>(rr);
+  rz = blockSum<256>(rz);
   if (threadIdx.x == 0) {
+    bc[0] = rr;
+    bc[1] = rz;
+  }
+  __syncthreads();
+  rr              = bc[0];
+  rz              = bc[1];
+  const bool conv = rr < sc->threshold;
+  const double bt = rz / sc->rz;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->rr = rr;
-    if (rr < sc->threshold) {
+    if (conv) {
       sc->done = 1;
     } else {
-      sc->beta   = rz / sc->rz;
-      sc->rz     = rz;
+      sc->beta   = bt;
       sc->rz_new = rz;
       sc->iters += 1;
       if (sc->iters >= maxIter) sc->done = 2;
     }
   }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void pcgPKernel(T* __restrict__ p, const T* __restrict__ r,
-                                                  const T* __restrict__ pre, int blockPre, int64_t n,
-                                                  const PcgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  const T beta = (T)sc->beta;
+  if (conv) return;
+  const T beta = (T)bt;
   if (!blockPre) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
       p[i] = pre[i] * r[i] + beta * p[i];
@@ -453,9 +464,8 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
       hipLaunchKernelGGL(pcgAlphaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, (int)sgrid, h->sc.p);
       hipLaunchKernelGGL((pcgUpdateKernel<double>), dim3(g), dim3(256), 0, h->stream, x, h->pr.p, h->pp.p, h->pq.p,
                          h->pdiag.p, 0, n, h->sc.p, h->partial.p);
-      hipLaunchKernelGGL(pcgBetaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, g, h->sc.p, maxIter);
-      hipLaunchKernelGGL((pcgPKernel<double>), dim3(g), dim3(256), 0, h->stream, h->pp.p, h->pr.p, h->pdiag.p, 0, n,
-                         h->sc.p);
+      hipLaunchKernelGGL((pcgBetaPKernel<double>), dim3(g), dim3(256), 0, h->stream, h->pp.p, h->pr.p, h->pdiag.p, 0,
+                         n, h->partial.p, g, h->sc.p, maxIter);
     }
     IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
     IK_HIP(hipStreamSynchronize(h->stream));
@@ -511,9 +521,8 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
         hipLaunchKernelGGL(pcgAlphaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, (int)sgrid, h->sc.p);
         hipLaunchKernelGGL((pcgUpdateKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fx.p, h->fr.p, h->fp.p,
                            h->fq.p, h->fdiag.p, 1, n, h->sc.p, h->partial.p);
-        hipLaunchKernelGGL(pcgBetaFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, gn, h->sc.p, maxIter);
-        hipLaunchKernelGGL((pcgPKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fp.p, h->fr.p, h->fdiag.p, 1,
-                           n, h->sc.p);
+        hipLaunchKernelGGL((pcgBetaPKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fp.p, h->fr.p, h->fdiag.p,
+                           1, n, h->partial.p, gn, h->sc.p, maxIter);
       }
       IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
       IK_HIP(hipStreamSynchronize(h->stream));
