@@ -179,8 +179,8 @@ def main(argv=None):
     tr.add_argument("--paths", type=int, default=256)
     tr.add_argument("--n", type=int, default=32)
     tr.add_argument("--steps", type=int, default=40)
-    tr.add_argument("--nodes-per-step", type=int, default=512)
-    tr.add_argument("--epochs", type=int, default=20)
+    tr.add_argument("--nodes-per-step", type=int, default=1024)
+    tr.add_argument("--epochs", type=int, default=40)
     tr.add_argument("--seed0", type=int, default=1000)
     tr.add_argument("--out", default=DEFAULT_WEIGHTS)
     args = ap.parse_args(argv)
