@@ -74,6 +74,9 @@ NEWTON_MODES = {
     # mixed form + learned predictor
     "mixed_mlp": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
                       pred=capi.PRED_MLP),
+    # precision alone: the plain form's Newton and predictor with the FP32 tangent and refined FP32 PCG
+    "mixed_exact": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.0,
+                        pred=capi.PRED_CONSTANT),
 }
 
 
@@ -180,11 +183,21 @@ def main():
     ap.add_argument("--profile-pass", action="store_true",
                     help="only the copy calibration and the per-kernel passes (the command profiled by rocprofv3)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "pmc_hex8_assembly_64cubed.txt"))
+    ap.add_argument("--config", choices=("c3", "c4"), default=None,
+                    help="run the 128x128x64 precision variant (c3) or the 96x96x48 EAS workload (c4) instead")
+    ap.add_argument("--c3-steps", type=int, default=5)
+    ap.add_argument("--c4-steps", type=int, default=5, help="timed R+K assemblies of the EAS workload")
+    ap.add_argument("--c4-load-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding design is "
               "described in DESIGN.md, nothing here uses a second GPU.", file=sys.stderr)
         sys.exit(2)
+
+    if args.config:
+        out = config_precision(args) if args.config == "c3" else config_eas(args)
+        print(json.dumps(out), flush=True)
+        return
 
     n = args.n
     p = pb.hex_block((n, n, n), seed=0)
@@ -304,6 +317,54 @@ def main():
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def config_precision(args):
+    """128x128x64 hex8 (3.24 M DOFs), Neo-Hooke and SVK: the FP32-tangent / FP32-PCG + FP64-refinement form
+    against the FP64-tangent form on the same seeded mesh and load, 5 load steps to ||R|| <= 1e-8."""
+    res = {}
+    for name, mat in (("neohooke", capi.NEOHOOKE), ("svk", capi.SVK)):
+        p = pb.hex_block((128, 128, 64), material=mat, seed=0)
+        log(f"[bench] c3 {name}: {p.ne} elements, {p.ndof} dofs")
+        out, us = {}, {}
+        for mode in ("fp64", "mixed_exact"):
+            out[mode], us[mode] = newton_path(p, mode, args.c3_steps)
+            out[mode]["seconds_per_load_step"] = round(out[mode]["seconds"] / args.c3_steps, 4)
+            log(f"[bench] c3 {name} {mode}: {out[mode]}")
+        ref = us["fp64"]
+        res[name] = {"modes": out,
+                     "rel_err_u_mixed_vs_fp64": float(np.linalg.norm(us["mixed_exact"] - ref) / np.linalg.norm(ref)),
+                     "max_abs_err_u": float(np.max(np.abs(us["mixed_exact"] - ref)))}
+    return {"config": "hex8 128x128x64 precision variant (NH and SVK), 5% jitter seed 0, top traction, "
+                      f"{args.c3_steps} load steps", "dofs": int(p.ndof), "results": res}
+
+
+def config_eas(args):
+    """96x96x48 hex8 EAS-21 Neo-Hooke, x=0 clamped, bending traction on x=2: FP64 R+K assembly throughput
+    (element matrices are checked against the oracle by the GPU tests) and a plain-FP64 load path."""
+    p = pb.bending_block((96, 96, 48), seed=0)
+    log(f"[bench] c4: {p.ne} elements, {p.ndof} dofs")
+    dev = ia.Device(p)
+    u = 2e-4 * np.random.default_rng(0).standard_normal(p.ndof) * (1 - p.fixed)
+    dev.set_requirement(u, 1.0)
+    for _ in range(2):
+        assembly_step(dev)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.c4_steps):
+        assembly_step(dev)
+    dev.synchronize()
+    dt = time.perf_counter() - t0
+    dev.close()
+    out = {"config": "hex8 EAS-21 Neo-Hooke 96x96x48 over [0,2]x[0,2]x[0,1], 5% jitter seed 0, x=0 clamped, "
+                     f"traction {[round(t, 3) for t in p.traction]} on x=2", "elements": p.ne, "dofs": p.ndof,
+           "assembly": {"ms_per_step": round(dt / args.c4_steps * 1e3, 3),
+                        "Melem_per_s": round(p.ne * args.c4_steps / dt / 1e6, 3)}}
+    if args.c4_load_steps > 0:
+        r, _ = newton_path(p, "fp64", args.c4_load_steps)
+        r["seconds_per_load_step"] = round(r["seconds"] / args.c4_load_steps, 4)
+        out["newton_fp64"] = r
+    return out
 
 
 def np_i64():

@@ -182,7 +182,8 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
                                                      const int32_t* __restrict__ conn, int64_t e0, int64_t count,
                                                      double lam, double mu, const int32_t* __restrict__ dest,
                                                      const int32_t* __restrict__ rdest, TK* __restrict__ Kebe,
-                                                     double* __restrict__ Rebe, unsigned long long* __restrict__ err) {
+                                                     double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
+                                                     double3 trac, unsigned long long* __restrict__ err) {
   const int lane     = threadIdx.x;
   const int kk       = lane >> 4, bk = (lane >> 2) & 3, i = lane & 3;
   const int64_t base = (int64_t)blockIdx.x * EPW;
@@ -341,11 +342,16 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
         rr[h][k] = v;
       }
     if (kk == 0 && valid) {
+      const unsigned bits = fbits ? fbits[e] : 0u;
+      const double tk[3]  = {trac.x, trac.y, trac.z};
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          Rebe[(rdest ? (int64_t)rdest[e * 8 + i + 4 * h] * 3 : e * 24 + 3 * (i + 4 * h)) + k] = rr[h][k];
+        for (int k = 0; k < 3; ++k) {
+          double v = rr[h][k];
+          if (bits) v += faceLoad(X, conn, e, bits, i + 4 * h, tk[k]);  // Traction skill, after the internal forces
+          Rebe[(rdest ? (int64_t)rdest[e * 8 + i + 4 * h] * 3 : e * 24 + 3 * (i + 4 * h)) + k] = v;
+        }
     }
   }
   if constexpr (WANT_K) {
@@ -406,7 +412,8 @@ template <int KIND>
 __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X, const double* __restrict__ U,
                                                    const int32_t* __restrict__ conn, int64_t e0, int64_t count,
                                                    double lam, double mu, const int32_t* __restrict__ rdest,
-                                                   double* __restrict__ Rebe, unsigned long long* __restrict__ err) {
+                                                   double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
+                                                   double3 trac, unsigned long long* __restrict__ err) {
   __shared__ double XU[8][8][6];
   __shared__ double part[8][8][25];  // [element][gauss point][3a+k] (+1 pad)
   const int lane     = threadIdx.x;
@@ -468,11 +475,14 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
   }
   __syncthreads();
   if (valid) {
+    const unsigned bits = fbits ? fbits[e] : 0u;
+    const double tk[3]  = {trac.x, trac.y, trac.z};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double acc = 0.0;
 #pragma unroll
       for (int g = 0; g < 8; ++g) acc += part[el][g][3 * q + k];
+      if (bits) acc += faceLoad(X, conn, e, bits, q, tk[k]);  // Traction skill, after the internal forces
       Rebe[(rdest ? (int64_t)rdest[e * 8 + q] * 3 : e * 24 + 3 * q) + k] = acc;
     }
   }
@@ -545,44 +555,6 @@ __global__ __launch_bounds__(256) void gatherR(const int32_t* __restrict__ ncptr
   R[d] = acc;
 }
 
-// Traction faces (traction.hh:106-138): 1-point rule at the face centre, R_a -= N_a t |J_face|.
-__global__ void tractionKernel(const double* __restrict__ X, const int32_t* __restrict__ conn,
-                               const int32_t* __restrict__ faceElem, const int8_t* __restrict__ faceAxis,
-                               const int8_t* __restrict__ faceSide, int64_t nf, double t0, double t1, double t2,
-                               const uint8_t* __restrict__ fixed, int maskDbc, double* __restrict__ R) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  const int64_t e = faceElem[f];
-  const int ax = faceAxis[f], sd = faceSide[f];
-  double xi[3] = {0.5, 0.5, 0.5};
-  xi[ax]       = (double)sd;
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  int32_t nodes[8];
-  for (int a = 0; a < 8; ++a) {
-    nodes[a] = conn[e * 8 + a];
-    double dN[3];
-    dShape(a, xi, dN);
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) J[i][j] += X[(int64_t)nodes[a] * 3 + i] * dN[j];
-  }
-  const int ta = ax == 0 ? 1 : 0, tb = ax == 2 ? 1 : 2;
-  const double c0 = J[1][ta] * J[2][tb] - J[2][ta] * J[1][tb];
-  const double c1 = J[2][ta] * J[0][tb] - J[0][ta] * J[2][tb];
-  const double c2 = J[0][ta] * J[1][tb] - J[1][ta] * J[0][tb];
-  const double meas = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
-  const double t[3] = {t0, t1, t2};
-  for (int a = 0; a < 8; ++a) {
-    double N = 1.0;
-    for (int k = 0; k < 3; ++k) N *= ((a >> k) & 1) ? xi[k] : 1.0 - xi[k];
-    if (N == 0.0) continue;
-    for (int k = 0; k < 3; ++k) {
-      const int64_t dof = (int64_t)nodes[a] * 3 + k;
-      if (maskDbc && fixed[dof]) continue;
-      unsafeAtomicAdd(R + dof, -N * t[k] * meas);
-    }
-  }
-}
-
 __global__ void pointLoadKernel(const int64_t* __restrict__ dof, const double* __restrict__ coef, int64_t n,
                                 double lambda, double* __restrict__ R) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,11 +593,12 @@ void launchElem(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, co
   if constexpr (!WK) {
     const unsigned grid = (unsigned)((count + 7) / 8);
     hipLaunchKernelGGL((hex8Residual<KIND>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p, h->conn.p, e0,
-                       count, h->lam, h->mu, rdest, Rebe, h->errFlag.p);
+                       count, h->lam, h->mu, rdest, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
   } else {
     const unsigned grid = (unsigned)((count + EPW - 1) / EPW);
     hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p,
-                       h->conn.p, e0, count, h->lam, h->mu, dest, rdest, Kebe, Rebe, h->errFlag.p);
+                       h->conn.p, e0, count, h->lam, h->mu, dest, rdest, Kebe, Rebe, h->faceBits.p,
+                       loadedTraction(h), h->errFlag.p);
   }
 }
 
@@ -754,11 +727,6 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
                        h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
   }
   if (h->neas) h->easCacheValid = true;
-  if (wantR && h->nfaces) {
-    hipLaunchKernelGGL(tractionKernel, dim3(gridFor(h->nfaces, 128)), dim3(128), 0, h->stream, h->X.p, h->conn.p,
-                       h->faceElem.p, h->faceAxis.p, h->faceSide.p, h->nfaces, h->trac[0] * h->lambda,
-                       h->trac[1] * h->lambda, h->trac[2] * h->lambda, h->fixed.p, mask, h->R.p);
-  }
   if (wantR && h->npl) {
     hipLaunchKernelGGL(pointLoadKernel, dim3(gridFor(h->npl, 128)), dim3(128), 0, h->stream, h->plDof.p,
                        h->plCoef.p, h->npl, h->lambda, h->R.p);

@@ -210,9 +210,14 @@ int ik_create(ik_handle** out, const ik_problem_desc* d) {
         require(d->face_elem[f] >= 0 && d->face_elem[f] < h->ne && d->face_axis[f] >= 0 && d->face_axis[f] < 3 &&
                     (d->face_side[f] == 0 || d->face_side[f] == 1),
                 IK_EINVAL, "bad traction face");
-      h->faceElem.upload(d->face_elem, h->nfaces, h->stream);
-      h->faceAxis.upload(d->face_axis, h->nfaces, h->stream);
-      h->faceSide.upload(d->face_side, h->nfaces, h->stream);
+      // per element, one bit per loaded face (the element residual kernels add the Traction skill)
+      std::vector<uint8_t> bits(h->ne, 0);
+      for (int64_t f = 0; f < h->nfaces; ++f) {
+        const uint8_t b = (uint8_t)(1u << (2 * d->face_axis[f] + d->face_side[f]));
+        require(!(bits[d->face_elem[f]] & b), IK_EINVAL, "traction face listed twice");
+        bits[d->face_elem[f]] |= b;
+      }
+      h->faceBits.upload(bits.data(), bits.size(), h->stream);
     }
     for (int k = 0; k < 3; ++k) h->trac[k] = h->mlpTraction[k] = d->traction[k];
     h->npl = d->num_point_loads;

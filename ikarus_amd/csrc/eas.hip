@@ -161,8 +161,9 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
                                                  const double* __restrict__ alpha, int64_t e0, double lam,
                                                  double mu, const int32_t* __restrict__ dest,
                                                  const int32_t* __restrict__ rdest, double* __restrict__ Kebe,
-                                                 double* __restrict__ Rebe,
-                                                 double* __restrict__ cache, unsigned long long* __restrict__ err) {
+                                                 double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
+                                                 double3 trac, double* __restrict__ cache,
+                                                 unsigned long long* __restrict__ err) {
   __shared__ EasLds s;
   const int t     = threadIdx.x;
   const int64_t e = e0 + blockIdx.x;
@@ -474,6 +475,11 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     const int r = t - 40;
     double acc  = s.Ru[r];
     for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * s.Y[i][24];
+    const unsigned bits = fbits ? fbits[e] : 0u;
+    if (bits) {  // Traction skill, after the EAS element's condensed internal forces
+      const double tk[3] = {trac.x, trac.y, trac.z};
+      acc += faceLoad(X, conn, e, bits, r / 3, tk[r % 3]);
+    }
     Rebe[rdest ? (int64_t)rdest[e * 8 + r / 3] * 3 + r % 3 : e * 24 + r] = acc;
   }
 }
@@ -500,11 +506,11 @@ template <bool WK>
 void launchEas(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, double* Kebe) {
   if (h->material == IK_SVK)
     hipLaunchKernelGGL((easElement<IK_SVK, WK>), dim3((unsigned)count), dim3(64), 0, h->stream, h->X.p, h->u.p,
-                       h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, rdest, Kebe, h->ebeR.p, h->easCache.p,
+                       h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, rdest, Kebe, h->ebeR.p, h->faceBits.p, loadedTraction(h), h->easCache.p,
                        h->errFlag.p);
   else
     hipLaunchKernelGGL((easElement<IK_NEOHOOKE, WK>), dim3((unsigned)count), dim3(64), 0, h->stream, h->X.p,
-                       h->u.p, h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, rdest, Kebe, h->ebeR.p, h->easCache.p,
+                       h->u.p, h->conn.p, h->alpha.p, e0, h->lam, h->mu, dest, rdest, Kebe, h->ebeR.p, h->faceBits.p, loadedTraction(h), h->easCache.p,
                        h->errFlag.p);
 }
 

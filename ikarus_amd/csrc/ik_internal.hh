@@ -97,8 +97,7 @@ struct ik_handle {
   ik::DevBuf<int32_t> fixedList;  // constrained DOF indices
   // loads
   int64_t nfaces = 0;
-  ik::DevBuf<int32_t> faceElem;
-  ik::DevBuf<int8_t> faceAxis, faceSide;
+  ik::DevBuf<uint8_t> faceBits;  // per element: bit 2*axis + side set for each loaded face (empty: none)
   double trac[3] = {0, 0, 0};
   int64_t npl = 0;
   ik::DevBuf<int64_t> plDof;
@@ -192,6 +191,45 @@ void mlpPredict(ik_handle* h, const double* duN, const double* duN1, double lamN
                 double dlamPrev2);
 
 // timing helpers (events on the handle's stream around one launch)
+// Traction skill of one element (traction.hh:106-138): for each loaded face (bit 2*axis + side of `bits`) the
+// one-point rule at the face centre adds -N_a t |J_face| to R_e[3a+k]. Returns that sum for corner a, component
+// k (t = the traction component at the current load factor).
+__device__ inline double faceLoad(const double* __restrict__ X, const int32_t* __restrict__ conn, int64_t e,
+                                  unsigned bits, int a, double t) {
+  double out = 0.0;
+  for (int f = 0; f < 6; ++f) {
+    const int ax = f >> 1, sd = f & 1;
+    if (!((bits >> f) & 1) || ((a >> ax) & 1) != sd) continue;  // N_a vanishes on faces without corner a
+    double xi[3] = {0.5, 0.5, 0.5};
+    xi[ax]       = (double)sd;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int b = 0; b < 8; ++b) {
+      double fb[3], dN[3];
+      for (int k = 0; k < 3; ++k) fb[k] = ((b >> k) & 1) ? xi[k] : 1.0 - xi[k];
+      for (int k = 0; k < 3; ++k) {
+        const double s = ((b >> k) & 1) ? 1.0 : -1.0;
+        dN[k]          = s * fb[(k + 1) % 3] * fb[(k + 2) % 3];
+      }
+      const int64_t nd = conn[e * 8 + b];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) J[i][j] += X[nd * 3 + i] * dN[j];
+    }
+    const int ta = ax == 0 ? 1 : 0, tb = ax == 2 ? 1 : 2;
+    const double c0 = J[1][ta] * J[2][tb] - J[2][ta] * J[1][tb];
+    const double c1 = J[2][ta] * J[0][tb] - J[0][ta] * J[2][tb];
+    const double c2 = J[0][ta] * J[1][tb] - J[1][ta] * J[0][tb];
+    double N        = 1.0;
+    for (int k = 0; k < 3; ++k) N *= ((a >> k) & 1) ? xi[k] : 1.0 - xi[k];
+    out -= N * t * sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  }
+  return out;
+}
+
+// traction vector at the bound load factor
+inline double3 loadedTraction(const ik_handle* h) {
+  return make_double3(h->trac[0] * h->lambda, h->trac[1] * h->lambda, h->trac[2] * h->lambda);
+}
+
 // HIP events around `launches` back-to-back launches of one kernel on stream s (timing mode only)
 struct ScopedKernelTimer {
   ik_handle* h;
