@@ -495,10 +495,8 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
 // transposed, to its mirror block below the diagonal (K and the Dirichlet treatment are symmetric).
 template <typename T>
 __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segptr, const int32_t* __restrict__ upBlk,
-                                               const int32_t* __restrict__ upT, const int32_t* __restrict__ blockRow,
-                                               const int32_t* __restrict__ colidx,
-                                               const uint8_t* __restrict__ fixed, int maskDbc,
-                                               const T* __restrict__ Kst, int64_t q0, int64_t q1,
+                                               const int32_t* __restrict__ upT, const uint16_t* __restrict__ upMask,
+                                               int maskDbc, const T* __restrict__ Kst, int64_t q0, int64_t q1,
                                                T* __restrict__ K) {
   const uint32_t loc = blockIdx.x * 256u + threadIdx.x;  // (q - q0) * 9 + t
   const uint32_t ql   = loc / 9u;
@@ -521,8 +519,8 @@ __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segpt
   }
   const int32_t p = upBlk[q];
   if (maskDbc) {
-    const int64_t r = (int64_t)blockRow[p] * 3 + k, cc = (int64_t)colidx[p] * 3 + l;
-    if (fixed[r] | fixed[cc]) acc = (r == cc) ? T(1) : T(0);
+    const unsigned m = upMask[q];
+    if ((m >> t) & 1u) acc = (k == l && ((m >> (9 + k)) & 1u)) ? T(1) : T(0);
   }
   K[(int64_t)p * 9 + t] = acc;
   const int32_t pt      = upT[q];
@@ -621,9 +619,10 @@ void launchZeroF(float* p, int64_t n, hipStream_t s) {
 }
 
 static void checkErrFlag(ik_handle* h) {
-  unsigned long long v = 0;
-  IK_HIP(hipMemcpyAsync(&v, h->errFlag.p, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+  if (!h->herr) IK_HIP(hipHostMalloc((void**)&h->herr, sizeof(unsigned long long), hipHostMallocDefault));
+  IK_HIP(hipMemcpyAsync(h->herr, h->errFlag.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
   IK_HIP(hipStreamSynchronize(h->stream));
+  const unsigned long long v = *h->herr;
   if (v != ~0ull) {
     h->bad_elem = (int64_t)v;
     unsigned long long reset = ~0ull;
@@ -648,7 +647,7 @@ template <typename T>
 static void launchGatherK(ik_handle* h, int64_t r0, int64_t r1, int mask, const T* Kebe, T* K, hipStream_t s) {
   const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
   hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)(((q1 - q0) * 9 + 255) / 256)), dim3(256), 0, s, h->segptr.p,
-                     h->upBlk.p, h->upT.p, h->blockRow.p, h->colidx.p, h->fixed.p, mask, Kebe, q0, q1, K);
+                     h->upBlk.p, h->upT.p, h->upMask.p, mask, Kebe, q0, q1, K);
 }
 
 // Tangent assembly. Element chunks run on the main stream; each chunk's gathers run on the second stream,

@@ -254,6 +254,7 @@ void ik_destroy(ik_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->hsc) (void)hipHostFree(h->hsc);
   if (h->hred) (void)hipHostFree(h->hred);
+  if (h->herr) (void)hipHostFree(h->herr);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream2) (void)hipStreamSynchronize(h->stream2);

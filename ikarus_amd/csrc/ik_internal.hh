@@ -114,6 +114,9 @@ struct ik_handle {
   // huptr[r] = first upper block of row r
   std::vector<int64_t> huptr;
   ik::DevBuf<int32_t> upBlk, upT;
+  // FULL Dirichlet treatment per upper block: bit t (t = 3k+l) zeroes entry t (constrained row or column),
+  // bit 9+k sets diagonal entry (k,k) of a diagonal block to one
+  ik::DevBuf<uint16_t> upMask;
   // staging order of the element blocks: upper block by upper block, each segment ordered by element.
   // bdest[e*36+sb] = (slot << 1) | transposed, segptr[q] = first slot of upper block q. Residual staging:
   // node by node, each node's (element, corner) incidences ordered by element; rdest[e*8+a] = slot,
@@ -148,6 +151,7 @@ struct ik_handle {
   ik::DevBuf<ik::PcgScalars> sc;
   ik::PcgScalars* hsc = nullptr;  // pinned host mirror
   double* hred = nullptr;         // pinned host scalar for reductions
+  unsigned long long* herr = nullptr;  // pinned host copy of the determinant-failure flag
 
   // learned predictor
   ik::Mlp* mlp = nullptr;

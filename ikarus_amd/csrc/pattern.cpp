@@ -8,13 +8,24 @@
 // (one per upper element block, pointing into the gather order of its K block). The scalar CSC is
 // re-derived from it on demand.
 #include <algorithm>
+#include <cstdlib>
 
 #include "ik_internal.hh"
 
 namespace ik {
 
-// elements per staging chunk (about 85 MB of FP64 element blocks)
-constexpr int64_t kChunkElements = 32768;
+// elements per staging chunk; IKARUS_AMD_CHUNK_ELEMENTS overrides it. Splitting the 64^3 mesh into chunks
+// whose gathers overlap the next chunk's element work measured slower than one launch of each (the two
+// kernels contend for the same memory path), so the default takes meshes up to 1 M elements in one chunk.
+constexpr int64_t kChunkElements = 1 << 20;
+
+static int64_t chunkElements() {
+  if (const char* v = std::getenv("IKARUS_AMD_CHUNK_ELEMENTS")) {
+    const long long c = std::atoll(v);
+    if (c >= 64) return (int64_t)c;
+  }
+  return kChunkElements;
+}
 
 void buildPattern(ik_handle* h) {
   const int64_t nn = h->nn, ne = h->ne;
@@ -124,7 +135,7 @@ void buildPattern(ik_handle* h) {
     for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) last[i] = std::max(last[i], inc[k]);
     if (i > 0) last[i] = std::max(last[i], last[i - 1]);
   }
-  const int64_t chunk = std::max<int64_t>(kChunkElements, 1);
+  const int64_t chunk = chunkElements();
   h->chunkElem.clear();
   h->chunkRow.clear();
   h->chunkElem.push_back(0);
@@ -143,6 +154,20 @@ void buildPattern(ik_handle* h) {
   h->colidx.upload(h->hcolidx.data(), h->hcolidx.size(), h->stream);
   h->diagBlk.upload(h->hdiag.data(), h->hdiag.size(), h->stream);
   h->blockRow.upload(brow.data(), brow.size(), h->stream);
+  std::vector<uint16_t> mask(up.size(), 0);
+  for (int64_t i = 0; i < nn; ++i)
+    for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
+      const int64_t c = h->hcolidx[p];
+      uint16_t m      = 0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l)
+          if (h->hfixed[3 * i + k] | h->hfixed[3 * c + l]) {
+            m |= (uint16_t)(1u << (3 * k + l));
+            if (3 * i + k == 3 * c + l) m |= (uint16_t)(1u << (9 + k));
+          }
+      mask[h->huptr[i] + (p - h->hdiag[i])] = m;
+    }
+  h->upMask.upload(mask.data(), mask.size(), h->stream);
   h->upBlk.upload(up.data(), up.size(), h->stream);
   h->upT.upload(upT.data(), upT.size(), h->stream);
   h->bdest.upload(dest.data(), dest.size(), h->stream);
