@@ -91,7 +91,7 @@ __device__ inline void jacobianAt(const double (*X)[3], const double xi[3], doub
 struct EasLds {
   double X[8][3], U[8][3];
   double alpha[NE];
-  double T0[6][6];     // T(xi0) detJ0, inverted in place
+  double T0[6][6];     // (T(xi0) detJ0)^-1
   double G[8][8][3];   // [gp][node][dir]
   double F[8][9];
   double S[8][6];      // S dV
@@ -105,7 +105,6 @@ struct EasLds {
   double rt[NE];       // Rt, then D^{-1} Rt
   double K[36][9];     // K_uu upper blocks
   double Ru[24];
-  double Y[NE][25];    // D^{-1} [L | Rt]
   int bad;
 };
 
@@ -177,44 +176,28 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   if (t < NE) s.alpha[t] = alpha[e * NE + t];
   if (t == 0) s.bad = 0;
   __syncthreads();
-  // T(xi0) * detJ0 (tensorutils.hh:440-465; helperfunctions.hh:19-25), then inverted by lane 0
+  // (T(xi0) detJ0)^-1 (tensorutils.hh:440-465; helperfunctions.hh:19-25). The Voigt strain transformation is
+  // a representation of the Jacobian, T(J)^-1 = T(J^-1), so each of 36 lanes writes one entry directly.
   if (t < 36) {
     const double xc[3] = {0.5, 0.5, 0.5};
     double J[3][3];
     jacobianAt(s.X, xc, J);
-    const double dJ = fabs(det3(J));
+    const double dJ = det3(J);
+    const double id = 1.0 / dJ;
+    double Ji[3][3];
+    Ji[0][0]       = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+    Ji[0][1]       = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[0][2]       = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][0]       = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    Ji[1][1]       = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[1][2]       = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][0]       = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    Ji[2][1]       = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[2][2]       = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
     const int r = t / 6, c = t % 6;
     const int a = vI(r), b = vJ(r), k = vI(c), l = vJ(c);
-    // Jt[i][j] = J[j][i]
-    const double v = J[k][a] * J[l][b] + J[l][a] * J[k][b];
-    s.T0[r][c]     = (a == b ? 0.5 * v : v) * dJ;
-  }
-  __syncthreads();
-  if (t == 0) {
-    // Gauss-Jordan with partial pivoting on the 6x6 (in LDS, into A's first 36 slots as scratch)
-    double Mx[6][12];
-    for (int r = 0; r < 6; ++r)
-      for (int c = 0; c < 12; ++c) Mx[r][c] = c < 6 ? s.T0[r][c] : (c - 6 == r ? 1.0 : 0.0);
-    for (int c = 0; c < 6; ++c) {
-      int p = c;
-      for (int r = c + 1; r < 6; ++r)
-        if (fabs(Mx[r][c]) > fabs(Mx[p][c])) p = r;
-      if (p != c)
-        for (int k = 0; k < 12; ++k) {
-          const double tmp = Mx[p][k];
-          Mx[p][k]         = Mx[c][k];
-          Mx[c][k]         = tmp;
-        }
-      const double piv = 1.0 / Mx[c][c];
-      for (int k = 0; k < 12; ++k) Mx[c][k] *= piv;
-      for (int r = 0; r < 6; ++r) {
-        if (r == c) continue;
-        const double f = Mx[r][c];
-        for (int k = 0; k < 12; ++k) Mx[r][k] -= f * Mx[c][k];
-      }
-    }
-    for (int r = 0; r < 6; ++r)
-      for (int c = 0; c < 6; ++c) s.T0[r][c] = Mx[r][c + 6];
+    const double v = Ji[k][a] * Ji[l][b] + Ji[l][a] * Ji[k][b];
+    s.T0[r][c]     = (a == b ? 0.5 * v : v) / fabs(dJ);
   }
   __syncthreads();
   // Gauss point kinematics (lanes 0-7)
@@ -283,7 +266,8 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   __syncthreads();
   if (s.bad) return;  // uniform: the error flag is already set
 
-  // accumulators: lane t < 36 owns K_uu block t; lanes own D / L / Rt / R_u entries (strided)
+  // accumulators: lane t < 36 owns K_uu block t; lanes own D / L / Rt / R_u entries (strided). The entry
+  // each lane owns does not change over the Gauss points, so its indices are decoded once here.
   double kb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double dacc[4] = {0, 0, 0, 0};   // D upper entries t, t+64, t+128, t+192 (< 231)
   double lacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // L entries t + 64 j (< 504)
@@ -298,30 +282,61 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     blkA = a;
     blkB = a + r;
   }
+  int dI[4], dC[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int r = t + 64 * j, i = 0;
+    if (r < 231)
+      while (r >= NE - i) {
+        r -= NE - i;
+        ++i;
+      }
+    dI[j] = i;
+    dC[j] = i + r;
+  }
+  int bV[3], bCol[3], bG[3], bK[3];  // B build / C B: entry idx = t + 64 j of the 6 x 24 matrices
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int idx = t + 64 * j;
+    bV[j]   = idx / 24;
+    bCol[j] = idx % 24;
+    bG[j]   = bCol[j] / 3;
+    bK[j]   = bCol[j] % 3;
+  }
   for (int g = 0; g < 8; ++g) {
     // B (6x24), C B and C M for this Gauss point
-    for (int idx = t; idx < 144; idx += 64) {
-      const int v = idx / 24, col = idx % 24, a = col / 3, k = col % 3;
-      const double* F  = s.F[g];
-      const double* ga = s.G[g][a];
-      const int I = vI(v), J = vJ(v);
-      double bv = F[k * 3 + I] * ga[J];
-      if (I != J) bv += F[k * 3 + J] * ga[I];
-      s.B[v][col] = bv;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (t + 64 * j < 144) {
+        const double* F  = s.F[g];
+        const double* ga = s.G[g][bG[j]];
+        const int I = vI(bV[j]), J = vJ(bV[j]), k = bK[j];
+        double bv = F[k * 3 + I] * ga[J];
+        if (I != J) bv += F[k * 3 + J] * ga[I];
+        s.B[bV[j]][bCol[j]] = bv;
+      }
     }
-    for (int idx = t; idx < 6 * NE; idx += 64) {
-      const int v = idx / NE, c = idx % NE;
-      double acc = 0.0;
-      for (int w = 0; w < 6; ++w) acc += s.C[g][sym6(v, w)] * s.M[g][w][c];
-      s.CM[v][c] = acc;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int idx = t + 64 * j;
+      if (idx < 6 * NE) {
+        const int v = idx / NE, c = idx % NE;
+        double acc = 0.0;
+#pragma unroll
+        for (int w = 0; w < 6; ++w) acc += s.C[g][sym6(v, w)] * s.M[g][w][c];
+        s.CM[v][c] = acc;
+      }
     }
     __syncthreads();
     if constexpr (WANT_K) {
-      for (int idx = t; idx < 144; idx += 64) {
-        const int v = idx / 24, col = idx % 24;
-        double acc = 0.0;
-        for (int w = 0; w < 6; ++w) acc += s.C[g][sym6(v, w)] * s.B[w][col];
-        s.CB[v][col] = acc;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        if (t + 64 * j < 144) {
+          double acc = 0.0;
+#pragma unroll
+          for (int w = 0; w < 6; ++w) acc += s.C[g][sym6(bV[j], w)] * s.B[w][bCol[j]];
+          s.CB[bV[j]][bCol[j]] = acc;
+        }
       }
     }
     __syncthreads();
@@ -334,9 +349,12 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
         const double kg  = ga[0] * (Sv[0] * gb[0] + Sv[5] * gb[1] + Sv[4] * gb[2]) +
                           ga[1] * (Sv[5] * gb[0] + Sv[1] * gb[1] + Sv[3] * gb[2]) +
                           ga[2] * (Sv[4] * gb[0] + Sv[3] * gb[1] + Sv[2] * gb[2]);
+#pragma unroll
         for (int k = 0; k < 3; ++k)
+#pragma unroll
           for (int l = 0; l < 3; ++l) {
             double acc = 0.0;
+#pragma unroll
             for (int v = 0; v < 6; ++v) acc += s.B[v][3 * blkA + k] * s.CB[v][3 * blkB + l];
             kb[k * 3 + l] += acc + (k == l ? kg : 0.0);
           }
@@ -345,16 +363,10 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     // D upper (231 entries) and L (504 entries)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int idx = t + 64 * j;
-      if (idx < 231) {
-        int r = idx, i = 0;
-        while (r >= NE - i) {
-          r -= NE - i;
-          ++i;
-        }
-        const int c = i + r;
-        double acc  = 0.0;
-        for (int v = 0; v < 6; ++v) acc += s.M[g][v][i] * s.CM[v][c];
+      if (t + 64 * j < 231) {
+        double acc = 0.0;
+#pragma unroll
+        for (int v = 0; v < 6; ++v) acc += s.M[g][v][dI[j]] * s.CM[v][dC[j]];
         dacc[j] += acc;
       }
     }
@@ -362,19 +374,22 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     for (int j = 0; j < 8; ++j) {
       const int idx = t + 64 * j;
       if (idx < NE * 24) {
-        const int i = idx / 24, col = idx % 24;
+        const int i = idx / 24, col = idx - 24 * (idx / 24);
         double acc  = 0.0;
+#pragma unroll
         for (int v = 0; v < 6; ++v) acc += s.CM[v][i] * s.B[v][col];
         lacc[j] += acc;
       }
     }
     if (t < NE) {
       double acc = 0.0;
+#pragma unroll
       for (int v = 0; v < 6; ++v) acc += s.M[g][v][t] * s.S[g][v];
       racc += acc;
     } else if (t >= 32 && t < 56) {
       const int col = t - 32;
       double acc    = 0.0;
+#pragma unroll
       for (int v = 0; v < 6; ++v) acc += s.B[v][col] * s.S[g][v];
       racc += acc;
     }
@@ -383,16 +398,9 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   // spill accumulators to LDS
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int idx = t + 64 * j;
-    if (idx < 231) {
-      int r = idx, i = 0;
-      while (r >= NE - i) {
-        r -= NE - i;
-        ++i;
-      }
-      const int c = i + r;
-      s.A[i][c]   = dacc[j];
-      s.A[c][i]   = dacc[j];
+    if (t + 64 * j < 231) {
+      s.A[dI[j]][dC[j]] = dacc[j];
+      s.A[dC[j]][dI[j]] = dacc[j];
     }
   }
 #pragma unroll
@@ -407,66 +415,68 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
       for (int i = 0; i < 9; ++i) s.K[t][i] = kb[i];
   }
   __syncthreads();
-  // LDL^T of D, right-looking, lower triangle in place: A[i][j] (i > j) = l_ij, A[j][j] = d_j
+
+  // Y = D^-1 [L | Rt] by Gauss-Jordan elimination on the augmented 21 x 46 matrix (D is SPD: no pivoting,
+  // like the LDL^T it replaces; enhancedassumedstrains.hh inverts D, :243,294,344). Lane c < 46 keeps column
+  // c in registers; each step the pivot column is broadcast through LDS.
+  double col[NE];
+#pragma unroll
+  for (int i = 0; i < NE; ++i) col[i] = t < NE ? s.A[i][t] : (t < NE + 24 ? s.Lm[i][t - NE] : (t == NE + 24 ? s.rt[i] : 0.0));
+  __syncthreads();
+  double* pivbuf = &s.A[0][0];  // D is in the lanes' registers from here on
+#pragma unroll
   for (int j = 0; j < NE; ++j) {
-    const double dj = s.A[j][j];
-    const int m     = NE - 1 - j;  // trailing size
-    for (int idx = t; idx < m * (m + 1) / 2; idx += 64) {
-      int r = idx, ii = 0;
-      while (r > ii) {
-        r -= ii + 1;
-        ++ii;
-      }
-      const int i = j + 1 + ii, k = j + 1 + r;  // k <= i
-      s.A[i][k] -= s.A[i][j] * s.A[k][j] / dj;
-    }
+    if (t == j)
+#pragma unroll
+      for (int i = 0; i < NE; ++i) pivbuf[i] = col[i];
     __syncthreads();
-    if (t > j && t < NE) s.A[t][j] /= dj;
+    const double f = col[j] / pivbuf[j];
+#pragma unroll
+    for (int i = 0; i < NE; ++i)
+      if (i != j) col[i] -= pivbuf[i] * f;
+    col[j] = f;
     __syncthreads();
   }
-  // Y = D^{-1} [L | Rt]: lane c < 25 solves column c
-  if (t < 25) {
-    double y[NE];
-    for (int i = 0; i < NE; ++i) y[i] = t < 24 ? s.Lm[i][t] : s.rt[i];
-    for (int i = 0; i < NE; ++i)
-      for (int k = 0; k < i; ++k) y[i] -= s.A[i][k] * y[k];
-    for (int i = 0; i < NE; ++i) y[i] /= s.A[i][i];
-    for (int i = NE - 1; i >= 0; --i)
-      for (int k = i + 1; k < NE; ++k) y[i] -= s.A[k][i] * y[k];
-    // cache layout per element: Y (21 x 24) row-major, then y (21)
-    double* cy = cache + e * (NE * 24 + NE);
+  // lanes NE..NE+23 hold Y = D^-1 L (column c = t - NE), lane NE+24 holds y = D^-1 Rt; cache them for the
+  // alpha update (layout per element: Y 21 x 24 row-major, then y)
+  double* cy = cache + e * (NE * 24 + NE);
+  if (t >= NE && t < NE + 24) {
+#pragma unroll
+    for (int i = 0; i < NE; ++i) cy[i * 24 + (t - NE)] = col[i];
+  } else if (t == NE + 24) {
+#pragma unroll
     for (int i = 0; i < NE; ++i) {
-      s.Y[i][t] = y[i];
-      if (t < 24)
-        cy[i * 24 + t] = y[i];
-      else
-        cy[NE * 24 + i] = y[i];
+      cy[NE * 24 + i] = col[i];
+      s.rt[i]         = col[i];  // y for the residual lanes
+    }
+  }
+  // K = K_uu - L^T Y on the upper blocks: lane of column c = 3b+l computes rows 3a+k of blocks a < b and,
+  // for the diagonal block, k <= l (mirrored afterwards: enhancedassumedstrains.hh:291-296)
+  if constexpr (WANT_K) {
+    if (t >= NE && t < NE + 24) {
+      const int c = t - NE, b = c / 3, l = c % 3;
+      for (int r = 0; r <= c; ++r) {
+        const int a = r / 3, k = r % 3;
+        if (a == b && k > l) continue;
+        double acc = s.K[upperBlock(a, b)][3 * k + l];
+#pragma unroll
+        for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * col[i];
+        s.K[upperBlock(a, b)][3 * k + l] = acc;
+        if (a == b && k != l) s.K[upperBlock(a, b)][3 * l + k] = acc;
+      }
     }
   }
   __syncthreads();
-  // K = K_uu - L^T Y (upper blocks), R = R_u - L^T y
   if constexpr (WANT_K) {
     if (t < 36) {
-      double* dst = Kebe + (e * 36 + t) * 9;
-      double out[9];
-      for (int k = 0; k < 3; ++k)
-        for (int l = 0; l < 3; ++l) {
-          double acc = s.K[t][k * 3 + l];
-          const int r = 3 * blkA + k, c = 3 * blkB + l;
-          for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * s.Y[i][c];
-          out[k * 3 + l] = acc;
-        }
-      if (blkA == blkB) {  // mirror the diagonal block's upper triangle (enhancedassumedstrains.hh:291-296)
-        out[3] = out[1];
-        out[6] = out[2];
-        out[7] = out[5];
-      }
+      const double* out = s.K[t];
       if (dest) {
         const int32_t code = dest[e * 36 + t];
-        dst                = Kebe + (int64_t)(code >> 1) * 9;
+        double* dst        = Kebe + (int64_t)(code >> 1) * 9;
         for (int k = 0; k < 3; ++k)
           for (int l = 0; l < 3; ++l) dst[3 * k + l] = (code & 1) ? out[3 * l + k] : out[3 * k + l];
       } else {
+        double* dst = Kebe + (e * 36 + t) * 9;
         for (int i = 0; i < 9; ++i) dst[i] = out[i];
       }
     }
@@ -474,11 +484,11 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   if (t >= 40 && t < 64) {
     const int r = t - 40;
     double acc  = s.Ru[r];
-    for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * s.Y[i][24];
+    for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * s.rt[i];
     const unsigned bits = fbits ? fbits[e] : 0u;
     if (bits) {  // Traction skill, after the EAS element's condensed internal forces
-      const double tk[3] = {trac.x, trac.y, trac.z};
-      acc += faceLoad(X, conn, e, bits, r / 3, tk[r % 3]);
+      const int k = r % 3;
+      acc += faceLoad(X, conn, e, bits, r / 3, k == 0 ? trac.x : (k == 1 ? trac.y : trac.z));
     }
     Rebe[rdest ? (int64_t)rdest[e * 8 + r / 3] * 3 + r % 3 : e * 24 + r] = acc;
   }

@@ -204,8 +204,8 @@ __device__ inline double faceLoad(const double* __restrict__ X, const int32_t* _
   for (int f = 0; f < 6; ++f) {
     const int ax = f >> 1, sd = f & 1;
     if (!((bits >> f) & 1) || ((a >> ax) & 1) != sd) continue;  // N_a vanishes on faces without corner a
-    double xi[3] = {0.5, 0.5, 0.5};
-    xi[ax]       = (double)sd;
+    double xi[3];
+    for (int k = 0; k < 3; ++k) xi[k] = k == ax ? (double)sd : 0.5;
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int b = 0; b < 8; ++b) {
       double fb[3], dN[3];
@@ -218,10 +218,15 @@ __device__ inline double faceLoad(const double* __restrict__ X, const int32_t* _
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) J[i][j] += X[nd * 3 + i] * dN[j];
     }
-    const int ta = ax == 0 ? 1 : 0, tb = ax == 2 ? 1 : 2;
-    const double c0 = J[1][ta] * J[2][tb] - J[2][ta] * J[1][tb];
-    const double c1 = J[2][ta] * J[0][tb] - J[0][ta] * J[2][tb];
-    const double c2 = J[0][ta] * J[1][tb] - J[1][ta] * J[0][tb];
+    // the two tangent columns of J (axes other than ax, in increasing order)
+    double u[3], w[3];
+    for (int i = 0; i < 3; ++i) {
+      u[i] = ax == 0 ? J[i][1] : J[i][0];
+      w[i] = ax == 2 ? J[i][1] : J[i][2];
+    }
+    const double c0 = u[1] * w[2] - u[2] * w[1];
+    const double c1 = u[2] * w[0] - u[0] * w[2];
+    const double c2 = u[0] * w[1] - u[1] * w[0];
     double N        = 1.0;
     for (int k = 0; k < 3; ++k) N *= ((a >> k) & 1) ? xi[k] : 1.0 - xi[k];
     out -= N * t * sqrt(c0 * c0 + c1 * c1 + c2 * c2);
