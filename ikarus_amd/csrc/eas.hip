@@ -96,10 +96,13 @@ struct EasLds {
   double F[8][9];
   double S[8][6];      // S dV
   double C[8][21];     // C dV (packed)
-  double M[8][6][NE];  // M(xi_gp)
+  double f[8][NE];     // E21 mode scalings f_c(xi_gp) / detJ_gp: column c of M = T0^-1[:, kRow[c]] f_c
   double B[6][24];     // current Gauss point
   double CB[6][24];    // C dV B
-  double CM[6][NE];    // C dV M
+  double CT[6][6];     // C dV T0^-1
+  double Ch[6][6];     // T0^-T C dV T0^-1
+  double Z[6][24];     // (C dV T0^-1)^T B
+  double TS[6];        // T0^-T S dV
   double A[NE][NE];    // D, then its LDL^T factors
   double Lm[NE][24];   // L, then D^{-1} L
   double rt[NE];       // Rt, then D^{-1} Rt
@@ -247,11 +250,8 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     for (int c = 0; c < NE; ++c) {
       const double f = ansatz(c, x, y, z) * sc;
       const int rr   = kRow[c];
-      for (int r = 0; r < 6; ++r) {
-        const double m = s.T0[r][rr] * f;
-        s.M[t][r][c]   = m;
-        Ev[r] += m * s.alpha[c];
-      }
+      s.f[t][c] = f;
+      for (int r = 0; r < 6; ++r) Ev[r] += s.T0[r][rr] * f * s.alpha[c];
     }
     double Sv[6], Cp[21];
     if (!material<KIND>(lam, mu, Ev, Sv, Cp)) {
@@ -294,7 +294,22 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     dI[j] = i;
     dC[j] = i + r;
   }
-  int bV[3], bCol[3], bG[3], bK[3];  // B build / C B: entry idx = t + 64 j of the 6 x 24 matrices
+  int dKi[4], dKc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    dKi[j] = kRow[dI[j] < NE ? dI[j] : 0];
+    dKc[j] = kRow[dC[j] < NE ? dC[j] : 0];
+  }
+  int lI[8], lCol[8], lK[8];  // L entry idx = t + 64 j: row (mode) i, column, kRow[i]
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int idx = t + 64 * j;
+    lI[j]   = idx / 24;
+    lCol[j] = idx % 24;
+    lK[j]   = kRow[lI[j] < NE ? lI[j] : 0];
+  }
+  const int rK = kRow[t < NE ? t : 0];
+  int bV[3], bCol[3], bG[3], bK[3];  // B build / C B / Z: entry idx = t + 64 j of the 6 x 24 matrices
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const int idx = t + 64 * j;
@@ -304,7 +319,7 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     bK[j]   = bCol[j] % 3;
   }
   for (int g = 0; g < 8; ++g) {
-    // B (6x24), C B and C M for this Gauss point
+    // B (6x24) and CT = C dV T0^-1 for this Gauss point
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       if (t + 64 * j < 144) {
@@ -316,28 +331,40 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
         s.B[bV[j]][bCol[j]] = bv;
       }
     }
+    if (t < 36) {
+      const int v = t / 6, w = t % 6;
+      double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int idx = t + 64 * j;
-      if (idx < 6 * NE) {
-        const int v = idx / NE, c = idx % NE;
-        double acc = 0.0;
-#pragma unroll
-        for (int w = 0; w < 6; ++w) acc += s.C[g][sym6(v, w)] * s.M[g][w][c];
-        s.CM[v][c] = acc;
-      }
+      for (int u = 0; u < 6; ++u) acc += s.C[g][sym6(v, u)] * s.T0[u][w];
+      s.CT[v][w] = acc;
     }
     __syncthreads();
-    if constexpr (WANT_K) {
+    // C B (for K_uu), Z = CT^T B (for L), Ch = T0^-T CT (for D), TS = T0^-T S (for Rt)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        if (t + 64 * j < 144) {
-          double acc = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      if (t + 64 * j < 144) {
+        double acc = 0.0, accz = 0.0;
 #pragma unroll
-          for (int w = 0; w < 6; ++w) acc += s.C[g][sym6(bV[j], w)] * s.B[w][bCol[j]];
-          s.CB[bV[j]][bCol[j]] = acc;
+        for (int w = 0; w < 6; ++w) {
+          if constexpr (WANT_K) acc += s.C[g][sym6(bV[j], w)] * s.B[w][bCol[j]];
+          accz += s.CT[w][bV[j]] * s.B[w][bCol[j]];
         }
+        if constexpr (WANT_K) s.CB[bV[j]][bCol[j]] = acc;
+        s.Z[bV[j]][bCol[j]] = accz;
       }
+    }
+    if (t < 36) {
+      const int p = t / 6, q = t % 6;
+      double acc = 0.0;
+#pragma unroll
+      for (int v = 0; v < 6; ++v) acc += s.T0[v][p] * s.CT[v][q];
+      s.Ch[p][q] = acc;
+    } else if (t >= 40 && t < 46) {
+      const int w = t - 40;
+      double acc = 0.0;
+#pragma unroll
+      for (int v = 0; v < 6; ++v) acc += s.T0[v][w] * s.S[g][v];
+      s.TS[w] = acc;
     }
     __syncthreads();
     if constexpr (WANT_K) {
@@ -360,32 +387,16 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
           }
       }
     }
-    // D upper (231 entries) and L (504 entries)
+    // D[i][c] += f_i f_c Ch[kRow_i][kRow_c]; L[i][col] += f_i Z[kRow_i][col]; Rt[i] += f_i TS[kRow_i]
+    const double* fg = s.f[g];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (t + 64 * j < 231) {
-        double acc = 0.0;
+    for (int j = 0; j < 4; ++j)
+      if (t + 64 * j < 231) dacc[j] += fg[dI[j]] * fg[dC[j]] * s.Ch[dKi[j]][dKc[j]];
 #pragma unroll
-        for (int v = 0; v < 6; ++v) acc += s.M[g][v][dI[j]] * s.CM[v][dC[j]];
-        dacc[j] += acc;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int idx = t + 64 * j;
-      if (idx < NE * 24) {
-        const int i = idx / 24, col = idx - 24 * (idx / 24);
-        double acc  = 0.0;
-#pragma unroll
-        for (int v = 0; v < 6; ++v) acc += s.CM[v][i] * s.B[v][col];
-        lacc[j] += acc;
-      }
-    }
+    for (int j = 0; j < 8; ++j)
+      if (t + 64 * j < NE * 24) lacc[j] += fg[lI[j]] * s.Z[lK[j]][lCol[j]];
     if (t < NE) {
-      double acc = 0.0;
-#pragma unroll
-      for (int v = 0; v < 6; ++v) acc += s.M[g][v][t] * s.S[g][v];
-      racc += acc;
+      racc += fg[t] * s.TS[rK];
     } else if (t >= 32 && t < 56) {
       const int col = t - 32;
       double acc    = 0.0;
