@@ -296,7 +296,7 @@ def main():
         pn = pb.hex_block((n, n, n), seed=args.newton_seed)
         mlp = np.fromfile(args.mlp, dtype=np.float32) if os.path.exists(args.mlp) else None
         res, us = {}, {}
-        for mode in NEWTON_MODES:
+        for mode in ("fp64", "fp64_inexact_secant", "mixed_secant", "mixed_mlp"):
             if mode == "mixed_mlp" and mlp is None:
                 continue
             log(f"[bench] newton {mode} ...")
