@@ -659,6 +659,15 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, T* Kebe, T* K) {
   const int mask = dbc == IK_DBC_RAW ? 0 : 1;
   const bool fp32 = sizeof(T) == 4;
   const size_t nc = h->chunkElem.size() - 1;
+  if (nc == 1 && !h->timing) {
+    // one chunk: nothing to overlap, so the gathers follow on the main stream (no cross-stream events)
+    launchChunk(h, h->chunkElem[0], h->chunkElem[1], wantR, Kebe, fp32);
+    launchGatherK<T>(h, h->chunkRow[0], h->chunkRow[1], mask, Kebe, K, h->stream);
+    if (wantR)
+      hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
+                         h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
+    return;
+  }
   if (h->timing) {
     {
       ScopedKernelTimer t(h, KASM_KR, h->stream, (int64_t)nc);
