@@ -170,3 +170,26 @@ def test_invalid_state_reports_element():
     # the handle stays usable
     dev.set_requirement(np.zeros(p.ndof), 0.0)
     assert np.all(dev.vector() == 0.0)
+
+
+@pytest.mark.parametrize("eas", [0, 21])
+def test_chunked_assembly_is_bitwise_identical(monkeypatch, eas):
+    """Element chunks whose gathers overlap the next chunk on the second stream (IKARUS_AMD_CHUNK_ELEMENTS)
+    give bitwise the same K and R as one chunk: every K entry and R component is summed in a fixed order."""
+    p = pb.hex_block((7, 6, 5), jitter=0.05, seed=9, eas=eas)
+    print(f"case: chunked assembly eas={eas}", flush=True)
+    u = 0.01 * np.random.default_rng(4).standard_normal(p.ndof) * (1 - p.fixed)
+    out = []
+    for chunk in (None, "64", "100"):
+        if chunk is None:
+            monkeypatch.delenv("IKARUS_AMD_CHUNK_ELEMENTS", raising=False)
+        else:
+            monkeypatch.setenv("IKARUS_AMD_CHUNK_ELEMENTS", chunk)
+        dev = ia.Device(p)
+        dev.set_requirement(u, 0.7)
+        dev.matrix(capi.DBC_FULL)
+        _, _, vals = dev.csc()
+        out.append((vals.copy(), dev.vector().copy()))
+        dev.close()
+    for vals, R in out[1:]:
+        assert np.array_equal(vals, out[0][0]) and np.array_equal(R, out[0][1])
