@@ -3,9 +3,12 @@
 Workload (default): 3D hex8 Neo-Hooke block, 64x64x64 elements (823,875 DOFs), unit cube, E=1000, nu=0.3,
 seeded +-5% interior jitter (seed 0), bottom face clamped, seeded top-face traction, FP64.
 
-One step = one Newton iteration's assembly of the hot path at a fixed state: residual R, then tangent K
-(FP64, FULL Dirichlet treatment, summed into the 3x3-block CSR pattern), inputs resident in HBM.
-value = elements assembled per second (Melem/s) over the timed steps.
+One step = the residual R and tangent K of one Newton state (the pair NewtonRaphson evaluates per state),
+FP64, FULL Dirichlet treatment, summed into the 3x3-block CSR pattern, inputs resident in HBM, through
+ik_assemble (the tangent's element kernel and gather, then the residual pass; one error check and one
+host synchronisation for the pair).
+value = elements assembled per second (Melem/s) over the timed steps. The same state assembled by the two
+separate calls (ik_vector, then ik_matrix) is timed beside it.
 
 Also reported (same run): the element kernel's roofline figures (HIP events on the library stream; HBM
 traffic per launch from the committed rocprofv3 counter summary), the gather and SpMV bandwidths, the CPU
@@ -30,6 +33,7 @@ from ikarus_amd import problems as pb  # noqa: E402
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_ELEM_KR = 86176.0   # 8 Gauss points x 10,772 (dense B^T C B + kinematics + material + kg + B^T S)
+FLOP_PER_ELEM_K = FLOP_PER_ELEM_KR - 8 * 288.0  # the tangent kernel alone: less B^T S (6x24 by 6) per point
 
 
 def log(msg):
@@ -104,16 +108,21 @@ def newton_path(p, mode, steps, mlp_params=None):
 
 
 def assembly_step(dev):
-    """One Newton iteration's assembly at a fixed state: R (FULL) then K (FULL, FP64), the reference order
-    (newtonraphson.hh:206-223)."""
+    """Residual and tangent of one Newton state (rx = residualFunction_(x), Ax = jacobianFunction_(x),
+    newtonraphson.hh:242-243), FULL, FP64, in one ik_assemble call."""
+    dev.check(capi.lib().ik_assemble(dev.h, capi.DBC_FULL, capi.FP64, None, capi.DEVICE))
+
+
+def separate_step(dev):
+    """The same state through the two assembler calls in the reference order: ik_vector, then ik_matrix."""
     dev.check(capi.lib().ik_vector(dev.h, capi.DBC_FULL, None, capi.DEVICE))
     dev.check(capi.lib().ik_matrix(dev.h, capi.DBC_FULL, capi.FP64))
 
 
 def kernel_pass(dev, p, steps):
     """Per-kernel launch durations from HIP events on the library stream: in timing mode the element chunks of
-    an assembly run back to back between one pair of events, then the gathers between another, so these are
-    the kernels' stand-alone durations (the timed steps above overlap them on two streams)."""
+    an assembly run back to back between one pair of events, then the gathers between another, then the
+    residual pass."""
     dev.timing(True)
     dev.timing_reset()
     for _ in range(steps):
@@ -121,8 +130,7 @@ def kernel_pass(dev, p, steps):
     out = {}
     for name, kid in (("element", capi.KERNEL_ASSEMBLE_KR), ("gather", capi.KERNEL_GATHER_K),
                       ("residual", capi.KERNEL_ASSEMBLE_R)):
-        n, ms = dev.timing_read(kid)
-        out[name] = (n, ms)
+        out[name] = dev.timing_read(kid)
     dev.timing(False)
     return out
 
@@ -171,8 +179,8 @@ def read_traffic(path):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=30, help="untimed steps (the clocks settle over ~20 ms)")
     ap.add_argument("--n", type=int, default=64, help="elements per direction")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-newton", action="store_true")
@@ -236,11 +244,20 @@ def main():
     ms_per_step = dt / args.steps * 1e3
     value = p.ne * args.steps / dt / 1e6
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} Melem/s")
+    for _ in range(args.warmup):
+        separate_step(dev)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        separate_step(dev)
+    dev.synchronize()
+    sep_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    log(f"[bench] separate calls (ik_vector, ik_matrix): {sep_ms:.3f} ms/step, {p.ne / sep_ms / 1e3:.1f} Melem/s")
 
     kp = kernel_pass(dev, p, args.steps)
     ne_, k_ms = kp["element"][0], kp["element"][1] / max(kp["element"][0], 1)
     elems_per_launch = p.ne * args.steps / max(ne_, 1)
-    achieved = FLOP_PER_ELEM_KR * elems_per_launch / (k_ms * 1e-3) / 1e12
+    achieved = FLOP_PER_ELEM_K * elems_per_launch / (k_ms * 1e-3) / 1e12
     ng, g_ms_tot = kp["gather"]
     g_ms = g_ms_tot / max(ng, 1)
     r_ms = kp["residual"][1] / max(kp["residual"][0], 1)
@@ -275,16 +292,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"hex8 Neo-Hooke R+K assembly (one Newton iteration), {n}x{n}x{n} elements, unit "
+        "config": {"workload": f"hex8 Neo-Hooke R+K assembly (one Newton state), {n}x{n}x{n} elements, unit "
                                "cube, 5% jitter seed 0, FULL Dirichlet, 3x3-block CSR", "elements": p.ne,
                    "dofs": p.ndof, "block_nnz": nnzb, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                      "kernel": "hex8Element<NeoHooke,double,K>", "kernel_ms_per_launch": round(k_ms, 4),
-                     "elements_per_launch": int(elems_per_launch), "flop_per_elem": FLOP_PER_ELEM_KR},
+                     "elements_per_launch": int(elems_per_launch), "flop_per_elem": FLOP_PER_ELEM_K},
         "assembly_breakdown_ms_per_step": {
             "element_K": round(kp["element"][1] / args.steps, 4), "gather_K": round(g_ms_tot / args.steps, 4),
-            "residual": round(kp["residual"][1] / args.steps, 4)},
+            "residual": round(r_ms, 4)},
+        "separate_calls": {"ms_per_step": round(sep_ms, 4), "Melem_per_s": round(p.ne / sep_ms / 1e3, 3),
+                           "residual_kernel_ms": round(r_ms, 4)},
         "gather": {"ms_per_launch": round(g_ms, 4), "GBs": round(gather_gbs, 1),
                    "frac_hbm": round(gather_gbs / PEAK_HBM_GBS, 4), "bytes_per_launch": int(gather_bytes)},
         "spmv": {"fp64": {"ms": round(spmv64_ms, 4), "GBs": round(spmv64_gbs, 1),

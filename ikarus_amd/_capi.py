@@ -21,7 +21,7 @@ KERNEL_ASSEMBLE_KR, KERNEL_ASSEMBLE_R, KERNEL_SPMV, KERNEL_MLP, KERNEL_GATHER_K 
 
 EXPORTED = [
     "ik_create", "ik_destroy", "ik_last_error", "ik_bad_element", "ik_size", "ik_reduced_size",
-    "ik_set_requirement", "ik_get_requirement", "ik_vector", "ik_matrix", "ik_matrix_csc", "ik_matrix_bsr",
+    "ik_set_requirement", "ik_get_requirement", "ik_vector", "ik_matrix", "ik_assemble", "ik_matrix_csc", "ik_matrix_bsr",
     "ik_element", "ik_update_state", "ik_get_alpha", "ik_set_alpha", "ik_solve", "ik_newton_solve",
     "ik_load_control", "ik_mlp_num_params", "ik_mlp_set_params", "ik_mlp_get_params", "ik_mlp_set_load",
     "ik_timing_enable", "ik_timing_read", "ik_timing_reset", "ik_synchronize",
@@ -95,6 +95,7 @@ def lib():
         L.ik_get_requirement.argtypes = [P, P, P, I]
         L.ik_vector.argtypes = [P, I, P, I]
         L.ik_matrix.argtypes = [P, I, I]
+        L.ik_assemble.argtypes = [P, I, I, P, I]
         L.ik_matrix_csc.argtypes = [P, I, P, P, P, P, P]
         L.ik_matrix_bsr.argtypes = [P, P, P, P, P, P, P]
         L.ik_element.argtypes = [P, I64, P, P]

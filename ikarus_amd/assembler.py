@@ -124,6 +124,13 @@ class Device:
     def matrix(self, dbc=capi.DBC_FULL, precision=capi.FP64):
         self.check(capi.lib().ik_matrix(self.h, dbc, precision))
 
+    def assemble(self, dbc=capi.DBC_FULL, precision=capi.FP64):
+        """Residual and tangent of the bound state in one call (ik_assemble); returns the residual."""
+        n = self.ndof if dbc != capi.DBC_REDUCED else int(capi.lib().ik_reduced_size(self.h))
+        R = np.zeros(n)
+        self.check(capi.lib().ik_assemble(self.h, dbc, precision, ptr(R), capi.HOST))
+        return R
+
     def csc(self, dbc=capi.DBC_FULL, values=True):
         L = capi.lib()
         n, nnz = C.c_int64(), C.c_int64()

@@ -213,6 +213,8 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
   // accumulators: K01[3k+l] tile (0,1) of K^{kl}; K00/K11[v] tiles (0,0)/(1,1) of K^{kl}, k <= l, v = voigt(k,l);
   // G01[v - 3] the Psi term of tile (0,1) for k < l (shared by K^{kl} and K^{lk})
   double K01[9], K00[NSYM], K11[NSYM], G01[3];
+  // residual: Rq[h] at lane 16 a + 4 bk + c accumulates R_e[3 (a + 4 h) + c], c < 3
+  double Rq[2] = {0.0, 0.0};
   if constexpr (WANT_K) {
 #pragma unroll
     for (int t = 0; t < 9; ++t) K01[t] = 0.0;
@@ -221,7 +223,6 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
 #pragma unroll
     for (int t = 0; t < 3; ++t) G01[t] = 0.0;
   }
-  double rr[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
 
 #pragma unroll
   for (int sblk = 0; sblk < 2; ++sblk) {
@@ -270,10 +271,16 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
 #pragma unroll
     for (int h = 0; h < 2; ++h) symMul(Sv, g[h], sg[h]);
     if constexpr (WANT_R) {
+      // R_ac = sum_gp sum_I g_aI T_cI with T = F S dV: A[a][gp] = g_aI (this lane's node, Gauss point),
+      // B[gp][c] = T_cI (row c = i of T; zero for i = 3), accumulated over I and both Gauss-point blocks
+      double Fr[3], T[3];
+#pragma unroll
+      for (int J = 0; J < 3; ++J) Fr[J] = i == 0 ? F[0][J] : (i == 1 ? F[1][J] : (i == 2 ? F[2][J] : 0.0));
+      symMul(Sv, Fr, T);
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) rr[h][k] += F[k][0] * sg[h][0] + F[k][1] * sg[h][1] + F[k][2] * sg[h][2];
+        for (int I = 0; I < 3; ++I) Rq[h] = __builtin_amdgcn_mfma_f64_4x4x4f64(g[h][I], T[I], Rq[h], 0, 0, 0);
     }
     if constexpr (WANT_K) {
       // M = F Ci, p = M g, Psi = w2 dV M F^T, cg = Ci g
@@ -331,27 +338,16 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
   }
 
   if constexpr (WANT_R) {
-    // sum over the four Gauss-point pairs kk (lanes 16 apart)
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double v = rr[h][k];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        rr[h][k] = v;
-      }
-    if (kk == 0 && valid) {
+    if (valid && i < 3) {
       const unsigned bits = fbits ? fbits[e] : 0u;
-      const double tk[3]  = {trac.x, trac.y, trac.z};
+      const double t      = i == 0 ? trac.x : (i == 1 ? trac.y : trac.z);
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double v = rr[h][k];
-          if (bits) v += faceLoad(X, conn, e, bits, i + 4 * h, tk[k]);  // Traction skill, after the internal forces
-          Rebe[(rdest ? (int64_t)rdest[e * 8 + i + 4 * h] * 3 : e * 24 + 3 * (i + 4 * h)) + k] = v;
-        }
+      for (int h = 0; h < 2; ++h) {
+        const int a = kk + 4 * h;
+        double v    = Rq[h];
+        if (bits) v += faceLoad(X, conn, e, bits, a, t);  // Traction skill, after the internal forces
+        Rebe[(rdest ? (int64_t)rdest[e * 8 + a] * 3 : e * 24 + 3 * a) + i] = v;
+      }
     }
   }
   if constexpr (WANT_K) {
@@ -587,26 +583,27 @@ inline unsigned gridFor(int64_t n, int bs) {
 
 template <typename T, int KIND, bool WK, bool WR>
 void launchElem(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, T* Kebe,
-                double* Rebe) {
+                double* Rebe, hipStream_t s) {
   if constexpr (!WK) {
     const unsigned grid = (unsigned)((count + 7) / 8);
-    hipLaunchKernelGGL((hex8Residual<KIND>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p, h->conn.p, e0,
-                       count, h->lam, h->mu, rdest, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
+    hipLaunchKernelGGL((hex8Residual<KIND>), dim3(grid), dim3(64), 0, s, h->X.p, h->u.p, h->conn.p, e0, count,
+                       h->lam, h->mu, rdest, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
   } else {
     const unsigned grid = (unsigned)((count + EPW - 1) / EPW);
-    hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR>), dim3(grid), dim3(64), 0, h->stream, h->X.p, h->u.p,
-                       h->conn.p, e0, count, h->lam, h->mu, dest, rdest, Kebe, Rebe, h->faceBits.p,
-                       loadedTraction(h), h->errFlag.p);
+    hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR>), dim3(grid), dim3(64), 0, s, h->X.p, h->u.p, h->conn.p, e0,
+                       count, h->lam, h->mu, dest, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h),
+                       h->errFlag.p);
   }
 }
 
 template <typename T, bool WK, bool WR>
 void launchElemMat(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, T* Kebe,
-                   double* Rebe) {
+                   double* Rebe, hipStream_t s = nullptr) {
+  if (!s) s = h->stream;
   if (h->material == IK_SVK)
-    launchElem<T, IK_SVK, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe);
+    launchElem<T, IK_SVK, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe, s);
   else
-    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe);
+    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe, s);
 }
 
 }  // namespace
@@ -650,23 +647,34 @@ static void launchGatherK(ik_handle* h, int64_t r0, int64_t r1, int mask, const 
                      h->upBlk.p, h->upT.p, h->upMask.p, mask, Kebe, q0, q1, K);
 }
 
+// residual-only element pass over the mesh on stream s (the EAS pass always runs on the main stream)
+static void launchResidual(ik_handle* h, hipStream_t s) {
+  if (h->neas)
+    easStage(h, 0, h->ne, false, nullptr, h->rdest.p);
+  else
+    launchElemMat<double, false, true>(h, 0, h->ne, nullptr, h->rdest.p, (double*)nullptr, h->ebeR.p, s);
+}
+static void launchGatherR(ik_handle* h, int mask, int64_t r0, int64_t r1, hipStream_t s) {
+  hipLaunchKernelGGL(gatherR, dim3((unsigned)(((r1 - r0) * 3 + 255) / 256)), dim3(256), 0, s, h->ncptr.p, h->fixed.p,
+                     mask, h->ebeR.p, r0, r1, h->R.p);
+}
+
 // Tangent assembly. Element chunks run on the main stream; each chunk's gathers run on the second stream,
 // released by an event recorded behind the chunk, so the gathers of one chunk overlap the element work of
-// the next while its staged blocks are still cached. In timing mode the chunks and then the gathers run
-// back to back on the main stream, each group between one pair of events, giving stand-alone durations.
+// the next while its staged blocks are still cached. With one chunk the gather follows on the main stream.
+// wantR: the element kernel also stages R_e (EAS), gathered with K. sideR: the residual pass of the plain
+// element follows the K gather on the main stream (run beside the gather on the second stream it slowed
+// both). In timing mode the chunks, the gathers and the residual pass run back to back on the main stream,
+// each group between one pair of events, giving stand-alone durations.
 template <typename T>
-static void runChunks(ik_handle* h, int dbc, bool wantR, T* Kebe, T* K) {
+static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T* K) {
   const int mask = dbc == IK_DBC_RAW ? 0 : 1;
   const bool fp32 = sizeof(T) == 4;
   const size_t nc = h->chunkElem.size() - 1;
-  if (nc == 1 && !h->timing) {
-    // one chunk: nothing to overlap, so the gathers follow on the main stream (no cross-stream events)
-    launchChunk(h, h->chunkElem[0], h->chunkElem[1], wantR, Kebe, fp32);
-    launchGatherK<T>(h, h->chunkRow[0], h->chunkRow[1], mask, Kebe, K, h->stream);
-    if (wantR)
-      hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
-                         h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
-    return;
+  while (h->chunkEvents.size() < nc) {
+    hipEvent_t ev;
+    IK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    h->chunkEvents.push_back(ev);
   }
   if (h->timing) {
     {
@@ -677,15 +685,20 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, T* Kebe, T* K) {
       ScopedKernelTimer t(h, KGATHER, h->stream, (int64_t)nc);
       for (size_t c = 0; c < nc; ++c) launchGatherK<T>(h, h->chunkRow[c], h->chunkRow[c + 1], mask, Kebe, K, h->stream);
     }
-    if (wantR)
-      hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
-                         h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
+    if (sideR) {
+      ScopedKernelTimer t(h, KASM_R, h->stream);
+      launchResidual(h, h->stream);
+    }
+    if (wantR || sideR) launchGatherR(h, mask, 0, h->nn, h->stream);
     return;
   }
-  while (h->chunkEvents.size() < nc) {
-    hipEvent_t ev;
-    IK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    h->chunkEvents.push_back(ev);
+  if (nc == 1) {
+    // one chunk: the K gather follows on the main stream
+    launchChunk(h, h->chunkElem[0], h->chunkElem[1], wantR, Kebe, fp32);
+    launchGatherK<T>(h, h->chunkRow[0], h->chunkRow[1], mask, Kebe, K, h->stream);
+    if (sideR) launchResidual(h, h->stream);
+    if (wantR || sideR) launchGatherR(h, mask, 0, h->nn, h->stream);
+    return;
   }
   for (size_t c = 0; c < nc; ++c) {
     launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], wantR, Kebe, fp32);
@@ -694,13 +707,13 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, T* Kebe, T* K) {
     IK_HIP(hipEventRecord(h->chunkEvents[c], h->stream));
     IK_HIP(hipStreamWaitEvent(h->stream2, h->chunkEvents[c], 0));
     launchGatherK<T>(h, r0, r1, mask, Kebe, K, h->stream2);
-    if (wantR)
-      hipLaunchKernelGGL(gatherR, dim3((unsigned)(((r1 - r0) * 3 + 255) / 256)), dim3(256), 0, h->stream2,
-                         h->ncptr.p, h->fixed.p, mask, h->ebeR.p, r0, r1, h->R.p);
+    if (wantR) launchGatherR(h, mask, r0, r1, h->stream2);
   }
+  if (sideR) launchResidual(h, h->stream);  // behind the last element chunk
   // join: later work on the main stream sees every gathered row
   IK_HIP(hipEventRecord(h->joinEvent, h->stream2));
   IK_HIP(hipStreamWaitEvent(h->stream, h->joinEvent, 0));
+  if (sideR) launchGatherR(h, mask, 0, h->nn, h->stream);
 }
 
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
@@ -711,28 +724,27 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
     // the EAS kernel computes the condensed R_e with K_e; both gathers run when K is wanted
     wantR = true;
   }
-  // the mixed form pairs an FP32 tangent with an FP64 residual: two passes
-  const bool fused = wantK && wantR && precision == IK_FP64;
+  // EAS: one element pass yields K_e and R_e. Plain element: the register-bound tangent kernel and the
+  // residual kernel stay separate (R accumulated in the tangent kernel costs it more registers than the
+  // residual kernel takes time); with both wanted they share one launch sequence and one error check
+  const bool easBoth = h->neas && wantK;
+  const bool sideR   = !h->neas && wantK && wantR;
   if (wantK && precision == IK_FP64) {
     h->K64.ensure((size_t)h->nnzb * 9);
     h->ebeK64.ensure((size_t)h->ne * 324);
-    runChunks<double>(h, dbc, fused, h->ebeK64.p, h->K64.p);
+    runChunks<double>(h, dbc, easBoth, sideR, h->ebeK64.p, h->K64.p);
   } else if (wantK) {
     h->K32.ensure((size_t)h->nnzb * 9);
     h->ebeK32.ensure((size_t)h->ne * 324);
-    runChunks<float>(h, dbc, false, h->ebeK32.p, h->K32.p);
+    runChunks<float>(h, dbc, false, sideR, h->ebeK32.p, h->K32.p);
   }
-  if (wantR && !fused) {
+  if (wantR && !wantK) {
     // residual alone: one element launch over the mesh, then one gather (a few MB; no staging overlap needed)
     {
       ScopedKernelTimer t(h, KASM_R, h->stream);
-      if (h->neas)
-        easStage(h, 0, h->ne, false, nullptr, h->rdest.p);
-      else
-        launchElemMat<double, false, true>(h, 0, h->ne, nullptr, h->rdest.p, (double*)nullptr, h->ebeR.p);
+      launchResidual(h, h->stream);
     }
-    hipLaunchKernelGGL(gatherR, dim3((unsigned)((h->nn * 3 + 255) / 256)), dim3(256), 0, h->stream, h->ncptr.p,
-                       h->fixed.p, mask, h->ebeR.p, (int64_t)0, h->nn, h->R.p);
+    launchGatherR(h, mask, 0, h->nn, h->stream);
   }
   if (h->neas) h->easCacheValid = true;
   if (wantR && h->npl) {

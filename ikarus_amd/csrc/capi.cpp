@@ -290,26 +290,31 @@ int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where) {
   });
 }
 
+// copies the handle's residual out: all DOFs (RAW/FULL) or the free DOFs only (REDUCED)
+static void residualOut(ik_handle* h, int dbc, double* R, int where) {
+  if (!R) return;  // assembled into the handle's residual only
+  if (dbc != IK_DBC_REDUCED) {
+    copyOut(h, R, h->R.p, h->ndof, where);
+    return;
+  }
+  const int64_t nr = h->ndof - h->nfixed;
+  std::vector<int32_t> freeList;
+  freeList.reserve(nr);
+  for (int64_t i = 0; i < h->ndof; ++i)
+    if (!h->hfixed[i]) freeList.push_back((int32_t)i);
+  ik::DevBuf<int32_t> fl;
+  fl.upload(freeList.data(), freeList.size(), h->stream);
+  h->tmp.ensure(std::max<int64_t>(nr, 1));
+  hipLaunchKernelGGL(gatherKernel, dim3(grid(nr)), dim3(256), 0, h->stream, h->R.p, fl.p, nr, h->tmp.p);
+  copyOut(h, R, h->tmp.p, nr, where);
+}
+
 int ik_vector(ik_handle* h, int dbc, double* R, int where) {
   return guard(h, [&] {
     requireBound(h);
     require(dbc >= 0 && dbc <= 2, IK_EINVAL, "bad DBC option");
     assembleR(h, dbc == IK_DBC_FULL ? IK_DBC_FULL : IK_DBC_RAW);
-    if (!R) return;  // assemble into the handle's residual only
-    if (dbc != IK_DBC_REDUCED) {
-      copyOut(h, R, h->R.p, h->ndof, where);
-      return;
-    }
-    const int64_t nr = h->ndof - h->nfixed;
-    std::vector<int32_t> freeList;
-    freeList.reserve(nr);
-    for (int64_t i = 0; i < h->ndof; ++i)
-      if (!h->hfixed[i]) freeList.push_back((int32_t)i);
-    ik::DevBuf<int32_t> fl;
-    fl.upload(freeList.data(), freeList.size(), h->stream);
-    h->tmp.ensure(std::max<int64_t>(nr, 1));
-    hipLaunchKernelGGL(gatherKernel, dim3(grid(nr)), dim3(256), 0, h->stream, h->R.p, fl.p, nr, h->tmp.p);
-    copyOut(h, R, h->tmp.p, nr, where);
+    residualOut(h, dbc, R, where);
   });
 }
 
@@ -320,6 +325,23 @@ int ik_matrix(ik_handle* h, int dbc, int precision) {
     require(precision == IK_FP64 || precision == IK_FP32, IK_EINVAL, "bad precision");
     assembleK(h, dbc == IK_DBC_FULL ? IK_DBC_FULL : IK_DBC_RAW, precision);
     h->kDbc = dbc;
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int ik_assemble(ik_handle* h, int dbc, int precision, double* R, int where) {
+  return guard(h, [&] {
+    requireBound(h);
+    require(dbc >= 0 && dbc <= 2, IK_EINVAL, "bad DBC option");
+    require(precision == IK_FP64 || precision == IK_FP32, IK_EINVAL, "bad precision");
+    ensurePattern(h);
+    const int d = dbc == IK_DBC_FULL ? IK_DBC_FULL : IK_DBC_RAW;
+    // FP64: one element pass produces K_e and R_e; FP32: the FP32 tangent and the FP64 residual are two passes
+    ik::assembleKR(h, d, precision, true, true);
+    h->kPrecision = precision;
+    h->kDbc       = dbc;
+    h->kValid     = true;
+    residualOut(h, dbc, R, where);
     IK_HIP(hipStreamSynchronize(h->stream));
   });
 }

@@ -94,6 +94,13 @@ int ik_vector(ik_handle* h, int dbc, double* R, int where);
  * Assembles into library-owned 3x3-block CSR storage on the device (FP64, or FP32 for IK_FP32). */
 int ik_matrix(ik_handle* h, int dbc, int precision);
 
+/* Residual and tangent at the bound requirement in one call: the pair residualFunction_(x),
+ * jacobianFunction_(x) that NewtonRaphson evaluates at every state (newtonraphson.hh:204-205, 242-243),
+ * i.e. ik_vector(h, dbc, R, where) followed by ik_matrix(h, dbc, precision) with identical results up to
+ * summation order. In FP64 one element pass computes K_e and R_e together; IK_FP32 pairs the FP32 tangent
+ * with the FP64 residual (two passes). */
+int ik_assemble(ik_handle* h, int dbc, int precision, double* R, int where);
+
 /* Eigen ColMajor CSC layout of the last assembled matrix (simpleassemblers.inl:207-251):
  * pass outer/inner/values = NULL to query n and nnz. outer: n+1, inner: nnz (int32), values: nnz (FP64). */
 int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* outer, int32_t* inner,
