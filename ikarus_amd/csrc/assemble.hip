@@ -163,40 +163,66 @@ __device__ inline void storeBlocks(const double* src, TK* dst, int n, int lane) 
 // symmetric-pair index of (k, l) = its Voigt index
 constexpr int NSYM = 6;
 
-// One launch covers elements [e0, e0 + count), EPW per 64-lane workgroup. Outputs per element e:
-//   dest == nullptr: Kebe[(e*36 + sb)*9 + 3k + l] = K_e[3a+k][3b+l] for the upper block sb = upperBlock(a, b)
-//   otherwise block sb goes to slot dest[e*36+sb] >> 1 of the gather order (transposed when bit 0 is set)
-//   Rebe[e*24 + 3a + k]          = R_e[3a+k]
-// The element values are computed in FP64; TK = float rounds the blocks once at the store.
+// Workgroup = 4 wavefronts = one tile of kTileElems elements, EPW per wavefront.
+//  TILED: the workgroup takes tile t = e0 + blockIdx.x, element s = tileElem[16 t + s] (-1: padding). Its
+//   576 upper element blocks go to LDS grouped by the K block they fall on (position epos, in the K block's
+//   orientation); each group is summed in list order and the sum goes to its staging slot gslot[g].
+//  otherwise: elements e0 + 16 blockIdx.x + s below e0 + count, stored in element order,
+//   Kebe[(e*36 + sb)*9 + 3k + l] = K_e[3a+k][3b+l] for the upper block sb = upperBlock(a, b).
+// WANT_R: Rebe[rdest ? rdest[e*8+a]*3 + k : e*24 + 3a + k] = R_e[3a+k] (element order only).
+// The element values are computed in FP64; TK = float rounds the group sums once at the store.
 //
-// Lane = 16 kk + 4 bk + i: element bk, Gauss points kk and kk + 4, nodes i and i + 4. In the
-// component-major ordering (row 8k + a) of K_e, the 8x8 node matrix K^{kl}[a][b] = K_ab[k][l] is
+// Lane = 16 kk + 4 bk + i of wavefront w: element s = 4 w + bk, Gauss points kk and kk + 4, nodes i and
+// i + 4. In the component-major ordering (row 8k + a) of K_e, the 8x8 node matrix K^{kl}[a][b] =
+// K_ab[k][l] is
 //   K^{kl} = sum_g [ w1 p_k p_l^T + w2 p_l p_k^T ] + sum_{g,I} g_I (Psi_kl Ci g + d_kl S g)_I^T,
 // (p_k = (p_ak)_a, g_I = (g_aI)_a over the nodes) so with the contraction index (gauss point, term) every
 // operand of v_mfma_f64_4x4x4 is a value the lane computed itself: A[a][k] at lane 16 k + 4 bk + a,
 // B[k][b] at lane 16 k + 4 bk + b, D[a][b] at lane 16 a + 4 bk + b. Tiles (alpha, beta) of K^{kl} cover
 // nodes 4 alpha..+3 x 4 beta..+3; (0,1) for all kl, (0,0) and (1,1) for k <= l (the rest follow from
 // K_ab[k][l] = K_ba[l][k]): 21 tiles x (2 gauss blocks x (2 W terms + 3 I)) contraction steps.
-template <int KIND, typename TK, bool WANT_K, bool WANT_R>
-__global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
-                                                     const int32_t* __restrict__ conn, int64_t e0, int64_t count,
-                                                     double lam, double mu, const int32_t* __restrict__ dest,
-                                                     const int32_t* __restrict__ rdest, TK* __restrict__ Kebe,
-                                                     double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
-                                                     double3 trac, unsigned long long* __restrict__ err) {
-  const int lane     = threadIdx.x;
-  const int kk       = lane >> 4, bk = (lane >> 2) & 3, i = lane & 3;
-  const int64_t base = (int64_t)blockIdx.x * EPW;
-  const int nvalid   = (int)(count - base < EPW ? count - base : EPW);
-  const bool valid   = bk < nvalid;
-  // lanes of a missing tail element compute the last valid element; only their stores are masked
-  const int64_t e = e0 + base + (valid ? bk : nvalid - 1);
+template <int KIND, typename TK, bool WANT_K, bool WANT_R, bool TILED>
+__global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __restrict__ X, const double* __restrict__ U,
+                                                      const int32_t* __restrict__ conn,
+                                                      const int32_t* __restrict__ tileElem, int64_t e0,
+                                                      int64_t count, double lam, double mu,
+                                                      const int32_t* __restrict__ tgptr,
+                                                      const int32_t* __restrict__ gslot,
+                                                      const uint16_t* __restrict__ epos,
+                                                      const uint16_t* __restrict__ goff,
+                                                      const int32_t* __restrict__ rdest, TK* __restrict__ Kebe,
+                                                      double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
+                                                      double3 trac, unsigned long long* __restrict__ err) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kk = lane >> 4, bk = (lane >> 2) & 3, i = lane & 3;
+  const int s  = EPW * wave + bk;  // element of the tile
+  const int64_t t  = e0 + blockIdx.x;
+  int64_t e;
+  bool valid;
+  int nvalidWave = 0;  // element-order stores: valid elements of this wavefront
+  if constexpr (TILED) {
+    const int32_t ev = tileElem[t * kTileElems + s];
+    valid            = ev >= 0;
+    // lanes of a padding slot compute the tile's first element; only their stores are dropped
+    e = valid ? ev : tileElem[t * kTileElems];
+  } else {
+    const int64_t base = (int64_t)blockIdx.x * kTileElems + EPW * wave;
+    nvalidWave         = (int)(count - base < EPW ? (count - base < 0 ? 0 : count - base) : EPW);
+    valid              = bk < nvalidWave;
+    const int64_t idx  = (int64_t)blockIdx.x * kTileElems + s;
+    e                  = e0 + (idx < count ? idx : count - 1);
+  }
 
-  // staging slots of the wavefront's element blocks, fetched early so the stores do not wait on them
-  __shared__ int32_t codes[EPW * 36];
-  if constexpr (WANT_K) {
-    if (dest != nullptr)
-      for (int j = lane; j < nvalid * 36; j += 64) codes[j] = dest[(e0 + base) * 36 + j];
+  // the tile's entry positions, group offsets and slots, fetched early so the stores do not wait on them
+  __shared__ uint16_t ep[kTileEntries], go[kTileEntries + 1];
+  __shared__ int32_t gs[kTileEntries];
+  int ng = 0;
+  if constexpr (WANT_K && TILED) {
+    const int32_t g0 = tgptr[t];
+    ng               = tgptr[t + 1] - g0;
+    for (int j = threadIdx.x; j < kTileEntries; j += kTileThreads) ep[j] = epos[t * kTileEntries + j];
+    for (int j = threadIdx.x; j <= ng; j += kTileThreads) go[j] = goff[g0 + t + j];
+    for (int j = threadIdx.x; j < ng; j += kTileThreads) gs[j] = gslot[g0 + j];
   }
 
   double x[2][3], u[2][3];
@@ -358,52 +384,75 @@ __global__ __launch_bounds__(64, 3) void hex8Element(const double* __restrict__ 
       K01[3 * l + k] += G01[v - 3];
     }
     // result lane = 16 r + 4 bk + c holds rows a = r (+4), columns b = c (+4); every upper-block entry is
-    // written once into LDS, then the 4 elements leave with 16-byte coalesced stores
-    __shared__ __align__(16) double ebe[EPW][324];
+    // written once into LDS: TILED at its entry's list position in the K block's orientation, otherwise
+    // entry j = 36 s + sb at ebe[j*9 ..]
+    __shared__ __align__(16) double ebe[kTileElems][324];
     {
       const int r = kk, c = i;
-      double* out = ebe[bk];
+      double* lds = &ebe[0][0];
+      // LDS base of element block sb and whether its entries are stored transposed
+      auto place = [&](int sb, int& base, bool& tr) {
+        if constexpr (TILED) {
+          const unsigned P = ep[36 * s + sb];
+          base             = (int)(P & 0x7FFFu) * 9;
+          tr               = (P >> 15) != 0;
+        } else {
+          base = (36 * s + sb) * 9;
+          tr   = false;
+        }
+      };
       {
-        double* blk = out + upperBlock(r, c + 4) * 9;
+        int base;
+        bool tr;
+        place(upperBlock(r, c + 4), base, tr);
 #pragma unroll
-        for (int t = 0; t < 9; ++t) blk[t] = K01[t];
+        for (int t = 0; t < 9; ++t) lds[base + (tr ? 3 * (t % 3) + t / 3 : t)] = K01[t];
       }
 #pragma unroll
       for (int al = 0; al < 2; ++al) {
         const int a = r + 4 * al, b = c + 4 * al;
+        int base;
+        bool tr;
+        place(r < c ? upperBlock(a, b) : (r == c ? upperBlock(a, a) : upperBlock(b, a)), base, tr);
 #pragma unroll
         for (int v = 0; v < 6; ++v) {
           const int k = vI(v), l = vJ(v);
           const double val = al ? K11[v] : K00[v];
-          if (r < c) {
-            out[upperBlock(a, b) * 9 + 3 * k + l] = val;
-          } else if (r == c) {
-            out[upperBlock(a, a) * 9 + 3 * k + l] = val;
-            if (k != l) out[upperBlock(a, a) * 9 + 3 * l + k] = val;
-          } else if (k != l) {
-            out[upperBlock(b, a) * 9 + 3 * l + k] = val;
-          }
+          // entry (k, l) of block (a, b) for r < c and of the symmetric diagonal block for r == c (with its
+          // mirror), entry (l, k) of block (b, a) for r > c
+          const int kl = r > c ? 3 * l + k : 3 * k + l, lk = r > c ? 3 * k + l : 3 * l + k;
+          if (r <= c || k != l) lds[base + (tr ? lk : kl)] = val;
+          if (r == c && k != l) lds[base + lk] = val;
         }
       }
     }
     __syncthreads();
-    if (dest == nullptr) {
-      storeBlocks<TK>(&ebe[0][0], Kebe + (e0 + base) * 324, nvalid * 324, lane);
-    } else if (lane < 63) {
-      // staging order: 9 lanes per block, 7 blocks per pass, each block to its slot (transposed if flagged)
-      const int t = lane % 9, k = t / 3, l = t - 3 * (t / 3);
-      for (int j = lane / 9; j < nvalid * 36; j += 7) {
-        const int el = j / 36, sb = j - 36 * (j / 36);
-        const int32_t code = codes[j];
-        Kebe[(int64_t)(code >> 1) * 9 + t] = (TK)ebe[el][sb * 9 + ((code & 1) ? 3 * l + k : t)];
+    if constexpr (!TILED) {
+      if (nvalidWave > 0)
+        storeBlocks<TK>(&ebe[EPW * wave][0], Kebe + (e0 + (int64_t)blockIdx.x * kTileElems + EPW * wave) * 324,
+                        nvalidWave * 324, lane);
+    } else if (threadIdx.x < kTileThreads / 9 * 9) {
+      // group sums: 9 lanes per group (entry tt = 3k + l of the 3x3 block), kTileThreads / 9 groups per pass; a group's
+      // blocks are contiguous in LDS and added in list order; groups come ordered by slot, so a pass writes
+      // ascending slots
+      const double* lds = &ebe[0][0];
+      const int tt      = threadIdx.x % 9;
+      for (int g = threadIdx.x / 9; g < ng; g += kTileThreads / 9) {
+        const int c1 = go[g + 1];
+        double v     = 0.0;
+        for (int c = go[g]; c < c1; ++c) v += lds[c * 9 + tt];
+        Kebe[(int64_t)gs[g] * 9 + tt] = (TK)v;
       }
     }
   }
 }
 
-// Residual-only element kernel (R_e = sum_gp F (S dV) g_a): lane (el, gp) = 8 elements x 8 Gauss points per
-// wavefront, nodes staged in LDS; lane (el, a) then sums node a's contributions over the Gauss points in a
-// fixed order.
+// Residual-only element kernel (R_e[3a+k] = sum_gp (F S dV g_a)_k): lane (el, gp) = 8 elements x 8 Gauss
+// points per wavefront, nodes staged in LDS. With dN_a the reference-cube gradients, J = sum_a x_a dN_a^T and
+// H = sum_a u_a dN_a^T, the point needs F = I + H J^-1 and, per corner, F S dV g_a = Q dN_a with
+// Q = F S dV J^-T (g_a = J^-T dN_a is never formed). The eight Gauss-point lanes of an element then
+// reduce-scatter their 8 x 3 partials (exchanges at lane distance 4, 2, 1, a fixed order), which leaves
+// corner q's three sums on lane q.
 template <int KIND>
 __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X, const double* __restrict__ U,
                                                    const int32_t* __restrict__ conn, int64_t e0, int64_t count,
@@ -411,7 +460,6 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
                                                    double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
                                                    double3 trac, unsigned long long* __restrict__ err) {
   __shared__ double XU[8][8][6];
-  __shared__ double part[8][8][25];  // [element][gauss point][3a+k] (+1 pad)
   const int lane     = threadIdx.x;
   const int el       = lane >> 3, q = lane & 7;
   const int64_t base = (int64_t)blockIdx.x * 8;
@@ -429,7 +477,7 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
   __syncthreads();
   double xi[3];
   gaussPoint(q, xi);
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     double dN[3];
@@ -437,47 +485,66 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) J[i][j] += XU[el][a][i] * dN[j];
+      for (int j = 0; j < 3; ++j) {
+        J[i][j] += XU[el][a][i] * dN[j];
+        H[i][j] += XU[el][a][3 + i] * dN[j];
+      }
   }
   const double detJ = det3(J);
   double Ji[3][3];
   inv3(J, detJ, Ji);
-  double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double F[3][3];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    double dN[3], g[3];
-    dShape(a, xi, dN);
+  for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g[i] = Ji[0][i] * dN[0] + Ji[1][i] * dN[1] + Ji[2][i] * dN[2];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) F[i][j] += XU[el][a][3 + i] * g[j];
-  }
+    for (int j = 0; j < 3; ++j)
+      F[i][j] = (i == j ? 1.0 : 0.0) + H[i][0] * Ji[0][j] + H[i][1] * Ji[1][j] + H[i][2] * Ji[2][j];
   double Sv[6], Ci[6], w2;
   if (!material<KIND>(lam, mu, F, Sv, Ci, w2) && valid) atomicMin(err, (unsigned long long)e);
   const double dV = fabs(detJ) * 0.125;
 #pragma unroll
   for (int v = 0; v < 6; ++v) Sv[v] *= dV;
+  // Q = (F S dV) J^-T: row k of F S dV is S dV times row k of F (S symmetric)
+  double Q[3][3];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    double dN[3], g[3], sg[3];
-    dShape(a, xi, dN);
+  for (int k = 0; k < 3; ++k) {
+    const double row[3] = {F[k][0], F[k][1], F[k][2]};
+    double T[3];
+    symMul(Sv, row, T);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g[i] = Ji[0][i] * dN[0] + Ji[1][i] * dN[1] + Ji[2][i] * dN[2];
-    symMul(Sv, g, sg);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) part[el][q][3 * a + k] = F[k][0] * sg[0] + F[k][1] * sg[1] + F[k][2] * sg[2];
+    for (int j = 0; j < 3; ++j) Q[k][j] = T[0] * Ji[j][0] + T[1] * Ji[j][1] + T[2] * Ji[j][2];
   }
-  __syncthreads();
+  // reduce-scatter over the element's eight lanes: keep the half of the corners selected by this lane's bit,
+  // add the partner's partials of that half
+  const bool b2 = (q >> 2) & 1, b1 = (q >> 1) & 1, b0 = q & 1;
+  double n4[4][3];  // corners j + 4 b2
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double dlo[3], dhi[3];
+    dShape(j, xi, dlo);
+    dShape(j + 4, xi, dhi);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double rlo = Q[k][0] * dlo[0] + Q[k][1] * dlo[1] + Q[k][2] * dlo[2];
+      const double rhi = Q[k][0] * dhi[0] + Q[k][1] * dhi[1] + Q[k][2] * dhi[2];
+      n4[j][k]         = (b2 ? rhi : rlo) + __shfl_xor(b2 ? rlo : rhi, 4);
+    }
+  }
+  double n2[2][3];  // corners j + 2 b1 + 4 b2
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      n2[j][k] = (b1 ? n4[j + 2][k] : n4[j][k]) + __shfl_xor(b1 ? n4[j][k] : n4[j + 2][k], 2);
+  double r[3];  // corner q
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r[k] = (b0 ? n2[1][k] : n2[0][k]) + __shfl_xor(b0 ? n2[0][k] : n2[1][k], 1);
   if (valid) {
     const unsigned bits = fbits ? fbits[e] : 0u;
     const double tk[3]  = {trac.x, trac.y, trac.z};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      double acc = 0.0;
-#pragma unroll
-      for (int g = 0; g < 8; ++g) acc += part[el][g][3 * q + k];
+      double acc = r[k];
       if (bits) acc += faceLoad(X, conn, e, bits, q, tk[k]);  // Traction skill, after the internal forces
       Rebe[(rdest ? (int64_t)rdest[e * 8 + q] * 3 : e * 24 + 3 * q) + k] = acc;
     }
@@ -485,9 +552,9 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
 }
 
 // K gather: thread per (upper block, entry) over the upper blocks [q0, q1) (in row order). The staged
-// blocks of upper block q occupy the contiguous slots [segptr[q], segptr[q+1]) (ordered by element), so
-// consecutive threads read consecutive values. The entry's sum is treated with FULL (rows/columns of
-// constrained DOFs zero, unit diagonal; simpleassemblers.inl:152-169) and written to the block and,
+// blocks of upper block q occupy the contiguous slots [segptr[q], segptr[q+1]) (ordered by element or
+// tile), so consecutive threads read consecutive values. The entry's sum is treated with FULL (rows/columns
+// of constrained DOFs zero, unit diagonal; simpleassemblers.inl:152-169) and written to the block and,
 // transposed, to its mirror block below the diagonal (K and the Dirichlet treatment are symmetric).
 template <typename T>
 __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segptr, const int32_t* __restrict__ upBlk,
@@ -503,8 +570,8 @@ __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segpt
   int32_t c        = segptr[q];
   const int32_t c1 = segptr[q + 1];
   T acc            = T(0);
-  // segments hold one block per element sharing the two nodes (8 at most on hexahedral meshes whose
-  // nodes have 8 elements); the loads of a group of 8 are issued together, then summed in order
+  // segments hold one block per element (or tile) sharing the two nodes (8 at most on hexahedral meshes
+  // whose nodes have 8 elements); the loads of a group of 8 are issued together, then summed in order
   for (; c < c1; c += 8) {
     T v[8];
 #pragma unroll
@@ -581,29 +648,36 @@ inline unsigned gridFor(int64_t n, int bs) {
   return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
 }
 
+// tiled: tangent staging over tiles [e0, e0 + count); otherwise elements [e0, e0 + count) in element order
 template <typename T, int KIND, bool WK, bool WR>
-void launchElem(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, T* Kebe,
-                double* Rebe, hipStream_t s) {
+void launchElem(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32_t* rdest, T* Kebe, double* Rebe,
+                hipStream_t s) {
   if constexpr (!WK) {
     const unsigned grid = (unsigned)((count + 7) / 8);
     hipLaunchKernelGGL((hex8Residual<KIND>), dim3(grid), dim3(64), 0, s, h->X.p, h->u.p, h->conn.p, e0, count,
                        h->lam, h->mu, rdest, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
+  } else if constexpr (!WR) {
+    // tangent staging over tiles (the residual instance below serves single elements in element order)
+    (void)tiled;
+    hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR, true>), dim3((unsigned)count), dim3(kTileThreads), 0, s, h->X.p,
+                       h->u.p, h->conn.p, h->tileElem.p, e0, count, h->lam, h->mu, h->tgptr.p, h->gslot.p,
+                       h->epos.p, h->goff.p, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
   } else {
-    const unsigned grid = (unsigned)((count + EPW - 1) / EPW);
-    hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR>), dim3(grid), dim3(64), 0, s, h->X.p, h->u.p, h->conn.p, e0,
-                       count, h->lam, h->mu, dest, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h),
-                       h->errFlag.p);
+    const unsigned grid = (unsigned)((count + kTileElems - 1) / kTileElems);
+    hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR, false>), dim3(grid), dim3(kTileThreads), 0, s, h->X.p, h->u.p,
+                       h->conn.p, nullptr, e0, count, h->lam, h->mu, nullptr, nullptr, nullptr, nullptr, rdest,
+                       Kebe, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
   }
 }
 
 template <typename T, bool WK, bool WR>
-void launchElemMat(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, const int32_t* rdest, T* Kebe,
+void launchElemMat(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32_t* rdest, T* Kebe,
                    double* Rebe, hipStream_t s = nullptr) {
   if (!s) s = h->stream;
   if (h->material == IK_SVK)
-    launchElem<T, IK_SVK, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe, s);
+    launchElem<T, IK_SVK, WK, WR>(h, e0, count, tiled, rdest, Kebe, Rebe, s);
   else
-    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, dest, rdest, Kebe, Rebe, s);
+    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, tiled, rdest, Kebe, Rebe, s);
 }
 
 }  // namespace
@@ -629,15 +703,14 @@ static void checkErrFlag(ik_handle* h) {
   }
 }
 
-static void launchChunk(ik_handle* h, int64_t e0, int64_t e1, bool wantR, void* Kebe, bool fp32) {
+// staging units [u0, u1): elements (EAS, K_e and R_e) or tiles (plain element, K_e)
+static void launchChunk(ik_handle* h, int64_t u0, int64_t u1, void* Kebe, bool fp32) {
   if (h->neas)
-    easStage(h, e0, e1 - e0, true, h->bdest.p, h->rdest.p);
+    easStage(h, u0, u1 - u0, true, h->bdest.p, h->rdest.p);
   else if (fp32)
-    launchElemMat<float, true, false>(h, e0, e1 - e0, h->bdest.p, nullptr, (float*)Kebe, nullptr);
-  else if (wantR)
-    launchElemMat<double, true, true>(h, e0, e1 - e0, h->bdest.p, h->rdest.p, (double*)Kebe, h->ebeR.p);
+    launchElemMat<float, true, false>(h, u0, u1 - u0, true, nullptr, (float*)Kebe, nullptr);
   else
-    launchElemMat<double, true, false>(h, e0, e1 - e0, h->bdest.p, nullptr, (double*)Kebe, nullptr);
+    launchElemMat<double, true, false>(h, u0, u1 - u0, true, nullptr, (double*)Kebe, nullptr);
 }
 
 template <typename T>
@@ -652,7 +725,7 @@ static void launchResidual(ik_handle* h, hipStream_t s) {
   if (h->neas)
     easStage(h, 0, h->ne, false, nullptr, h->rdest.p);
   else
-    launchElemMat<double, false, true>(h, 0, h->ne, nullptr, h->rdest.p, (double*)nullptr, h->ebeR.p, s);
+    launchElemMat<double, false, true>(h, 0, h->ne, false, h->rdest.p, (double*)nullptr, h->ebeR.p, s);
 }
 static void launchGatherR(ik_handle* h, int mask, int64_t r0, int64_t r1, hipStream_t s) {
   hipLaunchKernelGGL(gatherR, dim3((unsigned)(((r1 - r0) * 3 + 255) / 256)), dim3(256), 0, s, h->ncptr.p, h->fixed.p,
@@ -679,7 +752,7 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T*
   if (h->timing) {
     {
       ScopedKernelTimer t(h, KASM_KR, h->stream, (int64_t)nc);
-      for (size_t c = 0; c < nc; ++c) launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], wantR, Kebe, fp32);
+      for (size_t c = 0; c < nc; ++c) launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], Kebe, fp32);
     }
     {
       ScopedKernelTimer t(h, KGATHER, h->stream, (int64_t)nc);
@@ -694,14 +767,14 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T*
   }
   if (nc == 1) {
     // one chunk: the K gather follows on the main stream
-    launchChunk(h, h->chunkElem[0], h->chunkElem[1], wantR, Kebe, fp32);
+    launchChunk(h, h->chunkElem[0], h->chunkElem[1], Kebe, fp32);
     launchGatherK<T>(h, h->chunkRow[0], h->chunkRow[1], mask, Kebe, K, h->stream);
     if (sideR) launchResidual(h, h->stream);
     if (wantR || sideR) launchGatherR(h, mask, 0, h->nn, h->stream);
     return;
   }
   for (size_t c = 0; c < nc; ++c) {
-    launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], wantR, Kebe, fp32);
+    launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], Kebe, fp32);
     const int64_t r0 = h->chunkRow[c], r1 = h->chunkRow[c + 1];
     if (r1 <= r0) continue;
     IK_HIP(hipEventRecord(h->chunkEvents[c], h->stream));
@@ -731,11 +804,11 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
   const bool sideR   = !h->neas && wantK && wantR;
   if (wantK && precision == IK_FP64) {
     h->K64.ensure((size_t)h->nnzb * 9);
-    h->ebeK64.ensure((size_t)h->ne * 324);
+    h->ebeK64.ensure((size_t)h->nstage * 9);
     runChunks<double>(h, dbc, easBoth, sideR, h->ebeK64.p, h->K64.p);
   } else if (wantK) {
     h->K32.ensure((size_t)h->nnzb * 9);
-    h->ebeK32.ensure((size_t)h->ne * 324);
+    h->ebeK32.ensure((size_t)h->nstage * 9);
     runChunks<float>(h, dbc, false, sideR, h->ebeK32.p, h->K32.p);
   }
   if (wantR && !wantK) {
@@ -761,7 +834,7 @@ void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re) {
   if (h->neas)
     easStage(h, e, 1, true, nullptr, nullptr);
   else
-    launchElemMat<double, true, true>(h, e, 1, nullptr, nullptr, h->ebeK64.p, h->ebeR.p);
+    launchElemMat<double, true, true>(h, e, 1, false, nullptr, h->ebeK64.p, h->ebeR.p);
   checkErrFlag(h);
   DevBuf<double> kd, rd;
   kd.ensure(576);

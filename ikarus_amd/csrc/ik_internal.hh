@@ -72,6 +72,15 @@ enum KernelId { KASM_KR = 0, KASM_R = 1, KSPMV = 2, KMLP = 3, KGATHER = 4, KCOUN
 // Reduction partials: one double per block, final fold in fixed order (deterministic).
 constexpr int kReduceBlocks = 1024;
 
+// elements per tile of the plain element's tangent kernel (one workgroup, 4 elements per 64-lane wavefront)
+#ifndef IK_TILE_ELEMS
+#define IK_TILE_ELEMS 16
+#endif
+constexpr int kTileElems   = IK_TILE_ELEMS;
+constexpr int kTileThreads = kTileElems * 16;
+constexpr int kTileEntries = kTileElems * 36;  // upper element blocks per tile
+static_assert(kTileElems % 4 == 0 && kTileElems <= 16, "tiles hold whole wavefronts, entries fit 10 bits");
+
 }  // namespace ik
 
 struct ik_handle {
@@ -117,12 +126,26 @@ struct ik_handle {
   // FULL Dirichlet treatment per upper block: bit t (t = 3k+l) zeroes entry t (constrained row or column),
   // bit 9+k sets diagonal entry (k,k) of a diagonal block to one
   ik::DevBuf<uint16_t> upMask;
-  // staging order of the element blocks: upper block by upper block, each segment ordered by element.
-  // bdest[e*36+sb] = (slot << 1) | transposed, segptr[q] = first slot of upper block q. Residual staging:
-  // node by node, each node's (element, corner) incidences ordered by element; rdest[e*8+a] = slot,
-  // ncptr[n] = first slot of node n
+  // Tangent staging: slots ordered upper block by upper block (segptr[q] = first slot of upper block q),
+  // each segment in a fixed order, so the gather reads contiguous runs and sums them deterministically.
+  //  EAS element: one slot per element block; bdest[e*36+sb] = (slot << 1) | transposed, segments ordered
+  //  by element.
+  //  Plain element (tiled): elements grouped in tiles of kTileElems (Morton order of the element centroids);
+  //  the blocks of a tile that fall on the same upper block form a group, summed in the tile before staging
+  //  (one slot per group, segments ordered by tile). tileElem[t*16+s] = element (-1: padding of the last
+  //  tile); tgptr[t] = first group of tile t (groups ordered by upper block), gslot[g] = the slot of group
+  //  g. The tile's entries j = 36 s + sb are listed group by group (each group's in increasing j);
+  //  epos[t*576 + j] = the position of entry j in that list | transposed << 15 (padding entries keep
+  //  position j, past the listed ones), and the offsets of tile t's groups in the list are
+  //  goff[tgptr[t] + t + i], i = 0..ng (the last one ends it).
+  // Residual staging: node by node, each node's (element, corner) incidences ordered by element;
+  // rdest[e*8+a] = slot, ncptr[n] = first slot of node n
   ik::DevBuf<int32_t> bdest, segptr, ncptr, rdest;
-  // element chunks and the block rows that are complete after each chunk
+  bool tiled = false;
+  int64_t ntiles = 0, nstage = 0;  // tiles; staging slots of the tangent
+  ik::DevBuf<int32_t> tileElem, tgptr, gslot;
+  ik::DevBuf<uint16_t> epos, goff;
+  // staging chunks (tiles when tiled, elements otherwise) and the block rows complete after each chunk
   std::vector<int64_t> chunkElem, chunkRow;
   // element-by-element staging of K_e (36 upper 3x3 blocks) and R_e (24)
   ik::DevBuf<double> ebeK64, ebeR;

@@ -4,10 +4,11 @@
 // and keeps per element (3*8)^2 positions into the CSC value array (simpleassemblers.inl:207-222,
 // 254-265; eigensparseaddon.hh:19-29). For FlatInterleaved DOFs (dof = 3*node + k, fehelper.hh:146-152)
 // that scalar pattern is exactly the node-adjacency pattern expanded by 3x3 blocks, so the device keeps
-// the block form: block row = node, sorted block columns, and per hex8 element 36 int32 staging slots
-// (one per upper element block, pointing into the gather order of its K block). The scalar CSC is
-// re-derived from it on demand.
+// the block form: block row = node, sorted block columns, and the maps from element blocks to staging
+// slots in the gather order of their K blocks (per element for EAS, per tile group for the plain
+// element). The scalar CSC is re-derived from it on demand.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 #include "ik_internal.hh"
@@ -25,6 +26,63 @@ static int64_t chunkElements() {
     if (c >= 64) return (int64_t)c;
   }
   return kChunkElements;
+}
+
+// the low 21 bits of v spread to every third bit (bit i -> bit 3i)
+static uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1FFFFFu;
+  x          = (x | x << 32) & 0x1F00000000FFFFull;
+  x          = (x | x << 16) & 0x1F0000FF0000FFull;
+  x          = (x | x << 8) & 0x100F00F00F00F00Full;
+  x          = (x | x << 4) & 0x10C30C30C30C30C3ull;
+  x          = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+
+// Elements in Morton (Z) order of their centroids on a lattice of the median element extent per axis, ties
+// by element index. An element's extent along axis d is the largest |component d| of its three mean edge
+// vectors (the average of the four edges along each reference axis), which node jitter leaves unbiased.
+// On a structured grid the lattice is then the grid itself, so 16 consecutive elements form a 4x2x2 box
+// (45 nodes) whose 576 upper element blocks fall on 341 distinct K blocks; on other meshes the order still
+// keeps each tile compact.
+static std::vector<int32_t> mortonOrder(const ik_handle* h) {
+  const int64_t ne = h->ne, nn = h->nn;
+  const double* X  = h->hX.data();
+  const int32_t* conn = h->hconn.data();
+  double lo[3], step[3];
+  std::vector<double> ext(ne);
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = X[d];
+    for (int64_t n = 1; n < nn; ++n) lo[d] = std::min(lo[d], X[n * 3 + d]);
+    for (int64_t e = 0; e < ne; ++e) {
+      double best = 0.0;
+      for (int m = 0; m < 3; ++m) {
+        double v = 0.0;  // 4 x component d of the mean edge vector along reference axis m
+        for (int a = 0; a < 8; ++a)
+          if (!((a >> m) & 1)) v += X[conn[e * 8 + (a | (1 << m))] * 3 + d] - X[conn[e * 8 + a] * 3 + d];
+        best = std::max(best, std::fabs(0.25 * v));
+      }
+      ext[e] = best;
+    }
+    std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
+    step[d] = ext[ne / 2] > 0.0 ? ext[ne / 2] : 1.0;
+  }
+  std::vector<std::pair<uint64_t, int32_t>> key(ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    uint64_t k = 0;
+    for (int d = 0; d < 3; ++d) {
+      double c = 0.0;
+      for (int a = 0; a < 8; ++a) c += X[conn[e * 8 + a] * 3 + d];
+      const double q = std::floor((0.125 * c - lo[d]) / step[d]);
+      const uint32_t qi = q <= 0.0 ? 0u : (q >= 2097151.0 ? 2097151u : (uint32_t)q);
+      k |= spread3(qi) << d;
+    }
+    key[e] = {k, (int32_t)e};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> order(ne);
+  for (int64_t i = 0; i < ne; ++i) order[i] = key[i].second;
+  return order;
 }
 
 void buildPattern(ik_handle* h) {
@@ -68,8 +126,8 @@ void buildPattern(ik_handle* h) {
   // Element e computes its 36 upper blocks (a <= b) sb(a,b) = 8a - a(a-1)/2 + (b-a). Block (node_a, node_b)
   // is an upper block of K when node_a <= node_b and receives the element block as it is; otherwise the
   // upper block (node_b, node_a) receives its transpose. Staging slots are grouped by upper block and
-  // ordered by element inside each group, so the gather reads contiguous segments and sums them in a
-  // fixed order.
+  // ordered by staging unit (tile or element) inside each group, so the gather reads contiguous segments
+  // and sums them in a fixed order.
   require((int64_t)ne * 36 < (int64_t)INT32_MAX / 2, IK_EINVAL, "mesh too large for int32 staging slots");
   std::vector<int32_t> slots((size_t)ne * 36);  // block index p of each element block
   std::vector<int32_t> kcnt(h->nnzb + 1, 0);
@@ -95,15 +153,20 @@ void buildPattern(ik_handle* h) {
       up[q]           = p;
       upT[q]          = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
     }
-  // segments in upper-block order, then each element block's slot (elements visited in increasing order)
-  std::vector<int32_t> seg(h->huptr[nn] + 1, 0);
-  for (int64_t q = 0; q < h->huptr[nn]; ++q) seg[q + 1] = seg[q] + kcnt[up[q] + 1];
+  const int64_t nup = h->huptr[nn];
   std::vector<int32_t> qOf(h->nnzb, -1);
-  for (int64_t q = 0; q < h->huptr[nn]; ++q) qOf[up[q]] = (int32_t)q;
-  std::vector<int32_t> dest((size_t)ne * 36);
-  {
+  for (int64_t q = 0; q < nup; ++q) qOf[up[q]] = (int32_t)q;
+  std::vector<int32_t> seg(nup + 1, 0);
+  std::vector<int64_t> unitOf(ne);  // staging unit of each element: its tile (tiled) or itself
+  int64_t nunits = ne;
+  h->tiled = h->neas == 0;
+  if (!h->tiled) {
+    // EAS: one slot per element block, segments in upper-block order, elements in increasing order
+    for (int64_t q = 0; q < nup; ++q) seg[q + 1] = seg[q] + kcnt[up[q] + 1];
+    std::vector<int32_t> dest((size_t)ne * 36);
     std::vector<int32_t> pos(seg.begin(), seg.end() - 1);
-    for (int64_t e = 0; e < ne; ++e)
+    for (int64_t e = 0; e < ne; ++e) {
+      unitOf[e] = e;
       for (int a = 0; a < 8; ++a)
         for (int c = a; c < 8; ++c) {
           const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
@@ -111,9 +174,138 @@ void buildPattern(ik_handle* h) {
           const bool tr   = conn[e * 8 + a] > conn[e * 8 + c];
           dest[e * 36 + sb] = (pos[q]++ << 1) | (tr ? 1 : 0);
         }
-    for (int64_t q = 0; q < h->huptr[nn]; ++q)
+    }
+    for (int64_t q = 0; q < nup; ++q)
       require(pos[q] == seg[q + 1], IK_EINVAL, "internal error: staging segments do not tile the slots");
+    h->bdest.upload(dest.data(), dest.size(), h->stream);
+  } else {
+    // plain element: tiles of kTileElems in Morton order; per tile, the entries j = 36 s + sb that fall on one
+    // upper block form a group (groups in order of their first entry, entries in increasing j)
+    // Tiles are consecutive runs of the Morton order (compact boxes), then sorted by their lowest node, so
+    // tiles running at the same time write neighbouring parts of the segments.
+    const std::vector<int32_t> order = mortonOrder(h);
+    const int64_t ntiles = (ne + kTileElems - 1) / kTileElems;
+    std::vector<std::pair<int32_t, int32_t>> tkey(ntiles);  // (lowest node, Morton tile)
+    for (int64_t m = 0; m < ntiles; ++m) {
+      int32_t lo = INT32_MAX;
+      for (int64_t i = m * kTileElems; i < std::min(ne, (m + 1) * kTileElems); ++i)
+        for (int a = 0; a < 8; ++a) lo = std::min(lo, conn[(int64_t)order[i] * 8 + a]);
+      tkey[m] = {lo, (int32_t)m};
+    }
+    std::sort(tkey.begin(), tkey.end());
+    std::vector<int32_t> tileElem((size_t)ntiles * kTileElems, -1);
+    for (int64_t t = 0; t < ntiles; ++t) {
+      const int64_t m = tkey[t].second;
+      for (int64_t i = m * kTileElems; i < std::min(ne, (m + 1) * kTileElems); ++i) {
+        tileElem[t * kTileElems + (i - m * kTileElems)] = order[i];
+        unitOf[order[i]] = t;
+      }
+    }
+    std::vector<uint16_t> gent((size_t)ntiles * kTileEntries, 0), epos((size_t)ntiles * kTileEntries), goff;
+    std::vector<int32_t> tgptr(ntiles + 1, 0), gq, groupOf(nup, -1), touched;
+    gq.reserve((size_t)ne * 24);
+    goff.reserve((size_t)ne * 24 + ntiles);
+    std::vector<int32_t> entQ(kTileEntries), cnt(kTileEntries + 1), rankOf, sizeOf(kTileEntries);
+    std::vector<uint16_t> entTr(kTileEntries);
+    for (int64_t t = 0; t < ntiles; ++t) {
+      touched.clear();
+      int nent = 0;
+      for (int s = 0; s < kTileElems; ++s) {
+        const int32_t e = tileElem[t * kTileElems + s];
+        if (e < 0) continue;
+        for (int a = 0; a < 8; ++a)
+          for (int c = a; c < 8; ++c) {
+            const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
+            const int j     = 36 * s + sb;
+            const int32_t q = qOf[slots[(int64_t)e * 36 + sb]];
+            entQ[j]         = q;
+            entTr[j]        = conn[(int64_t)e * 8 + a] > conn[(int64_t)e * 8 + c] ? 0x8000u : 0u;
+            if (groupOf[q] < 0) {
+              groupOf[q] = (int32_t)touched.size();
+              touched.push_back(q);
+            }
+            nent = j + 1;
+          }
+      }
+      // groups ordered by upper block, i.e. by staging slot, so a store pass writes ascending slots; counting
+      // sort of the entries by that order, stable in j
+      const int ng = (int)touched.size();
+      std::fill(cnt.begin(), cnt.begin() + ng + 1, 0);
+      for (int j = 0; j < nent; ++j)
+        if (tileElem[t * kTileElems + j / 36] >= 0) cnt[groupOf[entQ[j]] + 1]++;
+      rankOf.resize(ng);
+      for (int g = 0; g < ng; ++g) rankOf[g] = g;
+      std::sort(rankOf.begin(), rankOf.end(), [&](int x, int y) { return touched[x] < touched[y]; });
+      for (int g = 0; g < ng; ++g) sizeOf[g] = cnt[g + 1];
+      for (int r = 0; r < ng; ++r) {
+        const int32_t q = touched[rankOf[r]];
+        groupOf[q]      = r;
+        cnt[r + 1]      = sizeOf[rankOf[r]];
+      }
+      for (int g = 0; g < ng; ++g) cnt[g + 1] += cnt[g];
+      for (int g = 0; g <= ng; ++g) goff.push_back((uint16_t)cnt[g]);
+      uint16_t* ge = gent.data() + t * kTileEntries;
+      for (int j = 0; j < nent; ++j)
+        if (tileElem[t * kTileElems + j / 36] >= 0) ge[cnt[groupOf[entQ[j]]]++] = (uint16_t)(j | entTr[j]);
+      // inverse: the list position of every listed entry (padding entries keep their own index, which lies
+      // past the listed positions because padding slots follow the tile's elements)
+      uint16_t* ep = epos.data() + t * kTileEntries;
+      for (int j = 0; j < kTileEntries; ++j) ep[j] = (uint16_t)j;
+      const int nlisted = ng > 0 ? cnt[ng - 1] : 0;
+      for (int c = 0; c < nlisted; ++c) ep[ge[c] & 0x7FFFu] = (uint16_t)(c | (ge[c] & 0x8000u));
+      for (int r = 0; r < ng; ++r) gq.push_back(touched[rankOf[r]]);
+      for (const int32_t q : touched) groupOf[q] = -1;
+      require(gq.size() < (size_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
+      tgptr[t + 1] = (int32_t)gq.size();
+    }
+    // slots in gather order: segments in upper-block order, each ordered by tile
+    for (const int32_t q : gq) seg[q + 1]++;
+    for (int64_t q = 0; q < nup; ++q) seg[q + 1] += seg[q];
+    std::vector<int32_t> pos(seg.begin(), seg.end() - 1), gslot(gq.size());
+    for (size_t g = 0; g < gq.size(); ++g) gslot[g] = pos[gq[g]]++;
+    // the kernel's indexing relies on these: a tile's group offsets increase from 0 to 36 x its elements,
+    // its entry list holds each entry of its elements exactly once, the slots are a permutation of
+    // [0, nstage) with each group inside its own upper block's segment
+    {
+      std::vector<uint8_t> seen(kTileEntries), slotSeen(gq.size(), 0);
+      for (size_t g = 0; g < gq.size(); ++g) {
+        const int32_t sl = gslot[g];
+        require(sl >= seg[gq[g]] && sl < seg[gq[g] + 1] && !slotSeen[sl], IK_EINVAL,
+                "internal error: staging slots are not a permutation");
+        slotSeen[sl] = 1;
+      }
+      for (int64_t t = 0; t < ntiles; ++t) {
+        std::fill(seen.begin(), seen.end(), 0);
+        const uint16_t* go = goff.data() + tgptr[t] + t;
+        const uint16_t* ge = gent.data() + t * kTileEntries;
+        const int ng       = tgptr[t + 1] - tgptr[t];
+        int nvalid         = 0;
+        for (int s = 0; s < kTileElems; ++s) nvalid += tileElem[t * kTileElems + s] >= 0;
+        require(go[0] == 0 && go[ng] == 36 * nvalid && tileElem[t * kTileElems] >= 0, IK_EINVAL,
+                "internal error: bad tile group offsets");
+        for (int j = 36 * nvalid; j < kTileEntries; ++j)  // padding entries land past the listed positions
+          require(epos[t * kTileEntries + j] == (uint16_t)j, IK_EINVAL, "internal error: bad padding entry");
+        for (int g = 0; g < ng; ++g) {
+          require(go[g] < go[g + 1], IK_EINVAL, "internal error: bad tile group");
+          for (int c = go[g]; c < go[g + 1]; ++c) {
+            const unsigned j = ge[c] & 0x7FFFu;
+            require(j < (unsigned)kTileEntries && !seen[j] && tileElem[t * kTileElems + j / 36] >= 0 &&
+                        epos[t * kTileEntries + j] == (uint16_t)(c | (ge[c] & 0x8000u)),
+                    IK_EINVAL, "internal error: bad tile entry");
+            seen[j] = 1;
+          }
+        }
+      }
+    }
+    h->ntiles = ntiles;
+    nunits    = ntiles;
+    h->tileElem.upload(tileElem.data(), tileElem.size(), h->stream);
+    h->tgptr.upload(tgptr.data(), tgptr.size(), h->stream);
+    h->gslot.upload(gslot.data(), gslot.size(), h->stream);
+    h->epos.upload(epos.data(), epos.size(), h->stream);
+    h->goff.upload(goff.data(), goff.size(), h->stream);
   }
+  h->nstage = seg[nup];
   std::vector<int32_t> brow(h->nnzb);
   for (int64_t i = 0; i < nn; ++i)
     for (int32_t p = h->hrowptr[i]; p < h->hrowptr[i + 1]; ++p) brow[p] = (int32_t)i;
@@ -125,29 +317,28 @@ void buildPattern(ik_handle* h) {
     for (int64_t e = 0; e < ne; ++e)
       for (int a = 0; a < 8; ++a) nlist[pos[conn[e * 8 + a]]++] = (int32_t)(e * 8 + a);
   }
-  // Element chunks: a block row is complete once its last incident element is staged. With the
-  // prefix maximum of that index over rows (monotone by construction), rows [0, r) are complete
-  // after all elements below pm[r-1], so each chunk's gather covers a contiguous row range right
-  // behind the chunk while its staged blocks are still cached. Any numbering is correct; lexicographic
-  // numberings keep the lag at about one row of nodes.
+  // Staging chunks (ranges of staging units: tiles, or elements for EAS): a block row is complete once the
+  // last unit touching it is staged. With the prefix maximum of that unit index over rows (monotone by
+  // construction), rows [0, r) are complete after all units below pm[r-1], so each chunk's gather covers a
+  // contiguous row range behind the chunk. Any numbering is correct.
   std::vector<int64_t> last(nn, -1);
   for (int64_t i = 0; i < nn; ++i) {
-    for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) last[i] = std::max(last[i], inc[k]);
+    for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) last[i] = std::max(last[i], unitOf[inc[k]]);
     if (i > 0) last[i] = std::max(last[i], last[i - 1]);
   }
-  const int64_t chunk = chunkElements();
+  const int64_t chunk = h->tiled ? (chunkElements() + kTileElems - 1) / kTileElems : chunkElements();
   h->chunkElem.clear();
   h->chunkRow.clear();
   h->chunkElem.push_back(0);
   h->chunkRow.push_back(0);
   int64_t r = 0;
-  for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
-    const int64_t e1 = std::min(ne, e0 + chunk);
-    if (e1 == ne)
+  for (int64_t u0 = 0; u0 < nunits; u0 += chunk) {
+    const int64_t u1 = std::min(nunits, u0 + chunk);
+    if (u1 == nunits)
       r = nn;
     else
-      while (r < nn && last[r] < e1) ++r;
-    h->chunkElem.push_back(e1);
+      while (r < nn && last[r] < u1) ++r;
+    h->chunkElem.push_back(u1);
     h->chunkRow.push_back(r);
   }
   h->rowptr.upload(h->hrowptr.data(), h->hrowptr.size(), h->stream);
@@ -170,7 +361,6 @@ void buildPattern(ik_handle* h) {
   h->upMask.upload(mask.data(), mask.size(), h->stream);
   h->upBlk.upload(up.data(), up.size(), h->stream);
   h->upT.upload(upT.data(), upT.size(), h->stream);
-  h->bdest.upload(dest.data(), dest.size(), h->stream);
   h->segptr.upload(seg.data(), seg.size(), h->stream);
   // residual staging slot of every element corner: its position in the node's incidence list
   std::vector<int32_t> rdest((size_t)ne * 8);
