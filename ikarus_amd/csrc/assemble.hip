@@ -16,18 +16,15 @@
 // and run on the FP64 matrix cores (v_mfma_f64_4x4x4: the four 4x4 blocks of one instruction are four
 // elements; 21 upper tiles x 2 k-steps per matrix).
 //
-// Element kernel (one 64-lane wavefront = 4 elements):
-//   phase 0  stage X, u of the 4 x 8 nodes in LDS;
-//   phase 1  lane pair (el, gp) evaluates one Gauss point, each lane half the nodes: J, g_a, F (partial
-//            sums combined across the pair with DPP), the material, p_a -> LDS; R_e is reduced over the
-//            Gauss points with DPP row rotations;
-//   phase 2  W1, W2 on MFMA, written into the LDS element blocks (W2 transposed within each 3x3 block);
-//   phase 3  lane (el, a, half) adds the Phi and g^T S g terms of its node pairs;
-//   phase 4  the 4 x 36 upper 3x3 blocks go to global memory with 16-byte coalesced stores.
-// Gather kernels then sum, for every 3x3 block of the CSR pattern, the element blocks listed for it
-// (a fixed order, so the result is bitwise reproducible), apply the FULL Dirichlet treatment on the fly
-// and write each block once. Elements are processed in chunks; each chunk's gather runs right behind
-// it on the rows the chunk completes, so the staged blocks are read back while still cached.
+// Tangent kernel (one workgroup = a tile of 16 elements, 4 per 64-lane wavefront):
+//   each lane evaluates two Gauss points of one element for two of its nodes (J and F partial sums
+//   combined across the lane quad with DPP), then the per-point material; the tangent's terms are
+//   accumulated on the FP64 matrix cores (operands as below); the element's upper 3x3 blocks go to LDS at
+//   the position of the K block they fall on, and each K block's sum over the tile's elements is staged.
+// Gather kernels then sum, for every 3x3 block of the CSR pattern, the staged blocks listed for it (a
+// fixed order, so the result is bitwise reproducible), apply the FULL Dirichlet treatment on the fly and
+// write each block once. Large meshes are processed in chunks whose gathers run behind them on the rows
+// each chunk completes.
 #include "ik_internal.hh"
 
 namespace ik {
