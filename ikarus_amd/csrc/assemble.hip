@@ -163,7 +163,8 @@ constexpr int NSYM = 6;
 // Workgroup = 4 wavefronts = one tile of kTileElems elements, EPW per wavefront.
 //  TILED: the workgroup takes tile t = e0 + blockIdx.x, element s = tileElem[16 t + s] (-1: padding). Its
 //   576 upper element blocks go to LDS grouped by the K block they fall on (position epos, in the K block's
-//   orientation); each group is summed in list order and the sum goes to its staging slot gslot[g].
+//   orientation); each group is summed in list order and the sum goes to its staging slot gdst[g] or, for
+//   a K block no other tile touches, straight into K (FULL-treated when maskDbc, and to its mirror block).
 //  otherwise: elements e0 + 16 blockIdx.x + s below e0 + count, stored in element order,
 //   Kebe[(e*36 + sb)*9 + 3k + l] = K_e[3a+k][3b+l] for the upper block sb = upperBlock(a, b).
 // WANT_R: Rebe[rdest ? rdest[e*8+a]*3 + k : e*24 + 3a + k] = R_e[3a+k] (element order only).
@@ -184,9 +185,12 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
                                                       const int32_t* __restrict__ tileElem, int64_t e0,
                                                       int64_t count, double lam, double mu,
                                                       const int32_t* __restrict__ tgptr,
-                                                      const int32_t* __restrict__ gslot,
+                                                      const int32_t* __restrict__ gdst,
+                                                      const int32_t* __restrict__ gpt,
+                                                      const uint16_t* __restrict__ gmsk,
                                                       const uint16_t* __restrict__ epos,
-                                                      const uint16_t* __restrict__ goff,
+                                                      const uint16_t* __restrict__ goff, TK* __restrict__ K,
+                                                      int maskDbc,
                                                       const int32_t* __restrict__ rdest, TK* __restrict__ Kebe,
                                                       double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
                                                       double3 trac, unsigned long long* __restrict__ err) {
@@ -210,16 +214,20 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
     e                  = e0 + (idx < count ? idx : count - 1);
   }
 
-  // the tile's entry positions, group offsets and slots, fetched early so the stores do not wait on them
-  __shared__ uint16_t ep[kTileEntries], go[kTileEntries + 1];
-  __shared__ int32_t gs[kTileEntries];
+  // the tile's entry positions, group offsets and targets, fetched early so the stores do not wait on them
+  __shared__ uint16_t ep[kTileEntries], go[kTileEntries + 1], gm[kTileEntries];
+  __shared__ int32_t gs[kTileEntries], gt[kTileEntries];
   int ng = 0;
   if constexpr (WANT_K && TILED) {
     const int32_t g0 = tgptr[t];
     ng               = tgptr[t + 1] - g0;
     for (int j = threadIdx.x; j < kTileEntries; j += kTileThreads) ep[j] = epos[t * kTileEntries + j];
     for (int j = threadIdx.x; j <= ng; j += kTileThreads) go[j] = goff[g0 + t + j];
-    for (int j = threadIdx.x; j < ng; j += kTileThreads) gs[j] = gslot[g0 + j];
+    for (int j = threadIdx.x; j < ng; j += kTileThreads) {
+      gs[j] = gdst[g0 + j];
+      gt[j] = gpt[g0 + j];
+      gm[j] = gmsk[g0 + j];
+    }
   }
 
   double x[2][3], u[2][3];
@@ -433,12 +441,24 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
       // blocks are contiguous in LDS and added in list order; groups come ordered by slot, so a pass writes
       // ascending slots
       const double* lds = &ebe[0][0];
-      const int tt      = threadIdx.x % 9;
+      const int tt = threadIdx.x % 9, k = tt / 3, l = tt - 3 * (tt / 3);
       for (int g = threadIdx.x / 9; g < ng; g += kTileThreads / 9) {
         const int c1 = go[g + 1];
         double v     = 0.0;
         for (int c = go[g]; c < c1; ++c) v += lds[c * 9 + tt];
-        Kebe[(int64_t)gs[g] * 9 + tt] = (TK)v;
+        const int32_t d = gs[g];
+        if (d >= 0) {
+          Kebe[(int64_t)d * 9 + tt] = (TK)v;
+        } else {
+          // complete K block: the FULL treatment as in the gather, then the block and its mirror
+          const int32_t p = -d - 1, pt = gt[g];
+          if (maskDbc) {
+            const unsigned m = gm[g];
+            if ((m >> tt) & 1u) v = (k == l && ((m >> (9 + k)) & 1u)) ? 1.0 : 0.0;
+          }
+          K[(int64_t)p * 9 + tt] = (TK)v;
+          if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = (TK)v;
+        }
       }
     }
   }
@@ -648,7 +668,7 @@ inline unsigned gridFor(int64_t n, int bs) {
 // tiled: tangent staging over tiles [e0, e0 + count); otherwise elements [e0, e0 + count) in element order
 template <typename T, int KIND, bool WK, bool WR>
 void launchElem(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32_t* rdest, T* Kebe, double* Rebe,
-                hipStream_t s) {
+                hipStream_t s, T* K, int mask) {
   if constexpr (!WK) {
     const unsigned grid = (unsigned)((count + 7) / 8);
     hipLaunchKernelGGL((hex8Residual<KIND>), dim3(grid), dim3(64), 0, s, h->X.p, h->u.p, h->conn.p, e0, count,
@@ -657,24 +677,25 @@ void launchElem(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32
     // tangent staging over tiles (the residual instance below serves single elements in element order)
     (void)tiled;
     hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR, true>), dim3((unsigned)count), dim3(kTileThreads), 0, s, h->X.p,
-                       h->u.p, h->conn.p, h->tileElem.p, e0, count, h->lam, h->mu, h->tgptr.p, h->gslot.p,
-                       h->epos.p, h->goff.p, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
+                       h->u.p, h->conn.p, h->tileElem.p, e0, count, h->lam, h->mu, h->tgptr.p, h->gdst.p, h->gpt.p,
+                       h->gmsk.p, h->epos.p, h->goff.p, K, mask, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h),
+                       h->errFlag.p);
   } else {
     const unsigned grid = (unsigned)((count + kTileElems - 1) / kTileElems);
     hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR, false>), dim3(grid), dim3(kTileThreads), 0, s, h->X.p, h->u.p,
-                       h->conn.p, nullptr, e0, count, h->lam, h->mu, nullptr, nullptr, nullptr, nullptr, rdest,
-                       Kebe, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
+                       h->conn.p, nullptr, e0, count, h->lam, h->mu, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, (T*)nullptr, 0, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h), h->errFlag.p);
   }
 }
 
 template <typename T, bool WK, bool WR>
 void launchElemMat(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32_t* rdest, T* Kebe,
-                   double* Rebe, hipStream_t s = nullptr) {
+                   double* Rebe, hipStream_t s = nullptr, T* K = nullptr, int mask = 0) {
   if (!s) s = h->stream;
   if (h->material == IK_SVK)
-    launchElem<T, IK_SVK, WK, WR>(h, e0, count, tiled, rdest, Kebe, Rebe, s);
+    launchElem<T, IK_SVK, WK, WR>(h, e0, count, tiled, rdest, Kebe, Rebe, s, K, mask);
   else
-    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, tiled, rdest, Kebe, Rebe, s);
+    launchElem<T, IK_NEOHOOKE, WK, WR>(h, e0, count, tiled, rdest, Kebe, Rebe, s, K, mask);
 }
 
 }  // namespace
@@ -700,19 +721,23 @@ static void checkErrFlag(ik_handle* h) {
   }
 }
 
-// staging units [u0, u1): elements (EAS, K_e and R_e) or tiles (plain element, K_e)
-static void launchChunk(ik_handle* h, int64_t u0, int64_t u1, void* Kebe, bool fp32) {
+// staging units [u0, u1): elements (EAS, K_e and R_e) or tiles (plain element, K_e; blocks complete in a
+// tile go straight into K)
+static void launchChunk(ik_handle* h, int64_t u0, int64_t u1, void* Kebe, void* K, bool fp32, int mask) {
   if (h->neas)
     easStage(h, u0, u1 - u0, true, h->bdest.p, h->rdest.p);
   else if (fp32)
-    launchElemMat<float, true, false>(h, u0, u1 - u0, true, nullptr, (float*)Kebe, nullptr);
+    launchElemMat<float, true, false>(h, u0, u1 - u0, true, nullptr, (float*)Kebe, nullptr, h->stream, (float*)K,
+                                      mask);
   else
-    launchElemMat<double, true, false>(h, u0, u1 - u0, true, nullptr, (double*)Kebe, nullptr);
+    launchElemMat<double, true, false>(h, u0, u1 - u0, true, nullptr, (double*)Kebe, nullptr, h->stream,
+                                       (double*)K, mask);
 }
 
 template <typename T>
 static void launchGatherK(ik_handle* h, int64_t r0, int64_t r1, int mask, const T* Kebe, T* K, hipStream_t s) {
   const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
+  if (q1 <= q0) return;  // every block of these rows was written by the element kernel
   hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)(((q1 - q0) * 9 + 255) / 256)), dim3(256), 0, s, h->segptr.p,
                      h->upBlk.p, h->upT.p, h->upMask.p, mask, Kebe, q0, q1, K);
 }
@@ -749,7 +774,7 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T*
   if (h->timing) {
     {
       ScopedKernelTimer t(h, KASM_KR, h->stream, (int64_t)nc);
-      for (size_t c = 0; c < nc; ++c) launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], Kebe, fp32);
+      for (size_t c = 0; c < nc; ++c) launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], Kebe, K, fp32, mask);
     }
     {
       ScopedKernelTimer t(h, KGATHER, h->stream, (int64_t)nc);
@@ -764,14 +789,14 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T*
   }
   if (nc == 1) {
     // one chunk: the K gather follows on the main stream
-    launchChunk(h, h->chunkElem[0], h->chunkElem[1], Kebe, fp32);
+    launchChunk(h, h->chunkElem[0], h->chunkElem[1], Kebe, K, fp32, mask);
     launchGatherK<T>(h, h->chunkRow[0], h->chunkRow[1], mask, Kebe, K, h->stream);
     if (sideR) launchResidual(h, h->stream);
     if (wantR || sideR) launchGatherR(h, mask, 0, h->nn, h->stream);
     return;
   }
   for (size_t c = 0; c < nc; ++c) {
-    launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], Kebe, fp32);
+    launchChunk(h, h->chunkElem[c], h->chunkElem[c + 1], Kebe, K, fp32, mask);
     const int64_t r0 = h->chunkRow[c], r1 = h->chunkRow[c + 1];
     if (r1 <= r0) continue;
     IK_HIP(hipEventRecord(h->chunkEvents[c], h->stream));

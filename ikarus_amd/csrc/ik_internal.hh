@@ -119,8 +119,8 @@ struct ik_handle {
   std::vector<int32_t> hrowptr, hcolidx, hdiag;
   int64_t nnzb = 0;
   ik::DevBuf<int32_t> rowptr, colidx, diagBlk, blockRow;
-  // upper blocks (column >= row) in row order: their block index and the index of their transpose;
-  // huptr[r] = first upper block of row r
+  // the upper blocks (column >= row) the gather assembles, in row order: their block index and the index
+  // of their transpose; huptr[r] = first of them in row r
   std::vector<int64_t> huptr;
   ik::DevBuf<int32_t> upBlk, upT;
   // FULL Dirichlet treatment per upper block: bit t (t = 3k+l) zeroes entry t (constrained row or column),
@@ -131,10 +131,12 @@ struct ik_handle {
   //  EAS element: one slot per element block; bdest[e*36+sb] = (slot << 1) | transposed, segments ordered
   //  by element.
   //  Plain element (tiled): elements grouped in tiles of kTileElems (Morton order of the element centroids);
-  //  the blocks of a tile that fall on the same upper block form a group, summed in the tile before staging
-  //  (one slot per group, segments ordered by tile). tileElem[t*16+s] = element (-1: padding of the last
-  //  tile); tgptr[t] = first group of tile t (groups ordered by upper block), gslot[g] = the slot of group
-  //  g. The tile's entries j = 36 s + sb are listed group by group (each group's in increasing j);
+  //  the blocks of a tile that fall on the same upper block form a group, summed in the tile. An upper
+  //  block touched by one tile only is then complete and written straight into K; the others get one
+  //  staging slot per group (segments ordered by tile). tileElem[t*16+s] = element (-1: padding of the
+  //  last tile); tgptr[t] = first group of tile t (groups ordered by upper block); gdst[g] = the slot of
+  //  group g, or -(K block + 1) for a direct group, whose mirror block is gpt[g] and FULL mask gmsk[g].
+  //  The tile's entries j = 36 s + sb are listed group by group (each group's in increasing j);
   //  epos[t*576 + j] = the position of entry j in that list | transposed << 15 (padding entries keep
   //  position j, past the listed ones), and the offsets of tile t's groups in the list are
   //  goff[tgptr[t] + t + i], i = 0..ng (the last one ends it).
@@ -143,8 +145,8 @@ struct ik_handle {
   ik::DevBuf<int32_t> bdest, segptr, ncptr, rdest;
   bool tiled = false;
   int64_t ntiles = 0, nstage = 0;  // tiles; staging slots of the tangent
-  ik::DevBuf<int32_t> tileElem, tgptr, gslot;
-  ik::DevBuf<uint16_t> epos, goff;
+  ik::DevBuf<int32_t> tileElem, tgptr, gdst, gpt;
+  ik::DevBuf<uint16_t> epos, goff, gmsk;
   // staging chunks (tiles when tiled, elements otherwise) and the block rows complete after each chunk
   std::vector<int64_t> chunkElem, chunkRow;
   // element-by-element staging of K_e (36 upper 3x3 blocks) and R_e (24)

@@ -156,7 +156,22 @@ void buildPattern(ik_handle* h) {
   const int64_t nup = h->huptr[nn];
   std::vector<int32_t> qOf(h->nnzb, -1);
   for (int64_t q = 0; q < nup; ++q) qOf[up[q]] = (int32_t)q;
+  // FULL Dirichlet treatment per upper block: bit 3k+l zeroes entry (k,l), bit 9+k sets a unit diagonal
+  std::vector<uint16_t> mask(nup, 0);
+  for (int64_t i = 0; i < nn; ++i)
+    for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
+      const int64_t c = h->hcolidx[p];
+      uint16_t m      = 0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l)
+          if (h->hfixed[3 * i + k] | h->hfixed[3 * c + l]) {
+            m |= (uint16_t)(1u << (3 * k + l));
+            if (3 * i + k == 3 * c + l) m |= (uint16_t)(1u << (9 + k));
+          }
+      mask[h->huptr[i] + (p - h->hdiag[i])] = m;
+    }
   std::vector<int32_t> seg(nup + 1, 0);
+  std::vector<uint8_t> gathered(nup, 1);  // upper blocks the gather assembles (the rest are written directly)
   std::vector<int64_t> unitOf(ne);  // staging unit of each element: its tile (tiled) or itself
   int64_t nunits = ne;
   h->tiled = h->neas == 0;
@@ -258,18 +273,43 @@ void buildPattern(ik_handle* h) {
       require(gq.size() < (size_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
       tgptr[t + 1] = (int32_t)gq.size();
     }
-    // slots in gather order: segments in upper-block order, each ordered by tile
-    for (const int32_t q : gq) seg[q + 1]++;
+    // An upper block that only one tile touches is complete after that tile's group sum, which the tile
+    // writes straight into K (with the FULL mask and the mirror block). The others are staged, in gather
+    // order: segments in upper-block order, each ordered by tile. gdst[g] = slot, or -(K block + 1) with
+    // gpt[g] = the mirror block and gmsk[g] = the mask of a direct group.
+    std::vector<int32_t> ntile(nup, 0);
+    for (const int32_t q : gq) ntile[q]++;
+    for (int64_t q = 0; q < nup; ++q) gathered[q] = ntile[q] > 1;
+    for (const int32_t q : gq)
+      if (gathered[q]) seg[q + 1]++;
     for (int64_t q = 0; q < nup; ++q) seg[q + 1] += seg[q];
-    std::vector<int32_t> pos(seg.begin(), seg.end() - 1), gslot(gq.size());
-    for (size_t g = 0; g < gq.size(); ++g) gslot[g] = pos[gq[g]]++;
+    std::vector<int32_t> pos(seg.begin(), seg.end() - 1), gdst(gq.size()), gpt(gq.size(), 0);
+    std::vector<uint16_t> gmsk(gq.size(), 0);
+    for (size_t g = 0; g < gq.size(); ++g) {
+      const int32_t q = gq[g];
+      if (gathered[q]) {
+        gdst[g] = pos[q]++;
+      } else {
+        gdst[g] = -(up[q] + 1);
+        gpt[g]  = upT[q];
+        gmsk[g] = mask[q];
+      }
+    }
     // the kernel's indexing relies on these: a tile's group offsets increase from 0 to 36 x its elements,
     // its entry list holds each entry of its elements exactly once, the slots are a permutation of
-    // [0, nstage) with each group inside its own upper block's segment
+    // [0, nstage) with each group inside its own upper block's segment, and every upper block is either
+    // staged by two or more tiles or written by exactly one
     {
-      std::vector<uint8_t> seen(kTileEntries), slotSeen(gq.size(), 0);
+      std::vector<uint8_t> seen(kTileEntries), slotSeen(seg[nup], 0);
+      for (int64_t q = 0; q < nup; ++q)
+        require(ntile[q] >= 1 && (gathered[q] ? seg[q + 1] - seg[q] == ntile[q] : seg[q + 1] == seg[q]), IK_EINVAL,
+                "internal error: upper block without a tile");
       for (size_t g = 0; g < gq.size(); ++g) {
-        const int32_t sl = gslot[g];
+        const int32_t sl = gdst[g];
+        if (sl < 0) {
+          require(!gathered[gq[g]] && -sl - 1 == up[gq[g]], IK_EINVAL, "internal error: bad direct group");
+          continue;
+        }
         require(sl >= seg[gq[g]] && sl < seg[gq[g] + 1] && !slotSeen[sl], IK_EINVAL,
                 "internal error: staging slots are not a permutation");
         slotSeen[sl] = 1;
@@ -301,11 +341,35 @@ void buildPattern(ik_handle* h) {
     nunits    = ntiles;
     h->tileElem.upload(tileElem.data(), tileElem.size(), h->stream);
     h->tgptr.upload(tgptr.data(), tgptr.size(), h->stream);
-    h->gslot.upload(gslot.data(), gslot.size(), h->stream);
+    h->gdst.upload(gdst.data(), gdst.size(), h->stream);
+    h->gpt.upload(gpt.data(), gpt.size(), h->stream);
+    h->gmsk.upload(gmsk.data(), gmsk.size(), h->stream);
     h->epos.upload(epos.data(), epos.size(), h->stream);
     h->goff.upload(goff.data(), goff.size(), h->stream);
   }
   h->nstage = seg[nup];
+  // the gather's upper blocks: the gathered ones in row order (huptr: first one of each row)
+  {
+    std::vector<int32_t> cup, cupT, cseg(1, 0);
+    std::vector<uint16_t> cmask;
+    std::vector<int64_t> chuptr(nn + 1, 0);
+    for (int64_t i = 0; i < nn; ++i) {
+      for (int64_t q = h->huptr[i]; q < h->huptr[i + 1]; ++q)
+        if (gathered[q]) {
+          cup.push_back(up[q]);
+          cupT.push_back(upT[q]);
+          cmask.push_back(mask[q]);
+          cseg.push_back(seg[q + 1]);
+          require(seg[q] == cseg[cseg.size() - 2], IK_EINVAL, "internal error: gather segments not contiguous");
+        }
+      chuptr[i + 1] = (int64_t)cup.size();
+    }
+    h->huptr.swap(chuptr);
+    h->upMask.upload(cmask.data(), cmask.size(), h->stream);
+    h->upBlk.upload(cup.data(), cup.size(), h->stream);
+    h->upT.upload(cupT.data(), cupT.size(), h->stream);
+    h->segptr.upload(cseg.data(), cseg.size(), h->stream);
+  }
   std::vector<int32_t> brow(h->nnzb);
   for (int64_t i = 0; i < nn; ++i)
     for (int32_t p = h->hrowptr[i]; p < h->hrowptr[i + 1]; ++p) brow[p] = (int32_t)i;
@@ -345,23 +409,6 @@ void buildPattern(ik_handle* h) {
   h->colidx.upload(h->hcolidx.data(), h->hcolidx.size(), h->stream);
   h->diagBlk.upload(h->hdiag.data(), h->hdiag.size(), h->stream);
   h->blockRow.upload(brow.data(), brow.size(), h->stream);
-  std::vector<uint16_t> mask(up.size(), 0);
-  for (int64_t i = 0; i < nn; ++i)
-    for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
-      const int64_t c = h->hcolidx[p];
-      uint16_t m      = 0;
-      for (int k = 0; k < 3; ++k)
-        for (int l = 0; l < 3; ++l)
-          if (h->hfixed[3 * i + k] | h->hfixed[3 * c + l]) {
-            m |= (uint16_t)(1u << (3 * k + l));
-            if (3 * i + k == 3 * c + l) m |= (uint16_t)(1u << (9 + k));
-          }
-      mask[h->huptr[i] + (p - h->hdiag[i])] = m;
-    }
-  h->upMask.upload(mask.data(), mask.size(), h->stream);
-  h->upBlk.upload(up.data(), up.size(), h->stream);
-  h->upT.upload(upT.data(), upT.size(), h->stream);
-  h->segptr.upload(seg.data(), seg.size(), h->stream);
   // residual staging slot of every element corner: its position in the node's incidence list
   std::vector<int32_t> rdest((size_t)ne * 8);
   for (size_t j = 0; j < nlist.size(); ++j) rdest[nlist[j]] = (int32_t)j;
