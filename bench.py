@@ -262,9 +262,10 @@ def main():
     g_ms = g_ms_tot / max(ng, 1)
     r_ms = kp["residual"][1] / max(kp["residual"][0], 1)
     nnzb = nnzb or 0
-    nup = (nnzb + p.X.shape[0]) // 2
-    # gather (scatter into the CSR): staged element blocks read, K written, per-upper-block indices
-    gather_bytes = (2592 * p.ne + 72 * nnzb + 20 * nup) / max(ng / args.steps, 1)
+    staged, gathered, direct = dev.assembly_layout()
+    # gather: staged blocks read, its upper blocks and their mirrors written, per-upper-block indices and
+    # segment pointers (diagonal blocks have no mirror; counted as if they had, a bound within 7 %)
+    gather_bytes = (72 * staged + 2 * 72 * gathered + 18 * gathered) / max(ng / args.steps, 1)
     gather_gbs = gather_bytes / (g_ms * 1e-3) / 1e9
     s64n, s64ms = spmv_pass(dev, p, capi.FP64)
     s32n, s32ms = spmv_pass(dev, p, capi.FP32)
@@ -305,7 +306,8 @@ def main():
         "separate_calls": {"ms_per_step": round(sep_ms, 4), "Melem_per_s": round(p.ne / sep_ms / 1e3, 3),
                            "residual_kernel_ms": round(r_ms, 4)},
         "gather": {"ms_per_launch": round(g_ms, 4), "GBs": round(gather_gbs, 1),
-                   "frac_hbm": round(gather_gbs / PEAK_HBM_GBS, 4), "bytes_per_launch": int(gather_bytes)},
+                   "frac_hbm": round(gather_gbs / PEAK_HBM_GBS, 4), "bytes_per_launch": int(gather_bytes),
+                   "staged_slots": staged, "gathered_upper_blocks": gathered, "direct_upper_blocks": direct},
         "spmv": {"fp64": {"ms": round(spmv64_ms, 4), "GBs": round(spmv64_gbs, 1),
                           "frac_hbm": round(spmv64_gbs / PEAK_HBM_GBS, 4), "bytes": spmv64_bytes},
                  "fp32": {"ms": round(spmv32_ms, 4), "GBs": round(spmv32_gbs, 1),

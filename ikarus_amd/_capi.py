@@ -24,7 +24,7 @@ EXPORTED = [
     "ik_set_requirement", "ik_get_requirement", "ik_vector", "ik_matrix", "ik_assemble", "ik_matrix_csc", "ik_matrix_bsr",
     "ik_element", "ik_update_state", "ik_get_alpha", "ik_set_alpha", "ik_solve", "ik_newton_solve",
     "ik_load_control", "ik_mlp_num_params", "ik_mlp_set_params", "ik_mlp_get_params", "ik_mlp_set_load",
-    "ik_timing_enable", "ik_timing_read", "ik_timing_reset", "ik_synchronize",
+    "ik_timing_enable", "ik_timing_read", "ik_timing_reset", "ik_synchronize", "ik_assembly_layout",
     "ik_mlp_create", "ik_mlp_destroy", "ik_mlp_last_error", "ik_mlp_params", "ik_mlp_load", "ik_mlp_forward",
     "ik_mlp_loss_grad", "ik_mlp_train_step",
 ]
@@ -114,6 +114,7 @@ def lib():
         L.ik_timing_read.argtypes = [P, I, P, P]
         L.ik_timing_reset.argtypes = [P]
         L.ik_synchronize.argtypes = [P]
+        L.ik_assembly_layout.argtypes = [P, P, P, P]
         L.ik_mlp_create.argtypes = [C.POINTER(C.c_void_p), C.c_uint64]
         L.ik_mlp_destroy.argtypes = [P]
         L.ik_mlp_destroy.restype = None

@@ -194,6 +194,12 @@ class Device:
     def synchronize(self):
         self.check(capi.lib().ik_synchronize(self.h))
 
+    def assembly_layout(self):
+        """(staged slots, gathered upper blocks, directly written upper blocks) of the tangent assembly."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.check(capi.lib().ik_assembly_layout(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
 
 def solver_opts(kind=capi.PCG_JACOBI_FP64, rtol=1e-10, max_iter=-1, inner_rtol=1e-4, max_refine=20):
     o = capi.SolverOpts()

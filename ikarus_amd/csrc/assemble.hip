@@ -162,9 +162,11 @@ constexpr int NSYM = 6;
 
 // Workgroup = 4 wavefronts = one tile of kTileElems elements, EPW per wavefront.
 //  TILED: the workgroup takes tile t = e0 + blockIdx.x, element s = tileElem[16 t + s] (-1: padding). Its
-//   576 upper element blocks go to LDS grouped by the K block they fall on (position epos, in the K block's
-//   orientation); each group is summed in list order and the sum goes to its staging slot gdst[g] or, for
-//   a K block no other tile touches, straight into K (FULL-treated when maskDbc, and to its mirror block).
+//   576 upper element blocks are summed in LDS per group (the blocks falling on one K block, in the K
+//   block's orientation): rank 0 of every group stores, ranks 1, 2, ... add in turn, one rank per barrier
+//   step, so each sum has a fixed order and no two lanes touch one value in a step. Each group sum then
+//   goes to its staging slot gdst[g] or, for a K block no other tile touches, straight into K
+//   (FULL-treated when maskDbc, and to its mirror block).
 //  otherwise: elements e0 + 16 blockIdx.x + s below e0 + count, stored in element order,
 //   Kebe[(e*36 + sb)*9 + 3k + l] = K_e[3a+k][3b+l] for the upper block sb = upperBlock(a, b).
 // WANT_R: Rebe[rdest ? rdest[e*8+a]*3 + k : e*24 + 3a + k] = R_e[3a+k] (element order only).
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
                                                       const int32_t* __restrict__ gpt,
                                                       const uint16_t* __restrict__ gmsk,
                                                       const uint16_t* __restrict__ epos,
-                                                      const uint16_t* __restrict__ goff, TK* __restrict__ K,
+                                                      const uint8_t* __restrict__ tmaxr, TK* __restrict__ K,
                                                       int maskDbc,
                                                       const int32_t* __restrict__ rdest, TK* __restrict__ Kebe,
                                                       double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
@@ -214,15 +216,15 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
     e                  = e0 + (idx < count ? idx : count - 1);
   }
 
-  // the tile's entry positions, group offsets and targets, fetched early so the stores do not wait on them
-  __shared__ uint16_t ep[kTileEntries], go[kTileEntries + 1], gm[kTileEntries];
+  // the tile's entry groups and ranks and its group targets, fetched early so the stores do not wait on them
+  __shared__ uint16_t ep[kTileEntries], gm[kTileEntries];
   __shared__ int32_t gs[kTileEntries], gt[kTileEntries];
-  int ng = 0;
+  int ng = 0, nrank = 1;
   if constexpr (WANT_K && TILED) {
     const int32_t g0 = tgptr[t];
     ng               = tgptr[t + 1] - g0;
+    nrank            = tmaxr[t];
     for (int j = threadIdx.x; j < kTileEntries; j += kTileThreads) ep[j] = epos[t * kTileEntries + j];
-    for (int j = threadIdx.x; j <= ng; j += kTileThreads) go[j] = goff[g0 + t + j];
     for (int j = threadIdx.x; j < ng; j += kTileThreads) {
       gs[j] = gdst[g0 + j];
       gt[j] = gpt[g0 + j];
@@ -388,64 +390,72 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
       K01[3 * k + l] += G01[v - 3];
       K01[3 * l + k] += G01[v - 3];
     }
-    // result lane = 16 r + 4 bk + c holds rows a = r (+4), columns b = c (+4); every upper-block entry is
-    // written once into LDS: TILED at its entry's list position in the K block's orientation, otherwise
-    // entry j = 36 s + sb at ebe[j*9 ..]
+    // result lane = 16 r + 4 bk + c holds rows a = r (+4), columns b = c (+4): block (r, c+4) whole and, of
+    // the blocks (a, b) = (r, c) and (r+4, c+4), the entries no other lane holds. TILED: entry values go to
+    // the accumulator of their group (ebe viewed as [576][9]) in the K block's orientation, rank by rank;
+    // otherwise element block j = 36 s + sb is accumulator j (rank 0, as computed).
     __shared__ __align__(16) double ebe[kTileElems][324];
     {
       const int r = kk, c = i;
-      double* lds = &ebe[0][0];
-      // LDS base of element block sb and whether its entries are stored transposed
-      auto place = [&](int sb, int& base, bool& tr) {
-        if constexpr (TILED) {
-          const unsigned P = ep[36 * s + sb];
-          base             = (int)(P & 0x7FFFu) * 9;
-          tr               = (P >> 15) != 0;
-        } else {
-          base = (36 * s + sb) * 9;
-          tr   = false;
-        }
-      };
-      {
-        int base;
-        bool tr;
-        place(upperBlock(r, c + 4), base, tr);
-#pragma unroll
-        for (int t = 0; t < 9; ++t) lds[base + (tr ? 3 * (t % 3) + t / 3 : t)] = K01[t];
+      double* acc = &ebe[0][0];
+      const int sbW   = upperBlock(r, c + 4);
+      const int sbA[2] = {r < c ? upperBlock(r, c) : (r == c ? upperBlock(r, r) : upperBlock(c, r)),
+                          r < c ? upperBlock(r + 4, c + 4)
+                                : (r == c ? upperBlock(r + 4, r + 4) : upperBlock(c + 4, r + 4))};
+      // group | rank << 10 | transposed << 15 of the three blocks
+      unsigned PW, PA[2];
+      if constexpr (TILED) {
+        PW    = ep[36 * s + sbW];
+        PA[0] = ep[36 * s + sbA[0]];
+        PA[1] = ep[36 * s + sbA[1]];
+      } else {
+        PW    = 36 * s + sbW;
+        PA[0] = 36 * s + sbA[0];
+        PA[1] = 36 * s + sbA[1];
       }
+      for (int rk = 0; rk < nrank; ++rk) {
+        auto put = [&](int idx, double val) {
+          if (rk == 0)
+            acc[idx] = val;
+          else
+            atomicAdd(&acc[idx], val);  // the only add to this value in this step
+        };
+        if (((PW >> 10) & 0xFu) == (unsigned)rk) {
+          const int base = (int)(PW & 0x3FFu) * 9;
+          const bool tr  = (PW >> 15) != 0;
 #pragma unroll
-      for (int al = 0; al < 2; ++al) {
-        const int a = r + 4 * al, b = c + 4 * al;
-        int base;
-        bool tr;
-        place(r < c ? upperBlock(a, b) : (r == c ? upperBlock(a, a) : upperBlock(b, a)), base, tr);
-#pragma unroll
-        for (int v = 0; v < 6; ++v) {
-          const int k = vI(v), l = vJ(v);
-          const double val = al ? K11[v] : K00[v];
-          // entry (k, l) of block (a, b) for r < c and of the symmetric diagonal block for r == c (with its
-          // mirror), entry (l, k) of block (b, a) for r > c
-          const int kl = r > c ? 3 * l + k : 3 * k + l, lk = r > c ? 3 * k + l : 3 * l + k;
-          if (r <= c || k != l) lds[base + (tr ? lk : kl)] = val;
-          if (r == c && k != l) lds[base + lk] = val;
+          for (int e9 = 0; e9 < 9; ++e9) put(base + (tr ? 3 * (e9 % 3) + e9 / 3 : e9), K01[e9]);
         }
+#pragma unroll
+        for (int al = 0; al < 2; ++al) {
+          if (((PA[al] >> 10) & 0xFu) != (unsigned)rk) continue;
+          const int base = (int)(PA[al] & 0x3FFu) * 9;
+          const bool tr  = (PA[al] >> 15) != 0;
+#pragma unroll
+          for (int v = 0; v < 6; ++v) {
+            const int k = vI(v), l = vJ(v);
+            const double val = al ? K11[v] : K00[v];
+            // entry (k, l) of block (a, b) for r < c and of the symmetric diagonal block for r == c (with
+            // its mirror), entry (l, k) of block (b, a) for r > c
+            const int kl = r > c ? 3 * l + k : 3 * k + l, lk = r > c ? 3 * k + l : 3 * l + k;
+            if (r <= c || k != l) put(base + (tr ? lk : kl), val);
+            if (r == c && k != l) put(base + lk, val);
+          }
+        }
+        __syncthreads();
       }
     }
-    __syncthreads();
     if constexpr (!TILED) {
       if (nvalidWave > 0)
         storeBlocks<TK>(&ebe[EPW * wave][0], Kebe + (e0 + (int64_t)blockIdx.x * kTileElems + EPW * wave) * 324,
                         nvalidWave * 324, lane);
     } else if (threadIdx.x < kTileThreads / 9 * 9) {
-      // group sums: 9 lanes per group (entry tt = 3k + l of the 3x3 block), kTileThreads / 9 groups per pass; a group's
-      // blocks are contiguous in LDS and added in list order; groups come ordered by slot, so a pass writes
-      // ascending slots
-      const double* lds = &ebe[0][0];
+      // group sums: 9 lanes per group (entry tt = 3k + l of the 3x3 block), kTileThreads / 9 groups per
+      // pass; groups come ordered by slot, so a pass writes ascending slots
+      const double* acc = &ebe[0][0];
       const int tt = threadIdx.x % 9, k = tt / 3, l = tt - 3 * (tt / 3);
       for (int g = threadIdx.x / 9; g < ng; g += kTileThreads / 9) {
-        const int c1 = go[g + 1];
-        double v     = 0.0;
-        for (int c = go[g]; c < c1; ++c) v += lds[c * 9 + tt];
+        double v = acc[g * 9 + tt];
         const int32_t d = gs[g];
         if (d >= 0) {
           Kebe[(int64_t)d * 9 + tt] = (TK)v;
@@ -678,7 +688,7 @@ void launchElem(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32
     (void)tiled;
     hipLaunchKernelGGL((hex8Element<KIND, T, WK, WR, true>), dim3((unsigned)count), dim3(kTileThreads), 0, s, h->X.p,
                        h->u.p, h->conn.p, h->tileElem.p, e0, count, h->lam, h->mu, h->tgptr.p, h->gdst.p, h->gpt.p,
-                       h->gmsk.p, h->epos.p, h->goff.p, K, mask, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h),
+                       h->gmsk.p, h->epos.p, h->tmaxr.p, K, mask, rdest, Kebe, Rebe, h->faceBits.p, loadedTraction(h),
                        h->errFlag.p);
   } else {
     const unsigned grid = (unsigned)((count + kTileElems - 1) / kTileElems);

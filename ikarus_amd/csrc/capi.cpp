@@ -586,5 +586,15 @@ int ik_timing_reset(ik_handle* h) {
 int ik_synchronize(ik_handle* h) {
   return guard(h, [&] { IK_HIP(hipStreamSynchronize(h->stream)); });
 }
+int ik_assembly_layout(ik_handle* h, int64_t* staged, int64_t* gathered, int64_t* direct) {
+  return guard(h, [&] {
+    ensurePattern(h);
+    const int64_t nup = (h->nnzb + h->nn) / 2;  // upper blocks incl. the diagonal (the pattern is symmetric)
+    const int64_t ng  = h->huptr.empty() ? 0 : h->huptr[h->nn];
+    if (staged) *staged = h->nstage;
+    if (gathered) *gathered = ng;
+    if (direct) *direct = nup - ng;
+  });
+}
 
 }  // extern "C"

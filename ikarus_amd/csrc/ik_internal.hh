@@ -136,17 +136,17 @@ struct ik_handle {
   //  staging slot per group (segments ordered by tile). tileElem[t*16+s] = element (-1: padding of the
   //  last tile); tgptr[t] = first group of tile t (groups ordered by upper block); gdst[g] = the slot of
   //  group g, or -(K block + 1) for a direct group, whose mirror block is gpt[g] and FULL mask gmsk[g].
-  //  The tile's entries j = 36 s + sb are listed group by group (each group's in increasing j);
-  //  epos[t*576 + j] = the position of entry j in that list | transposed << 15 (padding entries keep
-  //  position j, past the listed ones), and the offsets of tile t's groups in the list are
-  //  goff[tgptr[t] + t + i], i = 0..ng (the last one ends it).
+  //  epos[t*576 + j] = group | rank << 10 | transposed << 15 of the tile's entry j = 36 s + sb (rank = its
+  //  position in the group by increasing j; padding entries target spare accumulators past the groups),
+  //  tmaxr[t] = the tile's largest group size.
   // Residual staging: node by node, each node's (element, corner) incidences ordered by element;
   // rdest[e*8+a] = slot, ncptr[n] = first slot of node n
   ik::DevBuf<int32_t> bdest, segptr, ncptr, rdest;
   bool tiled = false;
   int64_t ntiles = 0, nstage = 0;  // tiles; staging slots of the tangent
   ik::DevBuf<int32_t> tileElem, tgptr, gdst, gpt;
-  ik::DevBuf<uint16_t> epos, goff, gmsk;
+  ik::DevBuf<uint16_t> epos, gmsk;
+  ik::DevBuf<uint8_t> tmaxr;
   // staging chunks (tiles when tiled, elements otherwise) and the block rows complete after each chunk
   std::vector<int64_t> chunkElem, chunkRow;
   // element-by-element staging of K_e (36 upper 3x3 blocks) and R_e (24)

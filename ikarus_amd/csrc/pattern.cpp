@@ -216,18 +216,23 @@ void buildPattern(ik_handle* h) {
         unitOf[order[i]] = t;
       }
     }
-    std::vector<uint16_t> gent((size_t)ntiles * kTileEntries, 0), epos((size_t)ntiles * kTileEntries), goff;
+    // per tile: groups (the entries j = 36 s + sb of one upper block) ordered by upper block, i.e. by staging
+    // slot; each entry's rank = its position in its group by increasing j (the order the tile sums them).
+    // epos[t*576 + j] = group | rank << 10 | transposed << 15; padding entries target spare accumulators
+    // ng + (j - 36 nvalid), rank 0, which the tile never stores. tmaxr[t] = the largest group size.
+    std::vector<uint16_t> epos((size_t)ntiles * kTileEntries);
+    std::vector<uint8_t> tmaxr(ntiles);
     std::vector<int32_t> tgptr(ntiles + 1, 0), gq, groupOf(nup, -1), touched;
     gq.reserve((size_t)ne * 24);
-    goff.reserve((size_t)ne * 24 + ntiles);
-    std::vector<int32_t> entQ(kTileEntries), cnt(kTileEntries + 1), rankOf, sizeOf(kTileEntries);
+    std::vector<int32_t> entQ(kTileEntries), order2, fill(kTileEntries);
     std::vector<uint16_t> entTr(kTileEntries);
     for (int64_t t = 0; t < ntiles; ++t) {
       touched.clear();
-      int nent = 0;
+      int nvalid = 0;
       for (int s = 0; s < kTileElems; ++s) {
         const int32_t e = tileElem[t * kTileElems + s];
         if (e < 0) continue;
+        ++nvalid;
         for (int a = 0; a < 8; ++a)
           for (int c = a; c < 8; ++c) {
             const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
@@ -236,40 +241,33 @@ void buildPattern(ik_handle* h) {
             entQ[j]         = q;
             entTr[j]        = conn[(int64_t)e * 8 + a] > conn[(int64_t)e * 8 + c] ? 0x8000u : 0u;
             if (groupOf[q] < 0) {
-              groupOf[q] = (int32_t)touched.size();
+              groupOf[q] = 0;
               touched.push_back(q);
             }
-            nent = j + 1;
           }
       }
-      // groups ordered by upper block, i.e. by staging slot, so a store pass writes ascending slots; counting
-      // sort of the entries by that order, stable in j
+      std::sort(touched.begin(), touched.end());
       const int ng = (int)touched.size();
-      std::fill(cnt.begin(), cnt.begin() + ng + 1, 0);
-      for (int j = 0; j < nent; ++j)
-        if (tileElem[t * kTileElems + j / 36] >= 0) cnt[groupOf[entQ[j]] + 1]++;
-      rankOf.resize(ng);
-      for (int g = 0; g < ng; ++g) rankOf[g] = g;
-      std::sort(rankOf.begin(), rankOf.end(), [&](int x, int y) { return touched[x] < touched[y]; });
-      for (int g = 0; g < ng; ++g) sizeOf[g] = cnt[g + 1];
-      for (int r = 0; r < ng; ++r) {
-        const int32_t q = touched[rankOf[r]];
-        groupOf[q]      = r;
-        cnt[r + 1]      = sizeOf[rankOf[r]];
-      }
-      for (int g = 0; g < ng; ++g) cnt[g + 1] += cnt[g];
-      for (int g = 0; g <= ng; ++g) goff.push_back((uint16_t)cnt[g]);
-      uint16_t* ge = gent.data() + t * kTileEntries;
-      for (int j = 0; j < nent; ++j)
-        if (tileElem[t * kTileElems + j / 36] >= 0) ge[cnt[groupOf[entQ[j]]]++] = (uint16_t)(j | entTr[j]);
-      // inverse: the list position of every listed entry (padding entries keep their own index, which lies
-      // past the listed positions because padding slots follow the tile's elements)
+      for (int g = 0; g < ng; ++g) groupOf[touched[g]] = g;
+      std::fill(fill.begin(), fill.begin() + ng, 0);
       uint16_t* ep = epos.data() + t * kTileEntries;
-      for (int j = 0; j < kTileEntries; ++j) ep[j] = (uint16_t)j;
-      const int nlisted = ng > 0 ? cnt[ng - 1] : 0;
-      for (int c = 0; c < nlisted; ++c) ep[ge[c] & 0x7FFFu] = (uint16_t)(c | (ge[c] & 0x8000u));
-      for (int r = 0; r < ng; ++r) gq.push_back(touched[rankOf[r]]);
-      for (const int32_t q : touched) groupOf[q] = -1;
+      int maxr     = 0;
+      for (int j = 0; j < kTileEntries; ++j) {
+        if (j < 36 * nvalid) {
+          const int g  = groupOf[entQ[j]];
+          const int rk = fill[g]++;
+          require(rk < 16, IK_EINVAL, "more than 16 element blocks of one tile on one K block");
+          maxr  = std::max(maxr, rk + 1);
+          ep[j] = (uint16_t)(g | (rk << 10) | entTr[j]);
+        } else {
+          ep[j] = (uint16_t)(ng + (j - 36 * nvalid));
+        }
+      }
+      tmaxr[t] = (uint8_t)maxr;
+      for (const int32_t q : touched) {
+        gq.push_back(q);
+        groupOf[q] = -1;
+      }
       require(gq.size() < (size_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
       tgptr[t + 1] = (int32_t)gq.size();
     }
@@ -295,12 +293,13 @@ void buildPattern(ik_handle* h) {
         gmsk[g] = mask[q];
       }
     }
-    // the kernel's indexing relies on these: a tile's group offsets increase from 0 to 36 x its elements,
-    // its entry list holds each entry of its elements exactly once, the slots are a permutation of
-    // [0, nstage) with each group inside its own upper block's segment, and every upper block is either
-    // staged by two or more tiles or written by exactly one
+    // the kernel's indexing relies on these: in every tile each valid entry has a distinct (group, rank)
+    // with group < ng and rank < its group's size < tmaxr + 1, every group is non-empty, padding entries
+    // target spare accumulators in [ng, 576); the slots are a permutation of [0, nstage) with each group
+    // inside its own upper block's segment, and every upper block is either staged by two or more tiles or
+    // written by exactly one
     {
-      std::vector<uint8_t> seen(kTileEntries), slotSeen(seg[nup], 0);
+      std::vector<uint8_t> slotSeen(seg[nup], 0);
       for (int64_t q = 0; q < nup; ++q)
         require(ntile[q] >= 1 && (gathered[q] ? seg[q + 1] - seg[q] == ntile[q] : seg[q + 1] == seg[q]), IK_EINVAL,
                 "internal error: upper block without a tile");
@@ -314,27 +313,30 @@ void buildPattern(ik_handle* h) {
                 "internal error: staging slots are not a permutation");
         slotSeen[sl] = 1;
       }
+      std::vector<uint32_t> used(kTileEntries * 16);
       for (int64_t t = 0; t < ntiles; ++t) {
-        std::fill(seen.begin(), seen.end(), 0);
-        const uint16_t* go = goff.data() + tgptr[t] + t;
-        const uint16_t* ge = gent.data() + t * kTileEntries;
-        const int ng       = tgptr[t + 1] - tgptr[t];
-        int nvalid         = 0;
+        const int ng = tgptr[t + 1] - tgptr[t];
+        int nvalid   = 0;
         for (int s = 0; s < kTileElems; ++s) nvalid += tileElem[t * kTileElems + s] >= 0;
-        require(go[0] == 0 && go[ng] == 36 * nvalid && tileElem[t * kTileElems] >= 0, IK_EINVAL,
-                "internal error: bad tile group offsets");
-        for (int j = 36 * nvalid; j < kTileEntries; ++j)  // padding entries land past the listed positions
-          require(epos[t * kTileEntries + j] == (uint16_t)j, IK_EINVAL, "internal error: bad padding entry");
-        for (int g = 0; g < ng; ++g) {
-          require(go[g] < go[g + 1], IK_EINVAL, "internal error: bad tile group");
-          for (int c = go[g]; c < go[g + 1]; ++c) {
-            const unsigned j = ge[c] & 0x7FFFu;
-            require(j < (unsigned)kTileEntries && !seen[j] && tileElem[t * kTileElems + j / 36] >= 0 &&
-                        epos[t * kTileEntries + j] == (uint16_t)(c | (ge[c] & 0x8000u)),
-                    IK_EINVAL, "internal error: bad tile entry");
-            seen[j] = 1;
+        require(tileElem[t * kTileElems] >= 0 && ng <= 36 * nvalid && tmaxr[t] >= 1 && tmaxr[t] <= 16, IK_EINVAL,
+                "internal error: bad tile");
+        std::fill(used.begin(), used.end(), 0);
+        std::fill(fill.begin(), fill.begin() + ng, 0);
+        const uint16_t* ep = epos.data() + t * kTileEntries;
+        for (int j = 0; j < kTileEntries; ++j) {
+          const int g = ep[j] & 0x3FF, rk = (ep[j] >> 10) & 0xF;
+          if (j >= 36 * nvalid) {
+            require(g >= ng && g < kTileEntries && rk == 0 && !(ep[j] & 0x8000u), IK_EINVAL,
+                    "internal error: bad padding entry");
+            continue;
           }
+          require(g < ng && rk < tmaxr[t] && !used[g * 16 + rk], IK_EINVAL, "internal error: bad tile entry");
+          used[g * 16 + rk] = 1;
+          fill[g]++;
         }
+        for (int g = 0; g < ng; ++g)
+          for (int rk = 0; rk < 16; ++rk)
+            require((rk < fill[g]) == (used[g * 16 + rk] != 0), IK_EINVAL, "internal error: gap in a tile group");
       }
     }
     h->ntiles = ntiles;
@@ -345,7 +347,7 @@ void buildPattern(ik_handle* h) {
     h->gpt.upload(gpt.data(), gpt.size(), h->stream);
     h->gmsk.upload(gmsk.data(), gmsk.size(), h->stream);
     h->epos.upload(epos.data(), epos.size(), h->stream);
-    h->goff.upload(goff.data(), goff.size(), h->stream);
+    h->tmaxr.upload(tmaxr.data(), tmaxr.size(), h->stream);
   }
   h->nstage = seg[nup];
   // the gather's upper blocks: the gathered ones in row order (huptr: first one of each row)

@@ -216,6 +216,10 @@ int ik_timing_enable(ik_handle* h, int on);
 int ik_timing_read(ik_handle* h, int kernel, int64_t* launches, double* total_ms);
 int ik_timing_reset(ik_handle* h);
 int ik_synchronize(ik_handle* h);
+/* Tangent staging layout (no counterpart in the reference; for bandwidth accounting): staged slots (3x3
+ * blocks written by the element kernel and read by the gather), upper blocks the gather assembles, and
+ * upper blocks the element kernel writes straight into K. Builds the pattern if needed. */
+int ik_assembly_layout(ik_handle* h, int64_t* staged_slots, int64_t* gathered_blocks, int64_t* direct_blocks);
 
 #ifdef __cplusplus
 }
