@@ -231,3 +231,62 @@ def test_fused_assembly_matches_separate_calls(dbc, eas):
         _, _, K4 = dev.csc(dbc)
         assert np.array_equal(K3, K4)
         assert np.array_equal(R3, R1)
+
+
+def _scrambled_block(n, seed, drop=0.25):
+    """A jittered block with a quarter of its elements removed, element order shuffled and nodes relabelled
+    at random (orphan nodes dropped): no lexicographic numbering left for the tiling to rely on."""
+    p = pb.hex_block(n, jitter=0.05, seed=seed, traction=(1.5, -2.0, 3.0))
+    rng = np.random.default_rng(seed)
+    keep = np.sort(rng.permutation(p.ne)[: int(round(p.ne * (1 - drop)))])
+    order = rng.permutation(len(keep))
+    old_elems = keep[order]                        # new element i is old element old_elems[i]
+    conn = p.conn[old_elems]
+    used = np.unique(conn)
+    relabel = np.full(p.X.shape[0], -1, np.int64)
+    relabel[used] = rng.permutation(len(used))     # new node id of each used old node
+    X = np.zeros((len(used), 3))
+    X[relabel[used]] = p.X[used]
+    fixed = np.zeros(3 * len(used), np.uint8)
+    fixed.reshape(-1, 3)[relabel[used]] = p.fixed.reshape(-1, 3)[used]
+    new_of_old = np.full(p.ne, -1, np.int64)
+    new_of_old[old_elems] = np.arange(len(old_elems))
+    q = pb.Problem(X, relabel[conn].astype(np.int32), p.material, p.lam, p.mu, name="scrambled")
+    q.fixed = fixed
+    faces = new_of_old[p.face_elem]
+    q.set_faces(np.sort(faces[faces >= 0]), int(p.face_axis[0]), int(p.face_side[0]))
+    q.traction = p.traction
+    return q
+
+
+def test_scrambled_mesh_assembly(monkeypatch):
+    """Parity of K (RAW and FULL) and R on a mesh with shuffled elements, relabelled nodes and holes, and
+    bitwise the same results for chunked assembly (tiles, direct writes and row completion away from any
+    lexicographic numbering)."""
+    p = _scrambled_block((9, 8, 7), seed=17)
+    print(f"case: scrambled mesh, {p.ne} elements, {p.X.shape[0]} nodes", flush=True)
+    o = _oracle_for(p)
+    u = 0.01 * np.random.default_rng(5).standard_normal(p.ndof) * (1 - p.fixed)
+    lam = 0.9
+    out = []
+    for chunk in (None, "64"):
+        if chunk is None:
+            monkeypatch.delenv("IKARUS_AMD_CHUNK_ELEMENTS", raising=False)
+        else:
+            monkeypatch.setenv("IKARUS_AMD_CHUNK_ELEMENTS", chunk)
+        dev = ia.Device(p)
+        dev.set_requirement(u, lam)
+        res = []
+        for dbc in (capi.DBC_RAW, capi.DBC_FULL):
+            R = dev.assemble(dbc)
+            outer, inner, vals = dev.csc(dbc)
+            if chunk is None:
+                oo, io = o.pattern()
+                assert np.array_equal(outer, oo) and np.array_equal(inner, io)
+                assert _relerr(vals, o.matrix(u, lam, dbc)) < 1e-10
+                assert _relerr(R, o.vector(u, lam, dbc)) < 1e-10
+            res += [vals, R]
+        out.append(res)
+        dev.close()
+    for a, b in zip(out[1], out[0]):
+        assert np.array_equal(a, b)
