@@ -27,6 +27,10 @@ struct Error {
 inline void require(bool ok, int code, const std::string& msg) {
   if (!ok) throw Error{code, msg};
 }
+// literal messages: the string is built only on failure (the pattern checks call this millions of times)
+inline void require(bool ok, int code, const char* msg) {
+  if (!ok) throw Error{code, msg};
+}
 
 // Owning device buffer. Grows only; contents are not preserved on growth.
 template <class T>
