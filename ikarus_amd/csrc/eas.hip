@@ -11,12 +11,11 @@
 //   alpha <- alpha - D^{-1}(Rt + L du_e) on CORRECTION_UPDATED, with D, L, Rt of the state the tangent
 //   was assembled at (the reference recomputes them at that same state).
 //
-// One 64-lane workgroup per element. Lanes 0-7 evaluate the 8 Gauss points (kinematics, enhanced strain,
-// S, C, M) into LDS; all 64 lanes then accumulate the 36 upper 3x3 blocks of K_uu, D, L, Rt and R_u per
-// Gauss point from LDS; the condensation is a right-looking LDL^T of D in LDS followed by 25
-// forward/backward substitutions (columns of L and Rt) and the rank-21 update of K and R. Outputs go to
-// the same element-by-element staging as the plain element (36 upper blocks + 24 residual entries), plus
-// the cached Y = D^{-1} L (21x24) and y = D^{-1} Rt (21) used by the alpha update.
+// One 64-lane workgroup (one wave) per element, all lanes busy in every phase: Gauss-point kinematics and
+// material per (point, node) lane pair, K_uu in the Ci form per (node, node) lane pair, D / L / Rt / R_u by
+// owned entries, Gauss-Jordan elimination of [D | L | Rt] one column per lane, and the rank-21 update of K and R.
+// Outputs go to the same element-by-element staging as the plain element (36 upper blocks + 24 residual
+// entries), plus the cached Y = D^{-1} L (21x24) and y = D^{-1} Rt (21) used by the alpha update.
 #include "ik_internal.hh"
 
 namespace ik {
@@ -31,9 +30,26 @@ __host__ __device__ constexpr int sym6(int p, int q) {
 }
 __host__ __device__ constexpr int upperBlock(int a, int b) { return 8 * a - (a * (a - 1)) / 2 + (b - a); }
 
+// Row-major upper triangle of an n x n matrix: entry idx in [0, n(n+1)/2) -> (row i, column c >= i). Rows
+// before i hold S(i) = i n - i(i-1)/2 entries, so i is the root of that quadratic, rounded down (the float
+// root is off by at most one, which the integer checks correct).
+__device__ inline void triDecode(int idx, int n, int& i, int& c) {
+  const float b = 2.0f * n + 1.0f;
+  int r         = (int)((b - sqrtf(fmaxf(b * b - 8.0f * idx, 0.0f))) * 0.5f);
+  r             = r < 0 ? 0 : (r > n - 1 ? n - 1 : r);
+  auto S        = [n](int k) { return k * n - (k * (k - 1)) / 2; };
+  if (r < n - 1 && S(r + 1) <= idx) ++r;
+  if (r > 0 && S(r) > idx) --r;
+  i = r;
+  c = r + (idx - S(r));
+}
+
 // E21 ansatz in reference coordinates: column c has one nonzero, in Voigt row kRow[c], of value
 // f_c(x, y, z) with x = 2 xi - 1 etc. (linearandglstrains.hh:300-322)
 __constant__ int kRow[NE] = {0, 1, 2, 3, 3, 4, 4, 5, 5, 3, 3, 4, 4, 5, 5, 0, 0, 1, 1, 2, 2};
+__host__ __device__ constexpr int kRowOf(int c) {
+  return c < 3 ? c : (c < 9 ? 3 + (c - 3) / 2 : (c < 15 ? 3 + (c - 9) / 2 : (c - 15) / 2));
+}
 __device__ inline double ansatz(int c, double x, double y, double z) {
   switch (c) {
     case 0: return x;
@@ -88,31 +104,52 @@ __device__ inline void jacobianAt(const double (*X)[3], const double xi[3], doub
   }
 }
 
+// Voigt index of the symmetric pair (i, j): 00 11 22 12 02 01
+__host__ __device__ constexpr int v6(int i, int j) {
+  return i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5));
+}
+
+// One 64-lane workgroup (one wave) per element; the LDS holds the Gauss-point data during the point loop and the
+// condensation operands afterwards (a union: ~10 KB, so the VGPR budget sets the occupancy).
 struct EasLds {
   double X[8][3], U[8][3];
   double alpha[NE];
-  double T0[6][6];     // (T(xi0) detJ0)^-1
-  double G[8][8][3];   // [gp][node][dir]
-  double F[8][9];
-  double S[8][6];      // S dV
-  double C[8][21];     // C dV (packed)
-  double f[8][NE];     // E21 mode scalings f_c(xi_gp) / detJ_gp: column c of M = T0^-1[:, kRow[c]] f_c
-  double B[6][24];     // current Gauss point
-  double CB[6][24];    // C dV B
-  double CT[6][6];     // C dV T0^-1
-  double Ch[6][6];     // T0^-T C dV T0^-1
-  double Z[6][24];     // (C dV T0^-1)^T B
-  double TS[6];        // T0^-T S dV
-  double A[NE][NE];    // D, then its LDL^T factors
-  double Lm[NE][24];   // L, then D^{-1} L
-  double rt[NE];       // Rt, then D^{-1} Rt
-  double K[36][9];     // K_uu upper blocks
-  double Ru[24];
+  double T0[6][6];  // (T(xi0) detJ0)^-1
+  union {
+    struct {
+      double G[8][8][3];  // [gp][node][dir] shape-function gradients g_a
+      double P[8][8][3];  // F Ci g_a
+      double F[8][9];
+      double S[8][6];     // S dV
+      double C[8][21];    // C dV (packed)
+      double Ci[8][6];    // Ci (Voigt order), the identity for SVK
+      double Phi[8][6];   // F Ci F^T (Voigt order)
+      double w[8][2];     // lambda dV, c2 dV: material tangent weights of the Ci form
+      double f[8][NE];    // E21 mode scalings f_c(xi_gp) / detJ_gp: column c of M = T0^-1[:, kRow[c]] f_c
+      double CT[6][6];    // current point: C dV T0^-1
+      double Ch[6][6];    // T0^-T C dV T0^-1
+      double TS[6];       // T0^-T S dV
+      double Z[6][24];    // (C dV T0^-1)^T B
+      double ru[8][8][3]; // R_u contribution F S dV g_a of point gp, node a
+    } g;
+    struct {
+      double A[NE][24];   // D (columns 0-20), then Y = D^-1 L
+      double Lm[NE][24];  // L
+      double rt[NE];      // Rt, then y = D^-1 Rt
+      double piv[NE];     // pivot column of the elimination step
+      double Ru[24];
+      double Kuu[36][9];  // K_uu upper blocks while the elimination runs
+    } c;
+  } u;
   int bad;
 };
 
+// S and C (Voigt) at the enhanced strain, plus the factors of the tangent's Ci form
+// C = lambda Ci(x)Ci + c2 (Ci box Ci + transposed): Ci (Voigt) and c2 (Neo-Hooke: mu - lambda ln J; SVK: Ci = I,
+// c2 = mu)
 template <int KIND>
-__device__ inline bool material(double lam, double mu, const double Ev[6], double S[6], double Cp[21]) {
+__device__ inline bool material(double lam, double mu, const double Ev[6], double S[6], double Cp[21], double Ci6[6],
+                                double& c2) {
   if constexpr (KIND == IK_SVK) {
     const double tr = Ev[0] + Ev[1] + Ev[2];
     for (int i = 0; i < 3; ++i) S[i] = lam * tr + 2.0 * mu * Ev[i];
@@ -121,6 +158,8 @@ __device__ inline bool material(double lam, double mu, const double Ev[6], doubl
     for (int p = 0; p < 3; ++p)
       for (int q = p; q < 3; ++q) Cp[sym6(p, q)] = lam + (p == q ? 2.0 * mu : 0.0);
     for (int p = 3; p < 6; ++p) Cp[sym6(p, p)] = mu;
+    for (int v = 0; v < 6; ++v) Ci6[v] = v < 3 ? 1.0 : 0.0;
+    c2 = mu;
     return true;
   } else {
     double C[3][3];
@@ -143,9 +182,10 @@ __device__ inline bool material(double lam, double mu, const double Ev[6], doubl
     Ci[2][1] = (C[0][1] * C[2][0] - C[0][0] * C[2][1]) * id;
     Ci[2][2] = (C[0][0] * C[1][1] - C[0][1] * C[1][0]) * id;
     const double lnJ = log(sqrt(detC));
-    const double c2  = mu - lam * lnJ;
+    c2               = mu - lam * lnJ;
     for (int v = 0; v < 6; ++v) {
       const double ci = Ci[vI(v)][vJ(v)];
+      Ci6[v]          = ci;
       S[v]            = mu * ((v < 3 ? 1.0 : 0.0) - ci) + lam * lnJ * ci;
     }
     for (int p = 0; p < 6; ++p)
@@ -157,6 +197,20 @@ __device__ inline bool material(double lam, double mu, const double Ev[6], doubl
   }
 }
 
+// y = A x for a symmetric 3x3 A in Voigt order
+__device__ inline void symMul(const double* A6, const double x[3], double y[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) y[i] = A6[v6(i, 0)] * x[0] + A6[v6(i, 1)] * x[1] + A6[v6(i, 2)] * x[2];
+}
+
+// Phases (all 64 lanes): (1) Gauss-point kinematics, enhanced strain and material, lane (gp, node): each lane
+// keeps its node's g_a and F Ci g_a, node 0 the point's data; (2) the point loop: K_uu in the Ci form,
+//   K_ab[k][l] += lambda dV p_ak p_bl + c2 dV (p_al p_bk + (g_a.Ci g_b) Phi_kl) + (g_a.S dV g_b) delta_kl,
+//   p_a = F Ci g_a, Phi = F Ci F^T (the material part of B^T C B written with C = lambda Ci(x)Ci + 2 c2 sym(Ci box
+//   Ci); SVK: Ci = I, c2 = mu), lane (a, b) per 3x3 block; then D, L = sum f_i Z[kRow_i], Rt and R_u
+//   = sum (F S dV g_b)_l with Z[w][3b+l] = (F Q^w g_b)_l, Q^w the symmetric tensor of column w of C dV T0^-1;
+// (3) Y = D^-1 [L | Rt] by Gauss-Jordan, one column per lane; (4) K = K_uu - L^T Y, lane (a, b) per block (the
+// diagonal blocks mirrored from their upper triangle), R = R_u - L^T y.
 template <int KIND, bool WANT_K>
 __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, const double* __restrict__ U,
                                                  const int32_t* __restrict__ conn,
@@ -167,17 +221,19 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
                                                  double3 trac, double* __restrict__ cache,
                                                  unsigned long long* __restrict__ err) {
   __shared__ EasLds s;
+  auto& sg        = s.u.g;
+  auto& sc        = s.u.c;
   const int t     = threadIdx.x;
   const int64_t e = e0 + blockIdx.x;
-  if (t < 8) {
-    const int64_t nd = conn[e * 8 + t];
-    for (int k = 0; k < 3; ++k) {
-      s.X[t][k] = X[nd * 3 + k];
-      s.U[t][k] = U[nd * 3 + k];
-    }
+  if (t < 24) {
+    const int64_t nd = conn[e * 8 + t / 3];
+    s.X[t / 3][t % 3] = X[nd * 3 + t % 3];
+    s.U[t / 3][t % 3] = U[nd * 3 + t % 3];
+  } else if (t >= 32 && t < 32 + NE) {
+    s.alpha[t - 32] = alpha[e * NE + (t - 32)];
+  } else if (t == 63) {
+    s.bad = 0;
   }
-  if (t < NE) s.alpha[t] = alpha[e * NE + t];
-  if (t == 0) s.bad = 0;
   __syncthreads();
   // (T(xi0) detJ0)^-1 (tensorutils.hh:440-465; helperfunctions.hh:19-25). The Voigt strain transformation is
   // a representation of the Jacobian, T(J)^-1 = T(J^-1), so each of 36 lanes writes one entry directly.
@@ -203,20 +259,20 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     s.T0[r][c]     = (a == b ? 0.5 * v : v) / fabs(dJ);
   }
   __syncthreads();
-  // Gauss point kinematics (lanes 0-7)
-  if (t < 8) {
+  // (1) Gauss point gp = t / 8 (kinematics repeated by its 8 lanes), node na = t % 8
+  {
+    const int gp = t >> 3, na = t & 7;
     const double d = 0.28867513459481288225;
     double xi[3];
-    for (int k = 0; k < 3; ++k) xi[k] = ((t >> k) & 1) ? 0.5 + d : 0.5 - d;
-    double dN[8][3];
-    for (int a = 0; a < 8; ++a) dShape(a, xi, dN[a]);
-    double J[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        double acc = 0.0;
-        for (int a = 0; a < 8; ++a) acc += s.X[a][i] * dN[a][j];
-        J[i][j] = acc;
-      }
+    for (int k = 0; k < 3; ++k) xi[k] = ((gp >> k) & 1) ? 0.5 + d : 0.5 - d;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      double dN[3];
+      dShape(a, xi, dN);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) J[i][j] += s.X[a][i] * dN[j];
+    }
     const double detJ = det3(J);
     const double id   = 1.0 / detJ;
     double Ji[3][3];
@@ -230,13 +286,19 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
     Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
     Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
     double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
     for (int a = 0; a < 8; ++a) {
-      double g[3];
-      for (int i = 0; i < 3; ++i) g[i] = Ji[0][i] * dN[a][0] + Ji[1][i] * dN[a][1] + Ji[2][i] * dN[a][2];
-      for (int i = 0; i < 3; ++i) {
-        s.G[t][a][i] = g[i];
+      double dN[3], g[3];
+      dShape(a, xi, dN);
+      for (int i = 0; i < 3; ++i) g[i] = Ji[0][i] * dN[0] + Ji[1][i] * dN[1] + Ji[2][i] * dN[2];
+      for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) F[i][j] += s.U[a][i] * g[j];
-      }
+    }
+    double go[3];
+    {
+      double dN[3];
+      dShape(na, xi, dN);
+      for (int i = 0; i < 3; ++i) go[i] = Ji[0][i] * dN[0] + Ji[1][i] * dN[1] + Ji[2][i] * dN[2];
     }
     double Ev[6];
     for (int v = 0; v < 6; ++v) {
@@ -244,207 +306,196 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
       const double c = F[0][I] * F[0][Jj] + F[1][I] * F[1][Jj] + F[2][I] * F[2][Jj];
       Ev[v]          = v < 3 ? 0.5 * (c - 1.0) : c;  // engineering shear 2E_IJ = C_IJ
     }
-    // M = T0^{-1} Mtilde / detJ; E_enh = E + M alpha
+    // E_enh = E + M alpha, M = T0^{-1} Mtilde / detJ: Mtilde alpha has one term per mode in row kRow[c]
     const double x = 2 * xi[0] - 1.0, y = 2 * xi[1] - 1.0, z = 2 * xi[2] - 1.0;
-    const double sc = 1.0 / fabs(detJ);
-    for (int c = 0; c < NE; ++c) {
-      const double f = ansatz(c, x, y, z) * sc;
-      const int rr   = kRow[c];
-      s.f[t][c] = f;
-      for (int r = 0; r < 6; ++r) Ev[r] += s.T0[r][rr] * f * s.alpha[c];
+    const double sc1 = 1.0 / fabs(detJ);
+    double wv[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < NE; ++c) wv[kRowOf(c)] += ansatz(c, x, y, z) * s.alpha[c];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc += s.T0[r][q] * wv[q];
+      Ev[r] += acc * sc1;
     }
-    double Sv[6], Cp[21];
-    if (!material<KIND>(lam, mu, Ev, Sv, Cp)) {
+    double Sv[6], Cp[21], Ci6[6], c2;
+    if (!material<KIND>(lam, mu, Ev, Sv, Cp, Ci6, c2)) {
       atomicMin(err, (unsigned long long)e);
       s.bad = 1;
     }
     const double dV = fabs(detJ) * 0.125;
-    for (int i = 0; i < 9; ++i) s.F[t][i] = F[i / 3][i % 3];
-    for (int v = 0; v < 6; ++v) s.S[t][v] = Sv[v] * dV;
-    for (int i = 0; i < 21; ++i) s.C[t][i] = Cp[i] * dV;
+    double q[3], p[3];
+    symMul(Ci6, go, q);
+    for (int i = 0; i < 3; ++i) p[i] = F[i][0] * q[0] + F[i][1] * q[1] + F[i][2] * q[2];
+    double sgo[3];
+    symMul(Sv, go, sgo);
+    for (int i = 0; i < 3; ++i) {
+      sg.G[gp][na][i]  = go[i];
+      sg.P[gp][na][i]  = p[i];
+      sg.ru[gp][na][i] = (F[i][0] * sgo[0] + F[i][1] * sgo[1] + F[i][2] * sgo[2]) * dV;
+    }
+    if (na == 0) {
+      for (int i = 0; i < 9; ++i) sg.F[gp][i] = F[i / 3][i % 3];
+      for (int v = 0; v < 6; ++v) sg.S[gp][v] = Sv[v] * dV;
+      for (int i = 0; i < 21; ++i) sg.C[gp][i] = Cp[i] * dV;
+      for (int v = 0; v < 6; ++v) sg.Ci[gp][v] = Ci6[v];
+      // Phi = F Ci F^T
+      double FC[3][3];
+      for (int i = 0; i < 3; ++i) symMul(Ci6, F[i], FC[i]);  // row i of F Ci (Ci symmetric)
+      for (int v = 0; v < 6; ++v) {
+        const int k = vI(v), l = vJ(v);
+        sg.Phi[gp][v] = FC[k][0] * F[l][0] + FC[k][1] * F[l][1] + FC[k][2] * F[l][2];
+      }
+      sg.w[gp][0] = lam * dV;
+      sg.w[gp][1] = c2 * dV;
+#pragma unroll
+      for (int c = 0; c < NE; ++c) sg.f[gp][c] = ansatz(c, x, y, z) * sc1;
+    }
   }
   __syncthreads();
   if (s.bad) return;  // uniform: the error flag is already set
 
-  // accumulators: lane t < 36 owns K_uu block t; lanes own D / L / Rt / R_u entries (strided). The entry
-  // each lane owns does not change over the Gauss points, so its indices are decoded once here.
+  // (2) point loop. Lane (ka, kb) accumulates K_uu block (ka, kb); D upper entries t + 64 j (< 231), L entries
+  // t + 64 j (< 504), Rt (lanes 0-20) and R_u (lanes 40-63) are owned by fixed lanes.
+  const int ka = t >> 3, kbn = t & 7;
   double kb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  double dacc[4] = {0, 0, 0, 0};   // D upper entries t, t+64, t+128, t+192 (< 231)
-  double lacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // L entries t + 64 j (< 504)
-  double racc = 0.0;               // lanes 0-20: Rt ; lanes 32-55: R_u
-  int blkA = 0, blkB = 0;
-  if (t < 36) {
-    int r = t, a = 0;
-    while (r >= 8 - a) {
-      r -= 8 - a;
-      ++a;
-    }
-    blkA = a;
-    blkB = a + r;
-  }
-  int dI[4], dC[4];
+  double dacc[4] = {0, 0, 0, 0};
+  double racc    = 0.0;
+  int dI[4], dC[4], dKi[4], dKc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    int r = t + 64 * j, i = 0;
-    if (r < 231)
-      while (r >= NE - i) {
-        r -= NE - i;
-        ++i;
-      }
-    dI[j] = i;
-    dC[j] = i + r;
+    triDecode(t + 64 * j < 231 ? t + 64 * j : 230, NE, dI[j], dC[j]);
+    dKi[j] = kRow[dI[j]];
+    dKc[j] = kRow[dC[j]];
   }
-  int dKi[4], dKc[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    dKi[j] = kRow[dI[j] < NE ? dI[j] : 0];
-    dKc[j] = kRow[dC[j] < NE ? dC[j] : 0];
-  }
-  int lI[8], lCol[8], lK[8];  // L entry idx = t + 64 j: row (mode) i, column, kRow[i]
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int idx = t + 64 * j;
-    lI[j]   = idx / 24;
-    lCol[j] = idx % 24;
-    lK[j]   = kRow[lI[j] < NE ? lI[j] : 0];
-  }
+  // L: lane t < 48 owns column lc = t % 24 of rows lh + 2m (lh = t / 24)
+  const int lc = t % 24, lh = t / 24;
+  double lacc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int rK = kRow[t < NE ? t : 0];
-  int bV[3], bCol[3], bG[3], bK[3];  // B build / C B / Z: entry idx = t + 64 j of the 6 x 24 matrices
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const int idx = t + 64 * j;
-    bV[j]   = idx / 24;
-    bCol[j] = idx % 24;
-    bG[j]   = bCol[j] / 3;
-    bK[j]   = bCol[j] % 3;
-  }
   for (int g = 0; g < 8; ++g) {
-    // B (6x24) and CT = C dV T0^-1 for this Gauss point
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      if (t + 64 * j < 144) {
-        const double* F  = s.F[g];
-        const double* ga = s.G[g][bG[j]];
-        const int I = vI(bV[j]), J = vJ(bV[j]), k = bK[j];
-        double bv = F[k * 3 + I] * ga[J];
-        if (I != J) bv += F[k * 3 + J] * ga[I];
-        s.B[bV[j]][bCol[j]] = bv;
-      }
-    }
     if (t < 36) {
       const int v = t / 6, w = t % 6;
       double acc = 0.0;
 #pragma unroll
-      for (int u = 0; u < 6; ++u) acc += s.C[g][sym6(v, u)] * s.T0[u][w];
-      s.CT[v][w] = acc;
+      for (int u = 0; u < 6; ++u) acc += sg.C[g][sym6(v, u)] * s.T0[u][w];
+      sg.CT[v][w] = acc;
     }
-    __syncthreads();
-    // C B (for K_uu), Z = CT^T B (for L), Ch = T0^-T CT (for D), TS = T0^-T S (for Rt)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      if (t + 64 * j < 144) {
-        double acc = 0.0, accz = 0.0;
-#pragma unroll
-        for (int w = 0; w < 6; ++w) {
-          if constexpr (WANT_K) acc += s.C[g][sym6(bV[j], w)] * s.B[w][bCol[j]];
-          accz += s.CT[w][bV[j]] * s.B[w][bCol[j]];
-        }
-        if constexpr (WANT_K) s.CB[bV[j]][bCol[j]] = acc;
-        s.Z[bV[j]][bCol[j]] = accz;
-      }
-    }
-    if (t < 36) {
-      const int p = t / 6, q = t % 6;
-      double acc = 0.0;
-#pragma unroll
-      for (int v = 0; v < 6; ++v) acc += s.T0[v][p] * s.CT[v][q];
-      s.Ch[p][q] = acc;
-    } else if (t >= 40 && t < 46) {
-      const int w = t - 40;
-      double acc = 0.0;
-#pragma unroll
-      for (int v = 0; v < 6; ++v) acc += s.T0[v][w] * s.S[g][v];
-      s.TS[w] = acc;
-    }
-    __syncthreads();
     if constexpr (WANT_K) {
-      if (t < 36) {
-        // K_uu block (a, b): sum_v B[v][3a+k] (C B)[v][3b+l] + g_a^T S g_b delta_kl
-        const double* ga = s.G[g][blkA];
-        const double* gb = s.G[g][blkB];
-        const double* Sv = s.S[g];
-        const double kg  = ga[0] * (Sv[0] * gb[0] + Sv[5] * gb[1] + Sv[4] * gb[2]) +
-                          ga[1] * (Sv[5] * gb[0] + Sv[1] * gb[1] + Sv[3] * gb[2]) +
-                          ga[2] * (Sv[4] * gb[0] + Sv[3] * gb[1] + Sv[2] * gb[2]);
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int l = 0; l < 3; ++l) {
-            double acc = 0.0;
-#pragma unroll
-            for (int v = 0; v < 6; ++v) acc += s.B[v][3 * blkA + k] * s.CB[v][3 * blkB + l];
-            kb[k * 3 + l] += acc + (k == l ? kg : 0.0);
-          }
+      double pa[3], pb[3], ga[3], gb[3], Ci6[6], S6[6], Ph[6];
+      for (int i = 0; i < 3; ++i) {
+        pa[i] = sg.P[g][ka][i];
+        pb[i] = sg.P[g][kbn][i];
+        ga[i] = sg.G[g][ka][i];
+        gb[i] = sg.G[g][kbn][i];
       }
+      for (int v = 0; v < 6; ++v) {
+        Ci6[v] = sg.Ci[g][v];
+        S6[v]  = sg.S[g][v];
+        Ph[v]  = sg.Phi[g][v];
+      }
+      const double wl = sg.w[g][0], wc = sg.w[g][1];
+      double qb[3], sb[3];
+      symMul(Ci6, gb, qb);
+      symMul(S6, gb, sb);
+      const double gCg = ga[0] * qb[0] + ga[1] * qb[1] + ga[2] * qb[2];
+      const double gSg = ga[0] * sb[0] + ga[1] * sb[1] + ga[2] * sb[2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          kb[3 * k + l] += wl * pa[k] * pb[l] + wc * (pa[l] * pb[k] + gCg * Ph[v6(k, l)]) + (k == l ? gSg : 0.0);
     }
-    // D[i][c] += f_i f_c Ch[kRow_i][kRow_c]; L[i][col] += f_i Z[kRow_i][col]; Rt[i] += f_i TS[kRow_i]
-    const double* fg = s.f[g];
+    __syncthreads();
+    // Z[w][3b+l] = (F Q^w g_b)_l (lane (w, b) < 48), Ch = T0^-T CT (lanes 48-63, then 43-47), TS (lanes 37-42)
+    if (t < 48) {
+      const int w = t >> 3, b = t & 7;
+      double Q[6], gbv[3], h[3];
+      for (int v = 0; v < 6; ++v) Q[v] = sg.CT[v][w];
+      for (int i = 0; i < 3; ++i) gbv[i] = sg.G[g][b][i];
+      symMul(Q, gbv, h);
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+        sg.Z[w][3 * b + l] = sg.F[g][3 * l] * h[0] + sg.F[g][3 * l + 1] * h[1] + sg.F[g][3 * l + 2] * h[2];
+    }
+    if (t >= 48 || (t >= 43 && t < 48)) {
+      int p, q;
+      triDecode(t >= 48 ? t - 48 : t - 27, 6, p, q);
+      double acc = 0.0;
+#pragma unroll
+      for (int v = 0; v < 6; ++v) acc += s.T0[v][p] * sg.CT[v][q];
+      sg.Ch[p][q] = acc;
+      sg.Ch[q][p] = acc;
+    }
+    if (t >= 37 && t < 43) {
+      const int w = t - 37;
+      double acc = 0.0;
+#pragma unroll
+      for (int v = 0; v < 6; ++v) acc += s.T0[v][w] * sg.S[g][v];
+      sg.TS[w] = acc;
+    }
+    __syncthreads();
+    // D[i][c] += f_i f_c Ch[kRow_i][kRow_c]; L[i][col] += f_i Z[kRow_i][col]; Rt[i] += f_i TS[kRow_i];
+    // R_u[3b+l] += (F S dV g_b)_l
+    const double* fg = sg.f[g];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      if (t + 64 * j < 231) dacc[j] += fg[dI[j]] * fg[dC[j]] * s.Ch[dKi[j]][dKc[j]];
+      if (t + 64 * j < 231) dacc[j] += fg[dI[j]] * fg[dC[j]] * sg.Ch[dKi[j]][dKc[j]];
+    if (t < 48) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (t + 64 * j < NE * 24) lacc[j] += fg[lI[j]] * s.Z[lK[j]][lCol[j]];
-    if (t < NE) {
-      racc += fg[t] * s.TS[rK];
-    } else if (t >= 32 && t < 56) {
-      const int col = t - 32;
-      double acc    = 0.0;
-#pragma unroll
-      for (int v = 0; v < 6; ++v) acc += s.B[v][col] * s.S[g][v];
-      racc += acc;
+      for (int m = 0; m < 11; ++m) {
+        if (m < 10 || lh == 0) {
+          const int kr = lh ? kRowOf(2 * m + 1) : kRowOf(2 * m);
+          lacc[m] += fg[lh + 2 * m] * sg.Z[kr][lc];
+        }
+      }
     }
-    __syncthreads();
+    if (t < NE) racc += fg[t] * sg.TS[rK];
   }
-  // spill accumulators to LDS
+  if (t >= 40) {
+    const int col = t - 40;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) racc += sg.ru[g][col / 3][col % 3];
+  }
+  __syncthreads();  // the point data is dead from here: the condensation operands take its place
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (t + 64 * j < 231) {
-      s.A[dI[j]][dC[j]] = dacc[j];
-      s.A[dC[j]][dI[j]] = dacc[j];
+      sc.A[dI[j]][dC[j]] = dacc[j];
+      sc.A[dC[j]][dI[j]] = dacc[j];
     }
   }
+  if (t < 48) {
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int idx = t + 64 * j;
-    if (idx < NE * 24) s.Lm[idx / 24][idx % 24] = lacc[j];
+    for (int m = 0; m < 11; ++m)
+      if (m < 10 || lh == 0) sc.Lm[lh + 2 * m][lc] = lacc[m];
   }
-  if (t < NE) s.rt[t] = racc;
-  if (t >= 32 && t < 56) s.Ru[t - 32] = racc;
+  if (t < NE) sc.rt[t] = racc;
+  if (t >= 40) sc.Ru[t - 40] = racc;
   if constexpr (WANT_K) {
-    if (t < 36)
-      for (int i = 0; i < 9; ++i) s.K[t][i] = kb[i];
+    if (ka <= kbn)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) sc.Kuu[upperBlock(ka, kbn)][i] = kb[i];
   }
   __syncthreads();
 
-  // Y = D^-1 [L | Rt] by Gauss-Jordan elimination on the augmented 21 x 46 matrix (D is SPD: no pivoting,
+  // (3) Y = D^-1 [L | Rt] by Gauss-Jordan elimination on the augmented 21 x 46 matrix (D is SPD: no pivoting,
   // like the LDL^T it replaces; enhancedassumedstrains.hh inverts D, :243,294,344). Lane c < 46 keeps column
   // c in registers; each step the pivot column is broadcast through LDS.
   double col[NE];
 #pragma unroll
-  for (int i = 0; i < NE; ++i) col[i] = t < NE ? s.A[i][t] : (t < NE + 24 ? s.Lm[i][t - NE] : (t == NE + 24 ? s.rt[i] : 0.0));
-  __syncthreads();
-  double* pivbuf = &s.A[0][0];  // D is in the lanes' registers from here on
+  for (int i = 0; i < NE; ++i)
+    col[i] = t < NE ? sc.A[i][t] : (t < NE + 24 ? sc.Lm[i][t - NE] : (t == NE + 24 ? sc.rt[i] : 0.0));
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
     if (t == j)
 #pragma unroll
-      for (int i = 0; i < NE; ++i) pivbuf[i] = col[i];
+      for (int i = 0; i < NE; ++i) sc.piv[i] = col[i];
     __syncthreads();
-    const double f = col[j] / pivbuf[j];
+    const double f = col[j] / sc.piv[j];
 #pragma unroll
     for (int i = 0; i < NE; ++i)
-      if (i != j) col[i] -= pivbuf[i] * f;
+      if (i != j) col[i] -= sc.piv[i] * f;
     col[j] = f;
     __syncthreads();
   }
@@ -453,49 +504,58 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   double* cy = cache + e * (NE * 24 + NE);
   if (t >= NE && t < NE + 24) {
 #pragma unroll
-    for (int i = 0; i < NE; ++i) cy[i * 24 + (t - NE)] = col[i];
+    for (int i = 0; i < NE; ++i) {
+      cy[i * 24 + (t - NE)] = col[i];
+      sc.A[i][t - NE]       = col[i];
+    }
   } else if (t == NE + 24) {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       cy[NE * 24 + i] = col[i];
-      s.rt[i]         = col[i];  // y for the residual lanes
-    }
-  }
-  // K = K_uu - L^T Y on the upper blocks: lane of column c = 3b+l computes rows 3a+k of blocks a < b and,
-  // for the diagonal block, k <= l (mirrored afterwards: enhancedassumedstrains.hh:291-296)
-  if constexpr (WANT_K) {
-    if (t >= NE && t < NE + 24) {
-      const int c = t - NE, b = c / 3, l = c % 3;
-      for (int r = 0; r <= c; ++r) {
-        const int a = r / 3, k = r % 3;
-        if (a == b && k > l) continue;
-        double acc = s.K[upperBlock(a, b)][3 * k + l];
-#pragma unroll
-        for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * col[i];
-        s.K[upperBlock(a, b)][3 * k + l] = acc;
-        if (a == b && k != l) s.K[upperBlock(a, b)][3 * l + k] = acc;
-      }
+      sc.rt[i]        = col[i];
     }
   }
   __syncthreads();
+  // (4) K = K_uu - L^T Y, block (ka, kb) per lane; upper blocks stored, the diagonal ones mirrored from their
+  // upper triangle (enhancedassumedstrains.hh:291-296)
   if constexpr (WANT_K) {
-    if (t < 36) {
-      const double* out = s.K[t];
+    const int blk = upperBlock(ka <= kbn ? ka : kbn, ka <= kbn ? kbn : ka);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) kb[i] = sc.Kuu[blk][i];
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+      double la[3], yb[3];
+      for (int k = 0; k < 3; ++k) {
+        la[k] = sc.Lm[i][3 * ka + k];
+        yb[k] = sc.A[i][3 * kbn + k];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) kb[3 * k + l] -= la[k] * yb[l];
+    }
+    if (ka == kbn) {
+      kb[3] = kb[1];
+      kb[6] = kb[2];
+      kb[7] = kb[5];
+    }
+    if (ka <= kbn) {
       if (dest) {
-        const int32_t code = dest[e * 36 + t];
+        const int32_t code = dest[e * 36 + blk];
         double* dst        = Kebe + (int64_t)(code >> 1) * 9;
         for (int k = 0; k < 3; ++k)
-          for (int l = 0; l < 3; ++l) dst[3 * k + l] = (code & 1) ? out[3 * l + k] : out[3 * k + l];
+          for (int l = 0; l < 3; ++l) dst[3 * k + l] = (code & 1) ? kb[3 * l + k] : kb[3 * k + l];
       } else {
-        double* dst = Kebe + (e * 36 + t) * 9;
-        for (int i = 0; i < 9; ++i) dst[i] = out[i];
+        double* dst = Kebe + (e * 36 + blk) * 9;
+        for (int i = 0; i < 9; ++i) dst[i] = kb[i];
       }
     }
   }
-  if (t >= 40 && t < 64) {
+  if (t >= 40) {
     const int r = t - 40;
-    double acc  = s.Ru[r];
-    for (int i = 0; i < NE; ++i) acc -= s.Lm[i][r] * s.rt[i];
+    double acc  = sc.Ru[r];
+#pragma unroll
+    for (int i = 0; i < NE; ++i) acc -= sc.Lm[i][r] * sc.rt[i];
     const unsigned bits = fbits ? fbits[e] : 0u;
     if (bits) {  // Traction skill, after the EAS element's condensed internal forces
       const int k = r % 3;
