@@ -109,22 +109,33 @@ __host__ __device__ constexpr int v6(int i, int j) {
   return i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5));
 }
 
-// One 64-lane workgroup (one wave) per element; the LDS holds the Gauss-point data during the point loop and the
-// condensation operands afterwards (a union: ~10 KB, so the VGPR budget sets the occupancy).
+// One 64-lane workgroup (one wave) per element. The LDS (13.6 KB; 11.2 KB for the residual alone) holds two
+// unions: the K_uu inputs (later the parked K_uu blocks and the elimination's pivot column) and the point data of
+// the D / L / Rt loop (later the condensation operands).
+template <bool WANT_K>
 struct EasLds {
+  static constexpr int NK = WANT_K ? 8 : 1;  // the residual-only kernel keeps no K_uu inputs
   double X[8][3], U[8][3];
   double alpha[NE];
   double T0[6][6];  // (T(xi0) detJ0)^-1
   union {
     struct {
       double G[8][8][3];  // [gp][node][dir] shape-function gradients g_a
-      double P[8][8][3];  // F Ci g_a
-      double F[8][9];
       double S[8][6];     // S dV
+      double P[NK][8][3]; // F Ci g_a
+      double Ci[NK][6];   // Ci (Voigt order), the identity for SVK
+      double Phi[NK][6];  // F Ci F^T (Voigt order)
+      double w[NK][2];    // lambda dV, c2 dV: material tangent weights of the Ci form
+    } k;
+    struct {
+      double Kuu[WANT_K ? 36 : 1][9];  // K_uu upper blocks while the elimination runs
+      double piv[NE];     // pivot column of the elimination step
+    } q;
+  } uk;
+  union {
+    struct {
+      double F[8][9];
       double C[8][21];    // C dV (packed)
-      double Ci[8][6];    // Ci (Voigt order), the identity for SVK
-      double Phi[8][6];   // F Ci F^T (Voigt order)
-      double w[8][2];     // lambda dV, c2 dV: material tangent weights of the Ci form
       double f[8][NE];    // E21 mode scalings f_c(xi_gp) / detJ_gp: column c of M = T0^-1[:, kRow[c]] f_c
       double CT[6][6];    // current point: C dV T0^-1
       double Ch[6][6];    // T0^-T C dV T0^-1
@@ -136,9 +147,7 @@ struct EasLds {
       double A[NE][24];   // D (columns 0-20), then Y = D^-1 L
       double Lm[NE][24];  // L
       double rt[NE];      // Rt, then y = D^-1 Rt
-      double piv[NE];     // pivot column of the elimination step
       double Ru[24];
-      double Kuu[36][9];  // K_uu upper blocks while the elimination runs
     } c;
   } u;
   int bad;
@@ -220,7 +229,9 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
                                                  double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
                                                  double3 trac, double* __restrict__ cache,
                                                  unsigned long long* __restrict__ err) {
-  __shared__ EasLds s;
+  __shared__ EasLds<WANT_K> s;
+  auto& sk        = s.uk.k;
+  auto& sq        = s.uk.q;
   auto& sg        = s.u.g;
   auto& sc        = s.u.c;
   const int t     = threadIdx.x;
@@ -325,30 +336,33 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
       s.bad = 1;
     }
     const double dV = fabs(detJ) * 0.125;
-    double q[3], p[3];
-    symMul(Ci6, go, q);
-    for (int i = 0; i < 3; ++i) p[i] = F[i][0] * q[0] + F[i][1] * q[1] + F[i][2] * q[2];
     double sgo[3];
     symMul(Sv, go, sgo);
     for (int i = 0; i < 3; ++i) {
-      sg.G[gp][na][i]  = go[i];
-      sg.P[gp][na][i]  = p[i];
+      sk.G[gp][na][i]  = go[i];
       sg.ru[gp][na][i] = (F[i][0] * sgo[0] + F[i][1] * sgo[1] + F[i][2] * sgo[2]) * dV;
+    }
+    if constexpr (WANT_K) {
+      double q[3];
+      symMul(Ci6, go, q);
+      for (int i = 0; i < 3; ++i) sk.P[gp][na][i] = F[i][0] * q[0] + F[i][1] * q[1] + F[i][2] * q[2];
     }
     if (na == 0) {
       for (int i = 0; i < 9; ++i) sg.F[gp][i] = F[i / 3][i % 3];
-      for (int v = 0; v < 6; ++v) sg.S[gp][v] = Sv[v] * dV;
+      for (int v = 0; v < 6; ++v) sk.S[gp][v] = Sv[v] * dV;
       for (int i = 0; i < 21; ++i) sg.C[gp][i] = Cp[i] * dV;
-      for (int v = 0; v < 6; ++v) sg.Ci[gp][v] = Ci6[v];
-      // Phi = F Ci F^T
-      double FC[3][3];
-      for (int i = 0; i < 3; ++i) symMul(Ci6, F[i], FC[i]);  // row i of F Ci (Ci symmetric)
-      for (int v = 0; v < 6; ++v) {
-        const int k = vI(v), l = vJ(v);
-        sg.Phi[gp][v] = FC[k][0] * F[l][0] + FC[k][1] * F[l][1] + FC[k][2] * F[l][2];
+      if constexpr (WANT_K) {
+        for (int v = 0; v < 6; ++v) sk.Ci[gp][v] = Ci6[v];
+        // Phi = F Ci F^T
+        double FC[3][3];
+        for (int i = 0; i < 3; ++i) symMul(Ci6, F[i], FC[i]);  // row i of F Ci (Ci symmetric)
+        for (int v = 0; v < 6; ++v) {
+          const int k = vI(v), l = vJ(v);
+          sk.Phi[gp][v] = FC[k][0] * F[l][0] + FC[k][1] * F[l][1] + FC[k][2] * F[l][2];
+        }
+        sk.w[gp][0] = lam * dV;
+        sk.w[gp][1] = c2 * dV;
       }
-      sg.w[gp][0] = lam * dV;
-      sg.w[gp][1] = c2 * dV;
 #pragma unroll
       for (int c = 0; c < NE; ++c) sg.f[gp][c] = ansatz(c, x, y, z) * sc1;
     }
@@ -356,10 +370,9 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   __syncthreads();
   if (s.bad) return;  // uniform: the error flag is already set
 
-  // (2) point loop. Lane (ka, kb) accumulates K_uu block (ka, kb); D upper entries t + 64 j (< 231), L entries
-  // t + 64 j (< 504), Rt (lanes 0-20) and R_u (lanes 40-63) are owned by fixed lanes.
+  // (2a) point loop for D, L and Rt: D upper entries t + 64 j (< 231), L columns and Rt (lanes 0-20) are owned by
+  // fixed lanes; R_u (lanes 40-63) is summed from the per-point contributions afterwards
   const int ka = t >> 3, kbn = t & 7;
-  double kb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double dacc[4] = {0, 0, 0, 0};
   double racc    = 0.0;
   int dI[4], dC[4], dKi[4], dKc[4];
@@ -381,38 +394,13 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
       for (int u = 0; u < 6; ++u) acc += sg.C[g][sym6(v, u)] * s.T0[u][w];
       sg.CT[v][w] = acc;
     }
-    if constexpr (WANT_K) {
-      double pa[3], pb[3], ga[3], gb[3], Ci6[6], S6[6], Ph[6];
-      for (int i = 0; i < 3; ++i) {
-        pa[i] = sg.P[g][ka][i];
-        pb[i] = sg.P[g][kbn][i];
-        ga[i] = sg.G[g][ka][i];
-        gb[i] = sg.G[g][kbn][i];
-      }
-      for (int v = 0; v < 6; ++v) {
-        Ci6[v] = sg.Ci[g][v];
-        S6[v]  = sg.S[g][v];
-        Ph[v]  = sg.Phi[g][v];
-      }
-      const double wl = sg.w[g][0], wc = sg.w[g][1];
-      double qb[3], sb[3];
-      symMul(Ci6, gb, qb);
-      symMul(S6, gb, sb);
-      const double gCg = ga[0] * qb[0] + ga[1] * qb[1] + ga[2] * qb[2];
-      const double gSg = ga[0] * sb[0] + ga[1] * sb[1] + ga[2] * sb[2];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-          kb[3 * k + l] += wl * pa[k] * pb[l] + wc * (pa[l] * pb[k] + gCg * Ph[v6(k, l)]) + (k == l ? gSg : 0.0);
-    }
     __syncthreads();
     // Z[w][3b+l] = (F Q^w g_b)_l (lane (w, b) < 48), Ch = T0^-T CT (lanes 48-63, then 43-47), TS (lanes 37-42)
     if (t < 48) {
       const int w = t >> 3, b = t & 7;
       double Q[6], gbv[3], h[3];
       for (int v = 0; v < 6; ++v) Q[v] = sg.CT[v][w];
-      for (int i = 0; i < 3; ++i) gbv[i] = sg.G[g][b][i];
+      for (int i = 0; i < 3; ++i) gbv[i] = sk.G[g][b][i];
       symMul(Q, gbv, h);
 #pragma unroll
       for (int l = 0; l < 3; ++l)
@@ -431,7 +419,7 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
       const int w = t - 37;
       double acc = 0.0;
 #pragma unroll
-      for (int v = 0; v < 6; ++v) acc += s.T0[v][w] * sg.S[g][v];
+      for (int v = 0; v < 6; ++v) acc += s.T0[v][w] * sk.S[g][v];
       sg.TS[w] = acc;
     }
     __syncthreads();
@@ -472,10 +460,39 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   }
   if (t < NE) sc.rt[t] = racc;
   if (t >= 40) sc.Ru[t - 40] = racc;
+  // (2b) K_uu, lane (ka, kb) per 3x3 block (the K_uu inputs are untouched by the spill above)
   if constexpr (WANT_K) {
+    double kb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 2  // fully unrolled, the hoisted loads would take every VGPR
+    for (int g = 0; g < 8; ++g) {
+      double pa[3], pb[3], ga[3], gb[3], Ci6[6], S6[6], Ph[6];
+      for (int i = 0; i < 3; ++i) {
+        pa[i] = sk.P[g][ka][i];
+        pb[i] = sk.P[g][kbn][i];
+        ga[i] = sk.G[g][ka][i];
+        gb[i] = sk.G[g][kbn][i];
+      }
+      for (int v = 0; v < 6; ++v) {
+        Ci6[v] = sk.Ci[g][v];
+        S6[v]  = sk.S[g][v];
+        Ph[v]  = sk.Phi[g][v];
+      }
+      const double wl = sk.w[g][0], wc = sk.w[g][1];
+      double qb[3], sb[3];
+      symMul(Ci6, gb, qb);
+      symMul(S6, gb, sb);
+      const double gCg = ga[0] * qb[0] + ga[1] * qb[1] + ga[2] * qb[2];
+      const double gSg = ga[0] * sb[0] + ga[1] * sb[1] + ga[2] * sb[2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          kb[3 * k + l] += wl * pa[k] * pb[l] + wc * (pa[l] * pb[k] + gCg * Ph[v6(k, l)]) + (k == l ? gSg : 0.0);
+    }
+    __syncthreads();  // the K_uu inputs are dead: park the blocks in their place
     if (ka <= kbn)
 #pragma unroll
-      for (int i = 0; i < 9; ++i) sc.Kuu[upperBlock(ka, kbn)][i] = kb[i];
+      for (int i = 0; i < 9; ++i) sq.Kuu[upperBlock(ka, kbn)][i] = kb[i];
   }
   __syncthreads();
 
@@ -490,12 +507,12 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   for (int j = 0; j < NE; ++j) {
     if (t == j)
 #pragma unroll
-      for (int i = 0; i < NE; ++i) sc.piv[i] = col[i];
+      for (int i = 0; i < NE; ++i) sq.piv[i] = col[i];
     __syncthreads();
-    const double f = col[j] / sc.piv[j];
+    const double f = col[j] / sq.piv[j];
 #pragma unroll
     for (int i = 0; i < NE; ++i)
-      if (i != j) col[i] -= sc.piv[i] * f;
+      if (i != j) col[i] -= sq.piv[i] * f;
     col[j] = f;
     __syncthreads();
   }
@@ -520,8 +537,9 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   // upper triangle (enhancedassumedstrains.hh:291-296)
   if constexpr (WANT_K) {
     const int blk = upperBlock(ka <= kbn ? ka : kbn, ka <= kbn ? kbn : ka);
+    double kb[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) kb[i] = sc.Kuu[blk][i];
+    for (int i = 0; i < 9; ++i) kb[i] = sq.Kuu[blk][i];
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
       double la[3], yb[3];
