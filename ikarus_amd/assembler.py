@@ -109,6 +109,12 @@ class Device:
         u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
         self.check(capi.lib().ik_set_requirement(self.h, ptr(u), float(lam), capi.HOST))
 
+    def set_dirichlet_values(self, ubar):
+        """Inhomogeneous Dirichlet values u_D(lambda) = lambda * ubar on constrained DOFs
+        (evaluateInhomogeneousBoundaryCondition, dirichletvalues.hh:214-257); None clears them."""
+        self._ubar = None if ubar is None else np.ascontiguousarray(ubar, dtype=np.float64)
+        self.check(capi.lib().ik_set_dirichlet_values(self.h, ptr(self._ubar), capi.HOST))
+
     def get_requirement(self):
         u = np.zeros(self.ndof)
         lam = C.c_double()

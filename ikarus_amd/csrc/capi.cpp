@@ -49,6 +49,20 @@ __global__ void historyKernel(const double* __restrict__ u, double* __restrict__
   }
 }
 
+// u[d] = lambda ubar[d] on the constrained DOFs where that value is nonzero: the update with the inhomogeneous
+// boundary values (nonlinearsolverfactory.hh:45-54, dirichletvalues.hh:214-257)
+__global__ void syncIdbcKernel(const int32_t* __restrict__ fl, int64_t nf, const double* __restrict__ ubar,
+                               double lambda, double* __restrict__ u) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nf; i += (int64_t)gridDim.x * 256) {
+    const int32_t d = fl[i];
+    const double v  = ubar[d] * lambda;
+    if (v != 0.0) u[d] = v;
+  }
+}
+__global__ void zeroListKernel(const int32_t* __restrict__ fl, int64_t nf, double* __restrict__ f) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nf; i += (int64_t)gridDim.x * 256) f[fl[i]] = 0.0;
+}
+
 inline int grid(int64_t n) {
   int64_t g = (n + 255) / 256;
   return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 4096);
@@ -87,6 +101,12 @@ void assembleK(ik_handle* h, int dbc, int precision) {
   h->kValid     = true;
 }
 
+void syncIdbc(ik_handle* h) {
+  if (!h->hasUbar || h->nfixed == 0) return;
+  hipLaunchKernelGGL(syncIdbcKernel, dim3(grid(h->nfixed)), dim3(256), 0, h->stream, h->fixedList.p, h->nfixed,
+                     h->ubar.p, h->lambda, h->u.p);
+}
+
 void linearSolve(ik_handle* h, const double* rhs, double* x, const ik_solver_opts* o, double rtol,
                  ik_solver_info* info) {
   require(h->kValid && h->kDbc == IK_DBC_FULL, IK_EINVAL, "ik_solve needs a FULL-DBC matrix assembled first");
@@ -110,14 +130,24 @@ void linearSolve(ik_handle* h, const double* rhs, double* x, const ik_solver_opt
 }
 
 void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_info* info) {
-  (void)stepSize;  // inhomogeneous Dirichlet forces are not part of the device path
   requireBound(h);
   ensurePattern(h);
   const int64_t n = h->ndof;
   h->du.ensure(n);
-  *info         = ik_nr_info{};
-  info->success = 1;
+  *info              = ik_nr_info{};
+  info->success      = 1;
+  const bool idbcF   = h->hasUbar && h->nfixed > 0 && stepSize != 0.0;
+  if (idbcF) {
+    // force of the inhomogeneous boundary values, K_raw(x) ubar with the constrained entries zeroed, added to
+    // the first residual scaled by the step (newtonraphson.hh:215-218; functionhelper.hh:170-184)
+    assembleK(h, IK_DBC_RAW, IK_FP64);
+    h->idbcF.ensure(n);
+    ik::spmv64(h, h->ubar.p, h->idbcF.p);
+    hipLaunchKernelGGL(zeroListKernel, dim3(grid(h->nfixed)), dim3(256), 0, h->stream, h->fixedList.p, h->nfixed,
+                       h->idbcF.p);
+  }
   assembleR(h, IK_DBC_FULL);
+  if (idbcF) ik::axpy(h, stepSize, h->idbcF.p, h->R.p, n);
   double rNorm        = normR(h);
   double rPrev        = rNorm;
   double etaPrev      = 0.0;
@@ -151,6 +181,7 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     info->correction_norm = std::sqrt(ik::dot(h, h->du.p, h->du.p, n));
     if (h->neas) ik::easUpdate(h, h->du.p);  // CORRECTION_UPDATED (newtonraphson.hh:230)
     ik::axpy(h, 1.0, h->du.p, h->u.p, n);
+    syncIdbc(h);  // syncParameterAndGlobalSolution (newtonraphson.hh:236-237)
     h->kValid        = false;
     h->easCacheValid = false;
     assembleR(h, IK_DBC_FULL);
@@ -280,6 +311,25 @@ int ik_set_requirement(ik_handle* h, const double* u, double lambda, int where) 
     h->kValid        = false;
     h->easCacheValid = false;
     IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int ik_set_dirichlet_values(ik_handle* h, const double* ubar, int where) {
+  return guard(h, [&] {
+    if (!ubar) {
+      h->hasUbar = false;
+      return;
+    }
+    std::vector<double> hb(h->ndof);
+    if (where == IK_DEVICE)
+      IK_HIP(hipMemcpy(hb.data(), ubar, h->ndof * sizeof(double), hipMemcpyDeviceToHost));
+    else
+      std::copy(ubar, ubar + h->ndof, hb.begin());
+    for (int64_t i = 0; i < h->ndof; ++i)
+      require(hb[i] == 0.0 || h->hfixed[i], IK_EINVAL, "inhomogeneous Dirichlet value on an unconstrained DOF");
+    h->ubar.upload(hb.data(), hb.size(), h->stream);
+    IK_HIP(hipStreamSynchronize(h->stream));
+    h->hasUbar = true;
   });
 }
 
@@ -525,6 +575,7 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
       h->kValid        = false;
       h->easCacheValid = false;
       newtonSolve(h, s, dl, &ni);
+      if (ni.iterations == 0) syncIdbc(h);  // a step that needed no correction still takes the new values
       if (iterations) iterations[ls + 1] = ni.iterations;
       if (residuals) residuals[ls + 1] = ni.residual_norm;
       info->solves += 1;

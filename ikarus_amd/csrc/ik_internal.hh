@@ -108,6 +108,9 @@ struct ik_handle {
   ik::DevBuf<int32_t> conn;
   ik::DevBuf<uint8_t> fixed;
   ik::DevBuf<int32_t> fixedList;  // constrained DOF indices
+  // inhomogeneous Dirichlet values u_D(lambda) = lambda * ubar on constrained DOFs (none unless set)
+  bool hasUbar = false;
+  ik::DevBuf<double> ubar, idbcF;
   // loads
   int64_t nfaces = 0;
   ik::DevBuf<uint8_t> faceBits;  // per element: bit 2*axis + side set for each loaded face (empty: none)

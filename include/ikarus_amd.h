@@ -85,6 +85,13 @@ int64_t ik_reduced_size(const ik_handle* h);
 int ik_set_requirement(ik_handle* h, const double* u, double lambda, int where);
 int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where);
 
+/* Inhomogeneous Dirichlet values u_D(lambda) = lambda * ubar (DirichletValues::
+ * evaluateInhomogeneousBoundaryCondition, dirichletvalues.hh:214-257; ubar is nonzero on constrained DOFs
+ * only, else IK_EINVAL). Newton then adds step_size * K_raw(u) ubar (constrained entries zeroed) to its first
+ * residual (newtonraphson.hh:215-218, utils::obtainForcesDueToIDBC, functionhelper.hh:170-184) and sets the
+ * constrained DOFs to lambda * ubar after every correction (nonlinearsolverfactory.hh:45-54). NULL clears. */
+int ik_set_dirichlet_values(ik_handle* h, const double* ubar, int where);
+
 /* VectorAssembler::vector(req, forces, dbc) (interface.hh:355-365; simpleassemblers.inl:60-118).
  * R has size() entries (RAW/FULL) or reducedSize() entries (REDUCED); R = NULL assembles into the
  * handle's device residual without copying it out. */
