@@ -14,8 +14,22 @@
 namespace ik {
 namespace {
 
-constexpr int SPMV_BS   = 256;            // threads per block
-constexpr int ROWS_PER_BLOCK = (SPMV_BS / 64) * 7;  // 7 block rows per wavefront
+constexpr int SPMV_BS = 256;  // threads per block
+
+// Block rows per wavefront of the SpMV lane layouts: FP64 values 9 lanes per row (7 rows), FP32 values
+// 3 lanes per row (21 rows); lane 63 idles in both.
+template <typename TV>
+constexpr int spmvRowsPerWave() {
+  return sizeof(TV) == 4 ? 21 : 7;
+}
+template <typename TV>
+constexpr int spmvRowsPerBlock() {
+  return (SPMV_BS / 64) * spmvRowsPerWave<TV>();
+}
+template <typename TV>
+inline unsigned spmvGrid(int64_t nrows) {
+  return (unsigned)((nrows + spmvRowsPerBlock<TV>() - 1) / spmvRowsPerBlock<TV>());
+}
 
 __device__ inline double waveSum(double v) {
 #pragma unroll
@@ -39,14 +53,17 @@ __device__ inline double blockSum(double v) {
   return s;
 }
 
-// y = K x over 3x3 block rows; optional fused partial dot d = sum_i x_i y_i per block.
-// Lane (r, m) of a wavefront: row r = lane / 9 (7 rows per wavefront, lane 63 idle), entry m = 3k + l of
-// every 3x3 block in the row. Each load instruction reads 7 contiguous 72-byte block runs; the loads of
-// 9 blocks (a third of an interior 27-block row) are issued before their products are summed, which keeps
-// enough bytes in flight per lane to stream K near the HBM rate; the three partial sums of row k are
-// combined with two lane shifts.
-constexpr int ROWS_PER_WAVE = 7;
-constexpr int SPMV_UNROLL   = 9;
+// y = K x over 3x3 block rows; optional fused partial dot d = sum_i x_i y_i per block. Two lane layouts:
+//  FP64 values: lane (r, m), row r = lane / 9 (7 rows per wavefront), entry m = 3k + l of every block; each
+//   load instruction reads 7 contiguous 72-byte block runs; the three partial sums of row k are combined
+//   with two lane shifts.
+//  FP32 values: lane (r, k), row r = lane / 3 (21 rows per wavefront), row k of every block (3 contiguous
+//   values) against the 3 components of x at the block's column: three times the bytes per load
+//   instruction, which one value of 4 bytes per load could not stream (64^3 pattern: 5.5 vs 3.9 TB/s; for
+//   FP64 values the two layouts measured within 3 % in isolation and 135.6 vs 153.3 us inside the PCG).
+// Both issue the loads of 9 blocks (a third of an interior 27-block row) before summing their products;
+// 9 blocks in flight beat 3, 4, 6, 12, 14 and 18.
+constexpr int SPMV_UNROLL = 9;
 
 // acc += sum over the row's blocks of V[j*9] * x[C[j]*3 + l] (in block order)
 template <typename TV, typename TX, typename TA>
@@ -68,6 +85,73 @@ __device__ inline TA rowDot(const TV* __restrict__ V, const int32_t* __restrict_
   for (; j < n; ++j) acc += (TA)V[(int64_t)j * 9] * (TA)x[(int64_t)C[j] * 3 + l];
   return acc;
 }
+
+// sum over the row's blocks of V[j*9 + 0..2] . x[C[j]*3 + 0..2] (in block order)
+template <typename TV, typename TX, typename TA>
+__device__ inline TA rowDot3(const TV* __restrict__ V, const int32_t* __restrict__ C, int n,
+                             const TX* __restrict__ x) {
+  TA acc = TA(0);
+  int j  = 0;
+  for (; j + SPMV_UNROLL <= n; j += SPMV_UNROLL) {
+    int32_t c[SPMV_UNROLL];
+    TV v[SPMV_UNROLL][3];
+    TX xv[SPMV_UNROLL][3];
+#pragma unroll
+    for (int u = 0; u < SPMV_UNROLL; ++u) {
+      c[u] = C[j + u];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) v[u][l] = V[(int64_t)(j + u) * 9 + l];
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_UNROLL; ++u)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) xv[u][l] = x[(int64_t)c[u] * 3 + l];
+#pragma unroll
+    for (int u = 0; u < SPMV_UNROLL; ++u)
+      acc += (TA)v[u][0] * (TA)xv[u][0] + (TA)v[u][1] * (TA)xv[u][1] + (TA)v[u][2] * (TA)xv[u][2];
+  }
+  for (; j < n; ++j) {
+    const int64_t cc = (int64_t)C[j] * 3;
+    acc += (TA)V[(int64_t)j * 9] * (TA)x[cc] + (TA)V[(int64_t)j * 9 + 1] * (TA)x[cc + 1] +
+           (TA)V[(int64_t)j * 9 + 2] * (TA)x[cc + 2];
+  }
+  return acc;
+}
+
+// (K x)[3 row + k] accumulated in TA, for the lane layout of the value type; valid on lanes with `out` set
+template <typename TV, typename TX, typename TA>
+__device__ inline TA spmvRow(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                             const TV* __restrict__ val, const TX* __restrict__ x, int64_t nrows, int64_t& row,
+                             int& k, bool& out) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  TA acc = TA(0);
+  if constexpr (sizeof(TV) == 4) {
+    const int rl = lane / 3;
+    k            = lane - 3 * rl;
+    row          = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * spmvRowsPerWave<TV>() + rl;
+    out          = rl < spmvRowsPerWave<TV>() && row < nrows;
+    if (out) {
+      const int32_t rb = rowptr[row], re = rowptr[row + 1];
+      acc = rowDot3<TV, TX, TA>(val + (int64_t)rb * 9 + 3 * k, colidx + rb, re - rb, x);
+    }
+  } else {
+    const int rl = lane / 9;
+    const int m  = lane - rl * 9;
+    k            = m / 3;
+    const int l  = m - 3 * k;
+    row          = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * spmvRowsPerWave<TV>() + rl;
+    const bool active = rl < spmvRowsPerWave<TV>() && row < nrows;
+    if (active) {
+      const int32_t rb = rowptr[row], re = rowptr[row + 1];
+      acc = rowDot<TV, TX, TA>(val + (int64_t)rb * 9 + m, colidx + rb, re - rb, x, l);
+    }
+    const TA a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
+    acc += a1 + a2;
+    out = active && l == 0;
+  }
+  return acc;
+}
+
 template <typename TV, typename TX, bool DOT>
 __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict__ rowptr,
                                                       const int32_t* __restrict__ colidx,
@@ -76,21 +160,12 @@ __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict_
                                                       double* __restrict__ partial,
                                                       const PcgScalars* __restrict__ sc) {
   if (sc && sc->done) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int rl   = lane / 9;
-  const int m    = lane - rl * 9;
-  const int k = m / 3, l = m - 3 * (m / 3);
-  const int64_t row = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * ROWS_PER_WAVE + rl;
-  const bool active = rl < ROWS_PER_WAVE && row < nrows;
-  TX acc = TX(0);
-  if (active) {
-    const int32_t rb = rowptr[row], re = rowptr[row + 1];
-    acc = rowDot<TV, TX, TX>(val + (int64_t)rb * 9 + m, colidx + rb, re - rb, x, l);
-  }
-  const TX a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
-  double d = 0.0;
-  if (active && l == 0) {
-    const TX yk = acc + a1 + a2;
+  int64_t row;
+  int k;
+  bool out;
+  const TX yk = spmvRow<TV, TX, TX>(rowptr, colidx, val, x, nrows, row, k, out);
+  double d    = 0.0;
+  if (out) {
     y[row * 3 + k] = yk;
     if (DOT) d = (double)x[row * 3 + k] * (double)yk;
   }
@@ -100,7 +175,7 @@ __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict_
   }
 }
 
-// r = b - K x with FP64 accumulation (K may be FP32); used by iterative refinement. Same lane mapping.
+// r = b - K x with FP64 accumulation (K may be FP32); used by iterative refinement. Same lane layouts.
 template <typename TV>
 __global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restrict__ rowptr,
                                                           const int32_t* __restrict__ colidx,
@@ -108,21 +183,13 @@ __global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restr
                                                           const double* __restrict__ x,
                                                           const double* __restrict__ b, double* __restrict__ r,
                                                           int64_t nrows, double* __restrict__ partial) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int rl   = lane / 9;
-  const int m    = lane - rl * 9;
-  const int k = m / 3, l = m - 3 * (m / 3);
-  const int64_t row = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * ROWS_PER_WAVE + rl;
-  const bool active = rl < ROWS_PER_WAVE && row < nrows;
-  double acc = 0.0;
-  if (active) {
-    const int32_t rb = rowptr[row], re = rowptr[row + 1];
-    acc = rowDot<TV, double, double>(val + (int64_t)rb * 9 + m, colidx + rb, re - rb, x, l);
-  }
-  const double a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
-  double d = 0.0;
-  if (active && l == 0) {
-    const double rk = b[row * 3 + k] - (acc + a1 + a2);
+  int64_t row;
+  int k;
+  bool out;
+  const double kx = spmvRow<TV, double, double>(rowptr, colidx, val, x, nrows, row, k, out);
+  double d        = 0.0;
+  if (out) {
+    const double rk = b[row * 3 + k] - kx;
     r[row * 3 + k]  = rk;
     d               = rk * rk;
   }
@@ -412,7 +479,7 @@ static double* pq2(ik_handle* h) { return h->partial.p + h->partial.n - 8 - kPqB
 // region of their own before the last 8 entries, and fold results in its last entry
 static void ensureWork(ik_handle* h) {
   const int64_t n = h->ndof;
-  const int64_t spmvBlocks = (h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+  const int64_t spmvBlocks = spmvGrid<double>(h->nn);  // the FP64 layout launches the most blocks
   h->partial.ensure((size_t)std::max<int64_t>(spmvBlocks, 2 * kReduceBlocks) + kPqBlocks + 8);
   if (!h->sc.p) {
     h->sc.ensure(1);
@@ -424,7 +491,7 @@ static void ensureWork(ik_handle* h) {
 
 void spmv64(ik_handle* h, const double* x, double* y) {
   ensureWork(h);
-  const unsigned grid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  const unsigned grid = spmvGrid<double>(h->nn);
   ScopedKernelTimer t(h, KSPMV);
   hipLaunchKernelGGL((spmvKernel<double, double, false>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
                      h->colidx.p, h->K64.p, x, y, h->nn, nullptr, nullptr);
@@ -432,7 +499,7 @@ void spmv64(ik_handle* h, const double* x, double* y) {
 
 void spmv32(ik_handle* h, const float* x, float* y) {
   ensureWork(h);
-  const unsigned grid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  const unsigned grid = spmvGrid<float>(h->nn);
   ScopedKernelTimer t(h, KSPMV);
   hipLaunchKernelGGL((spmvKernel<float, float, false>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
                      h->colidx.p, h->K32.p, x, y, h->nn, nullptr, nullptr);
@@ -454,7 +521,7 @@ void axpy(ik_handle* h, double a, const double* x, double* y, int64_t n) {
 
 void residualBlock64(ik_handle* h, const double* b, const double* x, double* r) {
   ensureWork(h);
-  const unsigned grid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  const unsigned grid = spmvGrid<double>(h->nn);
   hipLaunchKernelGGL((residualKernel<double>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p, h->colidx.p,
                      h->K64.p, x, b, r, h->nn, h->partial.p);
 }
@@ -470,7 +537,7 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
   h->pq.ensure(n);
   h->pdiag.ensure(n);
   const int g = vecGrid(n);
-  const unsigned sgrid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  const unsigned sgrid = spmvGrid<double>(h->nn);
   hipLaunchKernelGGL(jacobiDiagKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->diagBlk.p,
                      h->K64.p, h->nn, h->pdiag.p);
   hipLaunchKernelGGL(pcgInitKernel, dim3(g), dim3(256), 0, h->stream, b, h->pdiag.p, x, h->pr.p, h->pp.p, n,
@@ -511,7 +578,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
   h->pr.ensure(n);
   const int g = vecGrid(n);
   const int gn = vecGrid(h->nn);
-  const unsigned sgrid = (unsigned)((h->nn + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  const unsigned sgrid = spmvGrid<float>(h->nn);
   hipLaunchKernelGGL(blockInvKernel, dim3((unsigned)((h->nn + 255) / 256)), dim3(256), 0, h->stream, h->diagBlk.p,
                      h->K32.p, h->nn, h->fdiag.p);
   // x = 0, r = b
