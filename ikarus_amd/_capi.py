@@ -54,7 +54,7 @@ class SolverInfo(C.Structure):
 
 class NRSettings(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("min_iter", C.c_int32), ("precision", C.c_int32),
-                ("linear", SolverOpts), ("forcing", C.c_double)]
+                ("linear", SolverOpts), ("forcing", C.c_double), ("eta_max", C.c_double)]
 
 
 class NRInfo(C.Structure):

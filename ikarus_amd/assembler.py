@@ -213,11 +213,11 @@ def solver_opts(kind=capi.PCG_JACOBI_FP64, rtol=1e-10, max_iter=-1, inner_rtol=1
     return o
 
 
-def nr_settings(tol=1e-8, max_iter=20, min_iter=0, precision=capi.FP64, linear=None, forcing=0.0):
+def nr_settings(tol=1e-8, max_iter=20, min_iter=0, precision=capi.FP64, linear=None, forcing=0.0, eta_max=0.0):
     s = capi.NRSettings()
     s.tol, s.max_iter, s.min_iter, s.precision = tol, max_iter, min_iter, precision
     s.linear = linear if linear is not None else solver_opts()
-    s.forcing = forcing
+    s.forcing, s.eta_max = forcing, eta_max
     return s
 
 

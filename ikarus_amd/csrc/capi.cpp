@@ -159,9 +159,9 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     // K x = R ; correction = -x (solve(correction, -rx), newtonraphson.hh:222-223)
     double rtol = s->linear.rtol;
     if (s->forcing > 0.0) {
-      // inexact Newton, Eisenstat-Walker choice 2 (gamma = forcing, eta_max = 0.1) with the safeguard
+      // inexact Newton, Eisenstat-Walker choice 2 (gamma = forcing, eta <= eta_max) with the safeguard
       // eta >= gamma eta_prev^2 and a floor: no need to solve below half the absolute Newton tolerance
-      constexpr double etaMax = 0.1;
+      const double etaMax = s->eta_max > 0.0 ? s->eta_max : 0.1;
       double eta = etaMax;
       if (iter > 0) {
         eta = s->forcing * (rNorm / rPrev) * (rNorm / rPrev);

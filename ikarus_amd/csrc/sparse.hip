@@ -597,11 +597,13 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
   double rnorm      = bnorm;
   const int maxRef  = o->max_refine > 0 ? o->max_refine : 10;
   for (int sweep = 0; sweep < maxRef; ++sweep) {
-    // inner FP32 PCG on K32 d = r / ||r||
+    // inner FP32 PCG on K32 d = r / ||r||, to half the reduction the outer tolerance still needs (an inexact
+    // Newton step asks for little), but never below inner_rtol
+    const double innerTol = std::max(o->inner_rtol, 0.5 * o->rtol * bnorm / rnorm);
     hipLaunchKernelGGL(pcgInitF32, dim3(gn), dim3(256), 0, h->stream, h->pr.p, 1.0 / rnorm, h->fdiag.p, h->fx.p,
                        h->fr.p, h->fp.p, h->nn, h->partial.p);
-    hipLaunchKernelGGL(pcgInitFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, gn,
-                       o->inner_rtol * o->inner_rtol, h->sc.p);
+    hipLaunchKernelGGL(pcgInitFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, gn, innerTol * innerTol,
+                       h->sc.p);
     int enq = 0;
     for (;;) {
       for (int c = 0; c < kChunk && enq < maxIter; ++c, ++enq) {

@@ -133,7 +133,8 @@ typedef struct {
   int32_t kind;        /* IK_PCG_JACOBI_FP64 | IK_PCG_BJACOBI_FP32_IR                       */
   double rtol;         /* ||r|| <= rtol ||b||   (Eigen ConjugateGradient semantics)         */
   int32_t max_iter;    /* <= 0: 2n (Eigen default)                                         */
-  double inner_rtol;   /* FP32 inner PCG tolerance per refinement sweep (IR form)          */
+  double inner_rtol;   /* IR form: an FP32 sweep stops at half the relative reduction the
+                          outer rtol still needs, or at inner_rtol if that is larger      */
   int32_t max_refine;  /* maximum FP64 refinement sweeps (IR form)                         */
 } ik_solver_opts;
 
@@ -156,8 +157,9 @@ typedef struct {
   int32_t precision;     /* tangent precision IK_FP64 | IK_FP32                             */
   ik_solver_opts linear;
   double forcing;        /* > 0: inexact Newton, Eisenstat-Walker choice 2 with gamma =
-                            forcing, eta_max = 0.1, floor max(linear.rtol, tol/(2||R_k||));
+                            forcing, eta <= eta_max, floor max(linear.rtol, tol/(2||R_k||));
                             0: every linear solve to linear.rtol                        */
+  double eta_max;        /* upper bound of the forcing term (0: 0.1)                    */
 } ik_nr_settings;
 
 typedef struct {
