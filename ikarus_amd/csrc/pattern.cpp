@@ -10,10 +10,52 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <exception>
+#include <mutex>
+#include <thread>
 
 #include "ik_internal.hh"
 
 namespace ik {
+
+// Host threads of the build: ranges of nodes, elements or tiles whose outputs sit at positions fixed
+// before the threads start (variable-length outputs go to per-chunk buffers placed by a prefix sum after
+// the join), so every array is the same for any thread count. IK_PATTERN_THREADS sets the count (1:
+// serial); the default is the host's hardware threads, at most 16.
+static int patternThreads() {
+  if (const char* v = std::getenv("IK_PATTERN_THREADS")) {
+    const int t = std::atoi(v);
+    if (t >= 1) return std::min(t, 64);
+  }
+  const unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc, 16u));
+}
+static int64_t chunkCount(int64_t n) { return std::min<int64_t>(patternThreads(), std::max<int64_t>(1, n / 2048)); }
+
+// f(lo, hi, chunk) over chunk = 0 .. chunkCount(n) - 1; the first exception thrown is rethrown here
+template <class F>
+static void parallelChunks(int64_t n, F&& f) {
+  const int64_t T = chunkCount(n);
+  if (T <= 1) {
+    f(int64_t(0), n, int64_t(0));
+    return;
+  }
+  std::vector<std::thread> th;
+  std::exception_ptr err;
+  std::mutex m;
+  th.reserve(T);
+  for (int64_t c = 0; c < T; ++c)
+    th.emplace_back([&, c] {
+      try {
+        f(n * c / T, n * (c + 1) / T, c);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(m);
+        if (!err) err = std::current_exception();
+      }
+    });
+  for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
 
 // elements per staging chunk; IKARUS_AMD_CHUNK_ELEMENTS overrides it. Splitting the 64^3 mesh into chunks
 // whose gathers overlap the next chunk's element work measured slower than one launch of each (the two
@@ -54,31 +96,35 @@ static std::vector<int32_t> mortonOrder(const ik_handle* h) {
   for (int d = 0; d < 3; ++d) {
     lo[d] = X[d];
     for (int64_t n = 1; n < nn; ++n) lo[d] = std::min(lo[d], X[n * 3 + d]);
-    for (int64_t e = 0; e < ne; ++e) {
-      double best = 0.0;
-      for (int m = 0; m < 3; ++m) {
-        double v = 0.0;  // 4 x component d of the mean edge vector along reference axis m
-        for (int a = 0; a < 8; ++a)
-          if (!((a >> m) & 1)) v += X[conn[e * 8 + (a | (1 << m))] * 3 + d] - X[conn[e * 8 + a] * 3 + d];
-        best = std::max(best, std::fabs(0.25 * v));
+    parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) {
+      for (int64_t e = e0; e < e1; ++e) {
+        double best = 0.0;
+        for (int m = 0; m < 3; ++m) {
+          double v = 0.0;  // 4 x component d of the mean edge vector along reference axis m
+          for (int a = 0; a < 8; ++a)
+            if (!((a >> m) & 1)) v += X[conn[e * 8 + (a | (1 << m))] * 3 + d] - X[conn[e * 8 + a] * 3 + d];
+          best = std::max(best, std::fabs(0.25 * v));
+        }
+        ext[e] = best;
       }
-      ext[e] = best;
-    }
+    });
     std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
     step[d] = ext[ne / 2] > 0.0 ? ext[ne / 2] : 1.0;
   }
   std::vector<std::pair<uint64_t, int32_t>> key(ne);
-  for (int64_t e = 0; e < ne; ++e) {
-    uint64_t k = 0;
-    for (int d = 0; d < 3; ++d) {
-      double c = 0.0;
-      for (int a = 0; a < 8; ++a) c += X[conn[e * 8 + a] * 3 + d];
-      const double q = std::floor((0.125 * c - lo[d]) / step[d]);
-      const uint32_t qi = q <= 0.0 ? 0u : (q >= 2097151.0 ? 2097151u : (uint32_t)q);
-      k |= spread3(qi) << d;
+  parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) {
+    for (int64_t e = e0; e < e1; ++e) {
+      uint64_t k = 0;
+      for (int d = 0; d < 3; ++d) {
+        double c = 0.0;
+        for (int a = 0; a < 8; ++a) c += X[conn[e * 8 + a] * 3 + d];
+        const double q = std::floor((0.125 * c - lo[d]) / step[d]);
+        const uint32_t qi = q <= 0.0 ? 0u : (q >= 2097151.0 ? 2097151u : (uint32_t)q);
+        k |= spread3(qi) << d;
+      }
+      key[e] = {k, (int32_t)e};
     }
-    key[e] = {k, (int32_t)e};
-  }
+  });
   std::sort(key.begin(), key.end());
   std::vector<int32_t> order(ne);
   for (int64_t i = 0; i < ne; ++i) order[i] = key[i].second;
@@ -99,30 +145,46 @@ void buildPattern(ik_handle* h) {
     for (int64_t e = 0; e < ne; ++e)
       for (int a = 0; a < 8; ++a) inc[pos[conn[e * 8 + a]]++] = e;
   }
-  // rows: union of the nodes of incident elements
+  // rows: union of the nodes of incident elements (per chunk of rows, then placed by the row counts)
   h->hrowptr.assign(nn + 1, 0);
-  std::vector<int32_t> cols;
-  cols.reserve((size_t)nn * 27);
-  std::vector<int32_t> buf;
-  for (int64_t i = 0; i < nn; ++i) {
-    buf.clear();
-    for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) {
-      const int64_t e = inc[k];
-      for (int b = 0; b < 8; ++b) buf.push_back(conn[e * 8 + b]);
+  {
+    std::vector<std::vector<int32_t>> part(chunkCount(nn));
+    std::vector<int64_t> rc(nn);
+    parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t c) {
+      std::vector<int32_t>& cols = part[c];
+      cols.reserve((size_t)(i1 - i0) * 27);
+      std::vector<int32_t> buf;
+      for (int64_t i = i0; i < i1; ++i) {
+        buf.clear();
+        for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) {
+          const int64_t e = inc[k];
+          for (int b = 0; b < 8; ++b) buf.push_back(conn[e * 8 + b]);
+        }
+        std::sort(buf.begin(), buf.end());
+        buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+        cols.insert(cols.end(), buf.begin(), buf.end());
+        rc[i] = (int64_t)buf.size();
+      }
+    });
+    int64_t tot = 0;
+    for (int64_t i = 0; i < nn; ++i) {
+      tot += rc[i];
+      require(tot < (int64_t)INT32_MAX, IK_EINVAL, "pattern too large for int32 block indices");
+      h->hrowptr[i + 1] = (int32_t)tot;
     }
-    std::sort(buf.begin(), buf.end());
-    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
-    cols.insert(cols.end(), buf.begin(), buf.end());
-    h->hrowptr[i + 1] = (int32_t)cols.size();
+    h->hcolidx.resize(tot);
+    parallelChunks(nn, [&](int64_t i0, int64_t, int64_t c) {
+      std::copy(part[c].begin(), part[c].end(), h->hcolidx.begin() + h->hrowptr[i0]);
+    });
   }
-  require(cols.size() < (size_t)INT32_MAX, IK_EINVAL, "pattern too large for int32 block indices");
-  h->hcolidx.swap(cols);
   h->nnzb = (int64_t)h->hcolidx.size();
   h->hdiag.assign(nn, -1);
-  for (int64_t i = 0; i < nn; ++i) {
-    const auto b = h->hcolidx.begin() + h->hrowptr[i], en = h->hcolidx.begin() + h->hrowptr[i + 1];
-    h->hdiag[i] = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
-  }
+  parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
+    for (int64_t i = i0; i < i1; ++i) {
+      const auto b = h->hcolidx.begin() + h->hrowptr[i], en = h->hcolidx.begin() + h->hrowptr[i + 1];
+      h->hdiag[i] = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
+    }
+  });
   // Element e computes its 36 upper blocks (a <= b) sb(a,b) = 8a - a(a-1)/2 + (b-a). Block (node_a, node_b)
   // is an upper block of K when node_a <= node_b and receives the element block as it is; otherwise the
   // upper block (node_b, node_a) receives its transpose. Staging slots are grouped by upper block and
@@ -130,46 +192,52 @@ void buildPattern(ik_handle* h) {
   // and sums them in a fixed order.
   require((int64_t)ne * 36 < (int64_t)INT32_MAX / 2, IK_EINVAL, "mesh too large for int32 staging slots");
   std::vector<int32_t> slots((size_t)ne * 36);  // block index p of each element block
-  std::vector<int32_t> kcnt(h->nnzb + 1, 0);
-  for (int64_t e = 0; e < ne; ++e)
-    for (int a = 0; a < 8; ++a)
-      for (int c = a; c < 8; ++c) {
-        const int32_t na = conn[e * 8 + a], nc = conn[e * 8 + c];
-        const int32_t r = na < nc ? na : nc, col = na < nc ? nc : na;
-        const auto b = h->hcolidx.begin() + h->hrowptr[r], en = h->hcolidx.begin() + h->hrowptr[r + 1];
-        const int32_t p = (int32_t)(std::lower_bound(b, en, col) - h->hcolidx.begin());
-        slots[e * 36 + 8 * a - (a * (a - 1)) / 2 + (c - a)] = p;
-        kcnt[p + 1]++;
-      }
+  parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) {
+    for (int64_t e = e0; e < e1; ++e)
+      for (int a = 0; a < 8; ++a)
+        for (int c = a; c < 8; ++c) {
+          const int32_t na = conn[e * 8 + a], nc = conn[e * 8 + c];
+          const int32_t r = na < nc ? na : nc, col = na < nc ? nc : na;
+          const auto b = h->hcolidx.begin() + h->hrowptr[r], en = h->hcolidx.begin() + h->hrowptr[r + 1];
+          slots[e * 36 + 8 * a - (a * (a - 1)) / 2 + (c - a)] =
+              (int32_t)(std::lower_bound(b, en, col) - h->hcolidx.begin());
+        }
+  });
   // upper blocks and their transposes
   h->huptr.assign(nn + 1, 0);
   for (int64_t i = 0; i < nn; ++i) h->huptr[i + 1] = h->huptr[i] + (h->hrowptr[i + 1] - h->hdiag[i]);
   std::vector<int32_t> up(h->huptr[nn]), upT(h->huptr[nn]);
-  for (int64_t i = 0; i < nn; ++i)
-    for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
-      const int32_t c = h->hcolidx[p];
-      const auto b = h->hcolidx.begin() + h->hrowptr[c], en = h->hcolidx.begin() + h->hrowptr[c + 1];
-      const int64_t q = h->huptr[i] + (p - h->hdiag[i]);
-      up[q]           = p;
-      upT[q]          = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
-    }
+  parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
+    for (int64_t i = i0; i < i1; ++i)
+      for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
+        const int32_t c = h->hcolidx[p];
+        const auto b = h->hcolidx.begin() + h->hrowptr[c], en = h->hcolidx.begin() + h->hrowptr[c + 1];
+        const int64_t q = h->huptr[i] + (p - h->hdiag[i]);
+        up[q]           = p;
+        upT[q]          = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
+      }
+  });
   const int64_t nup = h->huptr[nn];
   std::vector<int32_t> qOf(h->nnzb, -1);
-  for (int64_t q = 0; q < nup; ++q) qOf[up[q]] = (int32_t)q;
+  parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
+    for (int64_t q = q0; q < q1; ++q) qOf[up[q]] = (int32_t)q;
+  });
   // FULL Dirichlet treatment per upper block: bit 3k+l zeroes entry (k,l), bit 9+k sets a unit diagonal
   std::vector<uint16_t> mask(nup, 0);
-  for (int64_t i = 0; i < nn; ++i)
-    for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
-      const int64_t c = h->hcolidx[p];
-      uint16_t m      = 0;
-      for (int k = 0; k < 3; ++k)
-        for (int l = 0; l < 3; ++l)
-          if (h->hfixed[3 * i + k] | h->hfixed[3 * c + l]) {
-            m |= (uint16_t)(1u << (3 * k + l));
-            if (3 * i + k == 3 * c + l) m |= (uint16_t)(1u << (9 + k));
-          }
-      mask[h->huptr[i] + (p - h->hdiag[i])] = m;
-    }
+  parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
+    for (int64_t i = i0; i < i1; ++i)
+      for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
+        const int64_t c = h->hcolidx[p];
+        uint16_t m      = 0;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l)
+            if (h->hfixed[3 * i + k] | h->hfixed[3 * c + l]) {
+              m |= (uint16_t)(1u << (3 * k + l));
+              if (3 * i + k == 3 * c + l) m |= (uint16_t)(1u << (9 + k));
+            }
+        mask[h->huptr[i] + (p - h->hdiag[i])] = m;
+      }
+  });
   std::vector<int32_t> seg(nup + 1, 0);
   std::vector<uint8_t> gathered(nup, 1);  // upper blocks the gather assembles (the rest are written directly)
   std::vector<int64_t> unitOf(ne);  // staging unit of each element: its tile (tiled) or itself
@@ -177,7 +245,9 @@ void buildPattern(ik_handle* h) {
   h->tiled = h->neas == 0;
   if (!h->tiled) {
     // EAS: one slot per element block, segments in upper-block order, elements in increasing order
-    for (int64_t q = 0; q < nup; ++q) seg[q + 1] = seg[q] + kcnt[up[q] + 1];
+    std::vector<int32_t> kcnt(h->nnzb, 0);  // element blocks per K block
+    for (const int32_t p : slots) kcnt[p]++;
+    for (int64_t q = 0; q < nup; ++q) seg[q + 1] = seg[q] + kcnt[up[q]];
     std::vector<int32_t> dest((size_t)ne * 36);
     std::vector<int32_t> pos(seg.begin(), seg.end() - 1);
     for (int64_t e = 0; e < ne; ++e) {
@@ -201,75 +271,95 @@ void buildPattern(ik_handle* h) {
     const std::vector<int32_t> order = mortonOrder(h);
     const int64_t ntiles = (ne + kTileElems - 1) / kTileElems;
     std::vector<std::pair<int32_t, int32_t>> tkey(ntiles);  // (lowest node, Morton tile)
-    for (int64_t m = 0; m < ntiles; ++m) {
-      int32_t lo = INT32_MAX;
-      for (int64_t i = m * kTileElems; i < std::min(ne, (m + 1) * kTileElems); ++i)
-        for (int a = 0; a < 8; ++a) lo = std::min(lo, conn[(int64_t)order[i] * 8 + a]);
-      tkey[m] = {lo, (int32_t)m};
-    }
+    parallelChunks(ntiles, [&](int64_t m0, int64_t m1, int64_t) {
+      for (int64_t m = m0; m < m1; ++m) {
+        int32_t lo = INT32_MAX;
+        for (int64_t i = m * kTileElems; i < std::min(ne, (m + 1) * kTileElems); ++i)
+          for (int a = 0; a < 8; ++a) lo = std::min(lo, conn[(int64_t)order[i] * 8 + a]);
+        tkey[m] = {lo, (int32_t)m};
+      }
+    });
     std::sort(tkey.begin(), tkey.end());
     std::vector<int32_t> tileElem((size_t)ntiles * kTileElems, -1);
-    for (int64_t t = 0; t < ntiles; ++t) {
-      const int64_t m = tkey[t].second;
-      for (int64_t i = m * kTileElems; i < std::min(ne, (m + 1) * kTileElems); ++i) {
-        tileElem[t * kTileElems + (i - m * kTileElems)] = order[i];
-        unitOf[order[i]] = t;
+    parallelChunks(ntiles, [&](int64_t t0, int64_t t1, int64_t) {
+      for (int64_t t = t0; t < t1; ++t) {
+        const int64_t m = tkey[t].second;
+        for (int64_t i = m * kTileElems; i < std::min(ne, (m + 1) * kTileElems); ++i) {
+          tileElem[t * kTileElems + (i - m * kTileElems)] = order[i];
+          unitOf[order[i]] = t;
+        }
       }
-    }
+    });
     // per tile: groups (the entries j = 36 s + sb of one upper block) ordered by upper block, i.e. by staging
     // slot; each entry's rank = its position in its group by increasing j (the order the tile sums them).
     // epos[t*576 + j] = group | rank << 10 | transposed << 15; padding entries target spare accumulators
     // ng + (j - 36 nvalid), rank 0, which the tile never stores. tmaxr[t] = the largest group size.
+    // Each chunk of tiles lists its groups' upper blocks in its own buffer; tgptr places them.
     std::vector<uint16_t> epos((size_t)ntiles * kTileEntries);
     std::vector<uint8_t> tmaxr(ntiles);
-    std::vector<int32_t> tgptr(ntiles + 1, 0), gq, groupOf(nup, -1), touched;
-    gq.reserve((size_t)ne * 24);
-    std::vector<int32_t> entQ(kTileEntries), order2, fill(kTileEntries);
-    std::vector<uint16_t> entTr(kTileEntries);
-    for (int64_t t = 0; t < ntiles; ++t) {
-      touched.clear();
-      int nvalid = 0;
-      for (int s = 0; s < kTileElems; ++s) {
-        const int32_t e = tileElem[t * kTileElems + s];
-        if (e < 0) continue;
-        ++nvalid;
-        for (int a = 0; a < 8; ++a)
-          for (int c = a; c < 8; ++c) {
-            const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
-            const int j     = 36 * s + sb;
-            const int32_t q = qOf[slots[(int64_t)e * 36 + sb]];
-            entQ[j]         = q;
-            entTr[j]        = conn[(int64_t)e * 8 + a] > conn[(int64_t)e * 8 + c] ? 0x8000u : 0u;
-            if (groupOf[q] < 0) {
-              groupOf[q] = 0;
-              touched.push_back(q);
+    std::vector<int32_t> tgptr(ntiles + 1, 0), gq;
+    std::vector<int64_t> tng(ntiles);
+    std::vector<std::vector<int32_t>> gpart(chunkCount(ntiles));
+    parallelChunks(ntiles, [&](int64_t t0, int64_t t1, int64_t ch) {
+      std::vector<int32_t> groupOf(nup, -1), touched, entQ(kTileEntries), fill(kTileEntries);
+      std::vector<uint16_t> entTr(kTileEntries);
+      std::vector<int32_t>& out = gpart[ch];
+      for (int64_t t = t0; t < t1; ++t) {
+        touched.clear();
+        int nvalid = 0;
+        for (int s = 0; s < kTileElems; ++s) {
+          const int32_t e = tileElem[t * kTileElems + s];
+          if (e < 0) continue;
+          ++nvalid;
+          for (int a = 0; a < 8; ++a)
+            for (int c = a; c < 8; ++c) {
+              const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
+              const int j     = 36 * s + sb;
+              const int32_t q = qOf[slots[(int64_t)e * 36 + sb]];
+              entQ[j]         = q;
+              entTr[j]        = conn[(int64_t)e * 8 + a] > conn[(int64_t)e * 8 + c] ? 0x8000u : 0u;
+              if (groupOf[q] < 0) {
+                groupOf[q] = 0;
+                touched.push_back(q);
+              }
             }
-          }
-      }
-      std::sort(touched.begin(), touched.end());
-      const int ng = (int)touched.size();
-      for (int g = 0; g < ng; ++g) groupOf[touched[g]] = g;
-      std::fill(fill.begin(), fill.begin() + ng, 0);
-      uint16_t* ep = epos.data() + t * kTileEntries;
-      int maxr     = 0;
-      for (int j = 0; j < kTileEntries; ++j) {
-        if (j < 36 * nvalid) {
-          const int g  = groupOf[entQ[j]];
-          const int rk = fill[g]++;
-          require(rk < 16, IK_EINVAL, "more than 16 element blocks of one tile on one K block");
-          maxr  = std::max(maxr, rk + 1);
-          ep[j] = (uint16_t)(g | (rk << 10) | entTr[j]);
-        } else {
-          ep[j] = (uint16_t)(ng + (j - 36 * nvalid));
         }
+        std::sort(touched.begin(), touched.end());
+        const int ng = (int)touched.size();
+        for (int g = 0; g < ng; ++g) groupOf[touched[g]] = g;
+        std::fill(fill.begin(), fill.begin() + ng, 0);
+        uint16_t* ep = epos.data() + t * kTileEntries;
+        int maxr     = 0;
+        for (int j = 0; j < kTileEntries; ++j) {
+          if (j < 36 * nvalid) {
+            const int g  = groupOf[entQ[j]];
+            const int rk = fill[g]++;
+            require(rk < 16, IK_EINVAL, "more than 16 element blocks of one tile on one K block");
+            maxr  = std::max(maxr, rk + 1);
+            ep[j] = (uint16_t)(g | (rk << 10) | entTr[j]);
+          } else {
+            ep[j] = (uint16_t)(ng + (j - 36 * nvalid));
+          }
+        }
+        tmaxr[t] = (uint8_t)maxr;
+        for (const int32_t q : touched) {
+          out.push_back(q);
+          groupOf[q] = -1;
+        }
+        tng[t] = ng;
       }
-      tmaxr[t] = (uint8_t)maxr;
-      for (const int32_t q : touched) {
-        gq.push_back(q);
-        groupOf[q] = -1;
+    });
+    {
+      int64_t tot = 0;
+      for (int64_t t = 0; t < ntiles; ++t) {
+        tot += tng[t];
+        require(tot < (int64_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
+        tgptr[t + 1] = (int32_t)tot;
       }
-      require(gq.size() < (size_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
-      tgptr[t + 1] = (int32_t)gq.size();
+      gq.resize(tot);
+      parallelChunks(ntiles, [&](int64_t t0, int64_t, int64_t ch) {
+        std::copy(gpart[ch].begin(), gpart[ch].end(), gq.begin() + tgptr[t0]);
+      });
     }
     // An upper block that only one tile touches is complete after that tile's group sum, which the tile
     // writes straight into K (with the FULL mask and the mirror block). The others are staged, in gather
@@ -313,31 +403,34 @@ void buildPattern(ik_handle* h) {
                 "internal error: staging slots are not a permutation");
         slotSeen[sl] = 1;
       }
-      std::vector<uint32_t> used(kTileEntries * 16);
-      for (int64_t t = 0; t < ntiles; ++t) {
-        const int ng = tgptr[t + 1] - tgptr[t];
-        int nvalid   = 0;
-        for (int s = 0; s < kTileElems; ++s) nvalid += tileElem[t * kTileElems + s] >= 0;
-        require(tileElem[t * kTileElems] >= 0 && ng <= 36 * nvalid && tmaxr[t] >= 1 && tmaxr[t] <= 16, IK_EINVAL,
-                "internal error: bad tile");
-        std::fill(used.begin(), used.end(), 0);
-        std::fill(fill.begin(), fill.begin() + ng, 0);
-        const uint16_t* ep = epos.data() + t * kTileEntries;
-        for (int j = 0; j < kTileEntries; ++j) {
-          const int g = ep[j] & 0x3FF, rk = (ep[j] >> 10) & 0xF;
-          if (j >= 36 * nvalid) {
-            require(g >= ng && g < kTileEntries && rk == 0 && !(ep[j] & 0x8000u), IK_EINVAL,
-                    "internal error: bad padding entry");
-            continue;
+      parallelChunks(ntiles, [&](int64_t t0, int64_t t1, int64_t) {
+        std::vector<uint8_t> used(kTileEntries * 16);
+        std::vector<int32_t> fill(kTileEntries);
+        for (int64_t t = t0; t < t1; ++t) {
+          const int ng = tgptr[t + 1] - tgptr[t];
+          int nvalid   = 0;
+          for (int s = 0; s < kTileElems; ++s) nvalid += tileElem[t * kTileElems + s] >= 0;
+          require(tileElem[t * kTileElems] >= 0 && ng <= 36 * nvalid && tmaxr[t] >= 1 && tmaxr[t] <= 16,
+                  IK_EINVAL, "internal error: bad tile");
+          std::fill(used.begin(), used.begin() + ng * 16, 0);
+          std::fill(fill.begin(), fill.begin() + ng, 0);
+          const uint16_t* ep = epos.data() + t * kTileEntries;
+          for (int j = 0; j < kTileEntries; ++j) {
+            const int g = ep[j] & 0x3FF, rk = (ep[j] >> 10) & 0xF;
+            if (j >= 36 * nvalid) {
+              require(g >= ng && g < kTileEntries && rk == 0 && !(ep[j] & 0x8000u), IK_EINVAL,
+                      "internal error: bad padding entry");
+              continue;
+            }
+            require(g < ng && rk < tmaxr[t] && !used[g * 16 + rk], IK_EINVAL, "internal error: bad tile entry");
+            used[g * 16 + rk] = 1;
+            fill[g]++;
           }
-          require(g < ng && rk < tmaxr[t] && !used[g * 16 + rk], IK_EINVAL, "internal error: bad tile entry");
-          used[g * 16 + rk] = 1;
-          fill[g]++;
+          for (int g = 0; g < ng; ++g)
+            for (int rk = 0; rk < 16; ++rk)
+              require((rk < fill[g]) == (used[g * 16 + rk] != 0), IK_EINVAL, "internal error: gap in a tile group");
         }
-        for (int g = 0; g < ng; ++g)
-          for (int rk = 0; rk < 16; ++rk)
-            require((rk < fill[g]) == (used[g * 16 + rk] != 0), IK_EINVAL, "internal error: gap in a tile group");
-      }
+      });
     }
     h->ntiles = ntiles;
     nunits    = ntiles;
@@ -373,8 +466,10 @@ void buildPattern(ik_handle* h) {
     h->segptr.upload(cseg.data(), cseg.size(), h->stream);
   }
   std::vector<int32_t> brow(h->nnzb);
-  for (int64_t i = 0; i < nn; ++i)
-    for (int32_t p = h->hrowptr[i]; p < h->hrowptr[i + 1]; ++p) brow[p] = (int32_t)i;
+  parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
+    for (int64_t i = i0; i < i1; ++i)
+      for (int32_t p = h->hrowptr[i]; p < h->hrowptr[i + 1]; ++p) brow[p] = (int32_t)i;
+  });
   // node incidences (e*8 + a), ordered by element
   std::vector<int32_t> ncnt(nn + 1, 0), nlist((size_t)ne * 8);
   for (int64_t i = 0; i < nn; ++i) ncnt[i + 1] = (int32_t)(cnt[i + 1] - cnt[0]);
@@ -413,7 +508,9 @@ void buildPattern(ik_handle* h) {
   h->blockRow.upload(brow.data(), brow.size(), h->stream);
   // residual staging slot of every element corner: its position in the node's incidence list
   std::vector<int32_t> rdest((size_t)ne * 8);
-  for (size_t j = 0; j < nlist.size(); ++j) rdest[nlist[j]] = (int32_t)j;
+  parallelChunks((int64_t)nlist.size(), [&](int64_t j0, int64_t j1, int64_t) {
+    for (int64_t j = j0; j < j1; ++j) rdest[nlist[j]] = (int32_t)j;
+  });
   h->ncptr.upload(ncnt.data(), ncnt.size(), h->stream);
   h->rdest.upload(rdest.data(), rdest.size(), h->stream);
   IK_HIP(hipStreamSynchronize(h->stream));

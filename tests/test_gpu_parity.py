@@ -290,3 +290,23 @@ def test_scrambled_mesh_assembly(monkeypatch):
         dev.close()
     for a, b in zip(out[1], out[0]):
         assert np.array_equal(a, b)
+
+
+def test_pattern_build_independent_of_thread_count(monkeypatch):
+    """The host pattern build splits nodes, elements and tiles over threads; a 48^3 scrambled mesh (several
+    chunks in every phase) built with 1 and with 5 threads gives the same pattern and bitwise the same K
+    and R."""
+    p = _scrambled_block((48, 48, 48), seed=23, drop=0.1)
+    print(f"case: pattern threads, {p.ne} elements", flush=True)
+    u = 0.01 * np.random.default_rng(9).standard_normal(p.ndof) * (1 - p.fixed)
+    out = []
+    for threads in ("1", "5"):
+        monkeypatch.setenv("IK_PATTERN_THREADS", threads)
+        dev = ia.Device(p)
+        dev.set_requirement(u, 0.7)
+        R = dev.assemble(capi.DBC_FULL)
+        out.append((dev.csc(capi.DBC_FULL), R, dev.assembly_layout()))
+        dev.close()
+    (c1, R1, l1), (c5, R5, l5) = out
+    assert all(np.array_equal(a, b) for a, b in zip(c1, c5))
+    assert np.array_equal(R1, R5) and l1 == l5
