@@ -5,6 +5,6 @@ The compute lives in libikarus_amd.so (hand-written HIP for gfx950) behind the C
 include/ikarus_amd.h; this package is the host-side mirror of the reference operator API.
 """
 from . import _capi as capi  # noqa: F401
-from .assembler import (FERequirement, LoadControl, NewtonRaphson, SparseFlatAssembler, Device,  # noqa: F401
-                        nr_settings, solver_opts)
+from .assembler import (FERequirement, LoadControl, NewtonRaphson, PathFollowing,  # noqa: F401
+                        SparseFlatAssembler, Device, nr_settings, solver_opts)
 from . import problems  # noqa: F401

@@ -24,7 +24,8 @@ EXPORTED = [
     "ik_set_requirement", "ik_get_requirement", "ik_set_dirichlet_values", "ik_vector", "ik_matrix", "ik_assemble",
     "ik_matrix_csc", "ik_matrix_bsr",
     "ik_element", "ik_update_state", "ik_get_alpha", "ik_set_alpha", "ik_solve", "ik_newton_solve",
-    "ik_load_control", "ik_mlp_num_params", "ik_mlp_set_params", "ik_mlp_get_params", "ik_mlp_set_load",
+    "ik_load_control", "ik_path_following", "ik_mlp_num_params", "ik_mlp_set_params", "ik_mlp_get_params",
+    "ik_mlp_set_load",
     "ik_timing_enable", "ik_timing_read", "ik_timing_reset", "ik_synchronize", "ik_assembly_layout",
     "ik_mlp_create", "ik_mlp_destroy", "ik_mlp_last_error", "ik_mlp_params", "ik_mlp_load", "ik_mlp_forward",
     "ik_mlp_loss_grad", "ik_mlp_train_step",
@@ -64,6 +65,10 @@ class NRInfo(C.Structure):
 
 class LCConfig(C.Structure):
     _fields_ = [("load_steps", C.c_int32), ("tbegin", C.c_double), ("tend", C.c_double), ("predictor", C.c_int32)]
+
+
+class PFConfig(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("step_size", C.c_double)]
 
 
 class ControlInfo(C.Structure):
@@ -107,6 +112,8 @@ def lib():
         L.ik_solve.argtypes = [P, P, P, I, C.POINTER(SolverOpts), C.POINTER(SolverInfo)]
         L.ik_newton_solve.argtypes = [P, C.POINTER(NRSettings), D, C.POINTER(NRInfo)]
         L.ik_load_control.argtypes = [P, C.POINTER(LCConfig), C.POINTER(NRSettings), C.POINTER(ControlInfo), P, P]
+        L.ik_path_following.argtypes = [P, C.POINTER(PFConfig), C.POINTER(NRSettings), C.POINTER(ControlInfo), P, P,
+                                        P]
         L.ik_mlp_num_params.argtypes = []
         L.ik_mlp_num_params.restype = I64
         L.ik_mlp_set_params.argtypes = [P, P, I]

@@ -186,6 +186,16 @@ class Device:
                                               ptr(res)))
         return info, iters, res
 
+    def path_following(self, config, settings):
+        """Arc-length PathFollowing::run: (info, iterations, residual norms, load factors) per solve."""
+        info = capi.ControlInfo()
+        iters = np.zeros(config.steps + 1, np.int32)
+        res = np.zeros(config.steps + 1)
+        lams = np.zeros(config.steps + 1)
+        self.check(capi.lib().ik_path_following(self.h, C.byref(config), C.byref(settings), C.byref(info),
+                                                ptr(iters), ptr(res), ptr(lams)))
+        return info, iters, res, lams
+
     def timing(self, on=True):
         self.check(capi.lib().ik_timing_enable(self.h, 1 if on else 0))
 
@@ -314,3 +324,24 @@ class LoadControl:
         req.u, req.lam = dev.get_requirement()
         return dict(success=bool(info.success), totalIterations=info.total_iterations,
                     linearIterations=info.linear_iterations, iterations=iters[:info.solves], residuals=res[:info.solves])
+
+
+class PathFollowing:
+    """PathFollowing mirror with the ArcLength technique (pathfollowing.inl:28-77): `steps` steps of arc
+    length `step_size` from an equilibrium start at lambda = 0, plain elements."""
+
+    def __init__(self, nonlinear_solver, steps, step_size):
+        if steps <= 0:
+            raise IkError(capi.IK_EINVAL, "Number of load steps should be greater than zero.")
+        self.nr = nonlinear_solver
+        self.cfg = capi.PFConfig()
+        self.cfg.steps, self.cfg.step_size = steps, step_size
+
+    def run(self, req):
+        dev = self.nr.assembler.dev
+        dev.set_requirement(req.u, req.lam)
+        info, iters, res, lams = dev.path_following(self.cfg, self.nr.settings)
+        req.u, req.lam = dev.get_requirement()
+        return dict(success=bool(info.success), totalIterations=info.total_iterations,
+                    linearIterations=info.linear_iterations, iterations=iters[:info.solves],
+                    residuals=res[:info.solves], loadFactors=lams[:info.solves])

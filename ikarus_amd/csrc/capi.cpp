@@ -59,6 +59,9 @@ __global__ void syncIdbcKernel(const int32_t* __restrict__ fl, int64_t nf, const
     if (v != 0.0) u[d] = v;
   }
 }
+__global__ void scaleKernel(double* __restrict__ x, double a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) x[i] *= a;
+}
 __global__ void zeroListKernel(const int32_t* __restrict__ fl, int64_t nf, double* __restrict__ f) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nf; i += (int64_t)gridDim.x * 256) f[fl[i]] = 0.0;
 }
@@ -186,6 +189,64 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     h->easCacheValid = false;
     assembleR(h, IK_DBC_FULL);
     rNorm               = normR(h);
+    info->residual_norm = rNorm;
+    ++iter;
+  }
+  if (iter == s->max_iter) info->success = 0;
+  info->iterations = iter;
+}
+
+// ---- arc-length path following -------------------------------------------------------------------
+struct ArcState {
+  double Dl = 0.0, psi = 0.0, stepSize = 0.0;
+  bool predicted = false;  // the initial prediction has run: the subsidiary derivatives are live
+};
+
+// ArcLength::operator() (pathfollowingfunctions.hh:103-115): f = sqrt(DD.DD + psi^2 Dl^2) - s; returns f,
+// 1/root (the factor of df/dDD = DD/root) and df/dDl, both zero before the initial prediction
+static void arcSubsidiary(ik_handle* h, const ArcState& a, double& f, double& invRoot, double& dfdDl) {
+  const double root = std::sqrt(ik::dot(h, h->arcDD.p, h->arcDD.p, h->ndof) + a.psi * a.psi * a.Dl * a.Dl);
+  f                 = root - a.stepSize;
+  invRoot           = a.predicted ? 1.0 / root : 0.0;
+  dfdDl             = a.predicted ? a.psi * a.psi * a.Dl / root : 0.0;
+}
+
+// NewtonRaphsonWithSubsidiaryFunction::solve (newtonraphsonwithscalarsubsidiaryfunction.hh:190-307) at the
+// bound requirement; h->arcF holds Fext0
+static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_info* info) {
+  const int64_t n = h->ndof;
+  *info           = ik_nr_info{};
+  info->success   = 1;
+  assembleR(h, IK_DBC_FULL);
+  double f, invRoot, dfdDl;
+  arcSubsidiary(h, a, f, invRoot, dfdDl);
+  double rNorm        = normR(h);  // the first norm leaves f out (:244)
+  info->residual_norm = rNorm;
+  int iter            = 0;
+  while (rNorm > s->tol && iter < s->max_iter) {
+    assembleK(h, IK_DBC_FULL, s->precision);
+    ik_solver_info li{};
+    linearSolve(h, h->R.p, h->arcDR.p, &s->linear, s->linear.rtol, &li);  // K x = R, dR = -x
+    info->linear_iterations += li.iterations;
+    hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcDR.p, n);
+    linearSolve(h, h->arcF.p, h->arcDL.p, &s->linear, s->linear.rtol, &li);  // K dL = Fext0
+    info->linear_iterations += li.iterations;
+    const double ddR = a.predicted ? ik::dot(h, h->arcDD.p, h->arcDR.p, n) : 0.0;
+    const double ddL = a.predicted ? ik::dot(h, h->arcDD.p, h->arcDL.p, n) : 0.0;
+    const double den = invRoot * ddL + dfdDl;
+    require(den != 0.0, IK_EINVAL,
+            "path following: the subsidiary equation has no load direction (the start state is not in equilibrium)");
+    const double dl = (-f - invRoot * ddR) / den;
+    ik::axpy(h, dl, h->arcDL.p, h->arcDR.p, n);  // the correction dR + dl dL
+    info->correction_norm = std::sqrt(ik::dot(h, h->arcDR.p, h->arcDR.p, n) + dl * dl);
+    ik::axpy(h, 1.0, h->arcDR.p, h->u.p, n);
+    ik::axpy(h, 1.0, h->arcDR.p, h->arcDD.p, n);
+    h->lambda += dl;
+    a.Dl += dl;
+    h->kValid = false;
+    assembleR(h, IK_DBC_FULL);
+    arcSubsidiary(h, a, f, invRoot, dfdDl);
+    rNorm               = std::sqrt(ik::dot(h, h->R.p, h->R.p, n) + f * f);
     info->residual_norm = rNorm;
     ++iter;
   }
@@ -585,6 +646,85 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
       if (c->predictor != IK_PRED_CONSTANT)
         hipLaunchKernelGGL(historyKernel, dim3(grid(h->ndof)), dim3(256), 0, h->stream, h->u.p, uPrev.p, duN.p,
                            duN1.p, h->ndof);
+    }
+    info->success = 1;
+  });
+}
+
+int ik_path_following(ik_handle* h, const ik_pf_config* c, const ik_nr_settings* s, ik_control_info* info,
+                      int32_t* iterations, double* residuals, double* lambdas) {
+  return guard(h, [&] {
+    require(c && s && info, IK_EINVAL, "null argument");
+    require(c->steps > 0, IK_EINVAL, "Number of load steps should be greater than zero.");
+    require(c->step_size > 0.0, IK_EINVAL, "the arc length must be positive");
+    require(h->neas == 0, IK_ENOTIMPL, "path following on the device supports plain elements only");
+    require(!h->hasUbar, IK_ENOTIMPL, "path following on the device has no inhomogeneous Dirichlet values");
+    requireBound(h);
+    ensurePattern(h);
+    const int64_t n = h->ndof;
+    *info           = ik_control_info{};
+    for (auto* b : {&h->arcDD, &h->arcDR, &h->arcDL, &h->arcF}) b->ensure(n);
+    // Fext0 = -R(0, 1): the load direction of dead loads (newtonraphsonwithscalarsubsidiaryfunction.hh:224-232)
+    {
+      h->tmp.ensure(n);
+      IK_HIP(hipMemcpyAsync(h->tmp.p, h->u.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      const double lam0 = h->lambda;
+      ik::launchZero(h->u.p, n, h->stream);
+      h->lambda = 1.0;
+      assembleR(h, IK_DBC_FULL);
+      IK_HIP(hipMemcpyAsync(h->arcF.p, h->R.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcF.p, n);
+      IK_HIP(hipMemcpyAsync(h->u.p, h->tmp.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      h->lambda = lam0;
+    }
+    ArcState a;
+    a.stepSize = c->step_size;
+    ik::launchZero(h->arcDD.p, n, h->stream);
+    ik_nr_info ni{};
+    auto record = [&](int k) {
+      if (iterations) iterations[k] = ni.iterations;
+      if (residuals) residuals[k] = ni.residual_norm;
+      if (lambdas) lambdas[k] = h->lambda;
+      info->solves += 1;
+      info->total_iterations += ni.iterations;
+      info->linear_iterations += ni.linear_iterations;
+    };
+    arcNewton(h, s, a, &ni);  // the start state (pathfollowing.inl:40-46)
+    record(0);
+    if (!ni.success) return;
+    // ArcLength::initialPrediction (pathfollowingfunctions.hh:131-170): K DD = -R at lambda = 1, psi = |DD|,
+    // DD and Dl = 1 scaled to the arc length, then psi = |DD| / Dl
+    h->lambda = 1.0;
+    assembleR(h, IK_DBC_FULL);
+    assembleK(h, IK_DBC_FULL, s->precision);
+    {
+      ik_solver_info li{};
+      linearSolve(h, h->R.p, h->arcDD.p, &s->linear, s->linear.rtol, &li);
+      info->linear_iterations += li.iterations;
+    }
+    hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcDD.p, n);
+    const double dd2 = ik::dot(h, h->arcDD.p, h->arcDD.p, n);
+    a.psi            = std::sqrt(dd2);
+    const double sc  = std::sqrt(dd2 + a.psi * a.psi);
+    require(sc > 0.0, IK_EINVAL, "path following: the load direction is zero");
+    hipLaunchKernelGGL(scaleKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcDD.p, a.stepSize / sc, n);
+    a.Dl = a.stepSize / sc;
+    ik::axpy(h, 1.0, h->arcDD.p, h->u.p, n);
+    h->lambda   = a.Dl;
+    h->kValid   = false;
+    a.psi       = std::sqrt(ik::dot(h, h->arcDD.p, h->arcDD.p, n) / (a.Dl * a.Dl));
+    a.predicted = true;
+    arcNewton(h, s, a, &ni);
+    record(1);
+    if (!ni.success) return;
+    for (int ls = 1; ls < c->steps; ++ls) {
+      // ArcLength::intermediatePrediction (pathfollowingfunctions.hh:172-185): the last step's increments
+      ik::axpy(h, 1.0, h->arcDD.p, h->u.p, n);
+      h->lambda += a.Dl;
+      h->kValid = false;
+      arcNewton(h, s, a, &ni);
+      record(ls + 1);
+      if (!ni.success) return;
     }
     info->success = 1;
   });

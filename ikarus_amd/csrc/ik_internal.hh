@@ -111,6 +111,8 @@ struct ik_handle {
   // inhomogeneous Dirichlet values u_D(lambda) = lambda * ubar on constrained DOFs (none unless set)
   bool hasUbar = false;
   ik::DevBuf<double> ubar, idbcF;
+  // arc length: the step's accumulated increment, the two correction directions and the load direction
+  ik::DevBuf<double> arcDD, arcDR, arcDL, arcF;
   // loads
   int64_t nfaces = 0;
   ik::DevBuf<uint8_t> faceBits;  // per element: bit 2*axis + side set for each loaded face (empty: none)

@@ -190,6 +190,21 @@ typedef struct {
 int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s, ik_control_info* info,
                     int32_t* iterations, double* residuals);
 
+typedef struct {
+  int32_t steps;     /* PathFollowing steps (the start-state solve comes on top)        */
+  double step_size;  /* arc length s of ArcLength                                       */
+} ik_pf_config;
+
+/* PathFollowing::run with ArcLength (pathfollowing.inl:28-77; pathfollowingfunctions.hh:88-190), each
+ * solve NewtonRaphsonWithSubsidiaryFunction::solve (newtonraphsonwithscalarsubsidiaryfunction.hh:190-307):
+ * per iteration K dR = -R and K dL = Fext0 = -R(0, 1) with the FULL tangent and s->linear, then the load
+ * increment from the linearized arc-length equation. The requirement must start at lambda = 0 in
+ * equilibrium (the start solve has no load direction yet, IK_EINVAL otherwise). Plain elements only (an
+ * EAS element's internal forces at u = 0 do not vanish, IK_ENOTIMPL), no inhomogeneous Dirichlet values.
+ * iterations/residuals/lambdas: optional arrays of steps+1 entries (per solve). s->forcing is not used. */
+int ik_path_following(ik_handle* h, const ik_pf_config* c, const ik_nr_settings* s, ik_control_info* info,
+                      int32_t* iterations, double* residuals, double* lambdas);
+
 /* ---- learned per-node predictor (no reference counterpart) ------------------------------------ */
 /* MLP 14 -> 128 -> 128 -> 128 -> 3, SiLU, BF16 operands on MFMA with FP32 accumulation. Parameters are
  * FP32 master copies laid out W1[128x14], b1[128], W2[128x128], b2, W3[128x128], b3, W4[3x128], b4[3]

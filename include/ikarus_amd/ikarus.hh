@@ -277,4 +277,41 @@ class LoadControl {
   int predictor_;
 };
 
+// PathFollowing(nr, loadSteps, stepSize) with the ArcLength technique (controlroutines/pathfollowing.hh);
+// the solve count is loadSteps + 1 (the start-state solve first), the load factor after each solve in
+// loadFactors
+class PathFollowing {
+ public:
+  PathFollowing(NewtonRaphson& nr, int loadSteps, double stepSize) : nr_(nr), steps_(loadSteps), s_(stepSize) {
+    if (loadSteps <= 0) throw Error(IK_EINVAL, "Number of load steps should be greater than zero.");
+  }
+
+  ControlInformation run(FERequirement& req) {
+    ik_handle* h = nr_.assembler().handle();
+    check(h, ik_set_requirement(h, req.globalSolution().data(), req.parameter(), IK_HOST));
+    const ik_nr_settings st = nr_.settings();
+    ik_pf_config c{steps_, s_};
+    ik_control_info info{};
+    std::vector<int32_t> it(steps_ + 1, 0);
+    std::vector<double> res(steps_ + 1, 0.0);
+    loadFactors.assign(steps_ + 1, 0.0);
+    check(h, ik_path_following(h, &c, &st, &info, it.data(), res.data(), loadFactors.data()));
+    check(h, ik_get_requirement(h, req.globalSolution().data(), &req.parameter(), IK_HOST));
+    ControlInformation ci;
+    ci.success         = info.success != 0;
+    ci.totalIterations = info.total_iterations;
+    ci.iterations.assign(it.begin(), it.begin() + info.solves);
+    ci.residualNorms.assign(res.begin(), res.begin() + info.solves);
+    loadFactors.resize(info.solves);
+    return ci;
+  }
+
+  std::vector<double> loadFactors;
+
+ private:
+  NewtonRaphson& nr_;
+  int steps_;
+  double s_;
+};
+
 }  // namespace Ikarus::AMD

@@ -1,0 +1,87 @@
+"""Arc-length path following on the device against the oracle's restatement (oracle/pathfollowing.py,
+pinned by the reference's two-DOF golden values in tests/test_oracle_golden.py), with the oracle's dense
+FULL-DBC residual and tangent of the same mesh.
+
+Bars: Newton iterations of every solve identical, load factor after every solve within 1e-10, final
+displacements within 1e-8 relative (the device solves with FP64 PCG to 1e-13, the oracle densely).
+"""
+import numpy as np
+import pytest
+
+import ikarus_amd as ia
+import pathfollowing as pf
+import pyoracle as po
+from ikarus_amd import _capi as capi
+from ikarus_amd import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_for(p):
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, p.eas, 2)
+    o.set_fixed(p.fixed)
+    o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction)
+    return o
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_arc_length_matches_oracle(material):
+    p = pb.hex_block((4, 3, 3), material=material, jitter=0.05, seed=41, mag_range=(20.0, 30.0))
+    print(f"case: arc length material={material}", flush=True)
+    o = _oracle_for(p)
+    res = lambda u, lam: o.vector(u, lam, po.FULL)
+    jac = lambda u, lam: o.matrix_dense(u, lam, po.FULL)
+    z = np.zeros(p.ndof)
+    d1 = np.linalg.solve(jac(z, 1.0), -res(z, 1.0))
+    s = 0.25 * np.sqrt(2.0) * np.linalg.norm(d1)  # about a quarter of the load per step
+    r = pf.arc_length(res, jac, z, steps=4, step_size=s)
+    assert r["success"]
+    dev = ia.Device(p)
+    dev.set_requirement(z, 0.0)
+    cfg = capi.PFConfig()
+    cfg.steps, cfg.step_size = 4, s
+    info, iters, rn, lams = dev.path_following(cfg, ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-13)))
+    ug, lam = dev.get_requirement()
+    dev.close()
+    assert info.success and info.solves == 5
+    assert list(iters) == list(r["iterations"])
+    assert np.all(np.abs(lams - np.array(r["lambdas"])) < 1e-10)
+    assert abs(lam - r["lam"]) < 1e-10 and 0.8 < lam < 1.2
+    assert np.linalg.norm(ug - r["u"]) / np.linalg.norm(r["u"]) < 1e-8
+    assert np.all(rn[1:] <= 1e-8)
+
+
+def test_arc_length_rejects_eas():
+    p = pb.hex_block((2, 2, 2), eas=21, seed=3)
+    print("case: arc length EAS", flush=True)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.PFConfig()
+    cfg.steps, cfg.step_size = 2, 0.1
+    with pytest.raises(RuntimeError):
+        dev.path_following(cfg, ia.nr_settings())
+    dev.close()
+
+
+def test_path_following_mirror():
+    """The host mirror (SparseFlatAssembler, NewtonRaphson, PathFollowing, FERequirement) runs the same path."""
+    p = pb.hex_block((3, 3, 2), jitter=0.05, seed=43, mag_range=(20.0, 30.0))
+    print("case: path following mirror", flush=True)
+    o = _oracle_for(p)
+    z = np.zeros(p.ndof)
+    d1 = np.linalg.solve(o.matrix_dense(z, 1.0, po.FULL), o.vector(z, 1.0, po.FULL))
+    step = 0.2 * np.sqrt(2.0) * np.linalg.norm(d1)  # about a fifth of the load per step
+    settings = ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-13))
+    asm = ia.SparseFlatAssembler(p)
+    req = ia.FERequirement(np.zeros(p.ndof), 0.0)
+    out = ia.PathFollowing(ia.NewtonRaphson(asm, settings), 3, step).run(req)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.PFConfig()
+    cfg.steps, cfg.step_size = 3, step
+    info, iters, _, lams = dev.path_following(cfg, settings)
+    ug, lam = dev.get_requirement()
+    dev.close()
+    assert out["success"] and info.success
+    assert list(out["iterations"]) == list(iters) and np.array_equal(out["loadFactors"], lams)
+    assert np.array_equal(req.u, ug) and req.lam == lam

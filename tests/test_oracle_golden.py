@@ -267,3 +267,23 @@ def test_newton_quadratic_convergence():
     r = o.load_control(2, 0.0, 1.0, tol=1e-12)
     assert r["success"] and np.all(r["iterations"][1:] <= 6)
     assert r["residuals"][-1] <= 1e-12
+
+
+# tests/src/testpathfollowing.cpp:26-68,376-377: two-DOF residual, arc length 0.1, 5 steps, LDL^T solves
+def test_arc_length_two_dof_golden():
+    import pathfollowing as pf
+
+    def residual(D, lam):
+        u, w = D
+        return np.array([16.0 * u - 3 * w - w ** 3 - (2.0 / 3.0) * lam, -3 * u + 4 * w - 3 * u * w * w - (9.0 / 4.0) * lam])
+
+    def jacobian(D, lam):
+        u, w = D
+        return np.array([[16.0, -3.0 * w * w - 3], [-3 * w * w - 3, -6 * u * w + 4]])
+
+    r = pf.arc_length(residual, jacobian, np.zeros(2), steps=5, step_size=0.1)
+    assert r["success"] and r["iterations"] == [0, 2, 2, 2, 2, 2]
+    assert np.allclose(r["u"], [0.0883524681713636, 0.3486891415306083], rtol=1e-12, atol=0)
+    assert abs(r["lam"] - 0.4877655071723067) < 1e-12
+    # every step advances the load by the same arc length of (u, psi lambda): the constraint holds
+    assert np.all(np.diff(r["lambdas"][1:]) > 0)
