@@ -21,8 +21,8 @@ KERNEL_ASSEMBLE_KR, KERNEL_ASSEMBLE_R, KERNEL_SPMV, KERNEL_MLP, KERNEL_GATHER_K 
 
 EXPORTED = [
     "ik_create", "ik_destroy", "ik_last_error", "ik_bad_element", "ik_size", "ik_reduced_size",
-    "ik_set_requirement", "ik_get_requirement", "ik_set_dirichlet_values", "ik_vector", "ik_matrix", "ik_assemble",
-    "ik_matrix_csc", "ik_matrix_bsr",
+    "ik_set_requirement", "ik_get_requirement", "ik_set_dirichlet_values", "ik_energy", "ik_vector", "ik_matrix",
+    "ik_assemble", "ik_matrix_csc", "ik_matrix_bsr",
     "ik_element", "ik_update_state", "ik_get_alpha", "ik_set_alpha", "ik_solve", "ik_newton_solve",
     "ik_load_control", "ik_path_following", "ik_mlp_num_params", "ik_mlp_set_params", "ik_mlp_get_params",
     "ik_mlp_set_load",
@@ -99,6 +99,7 @@ def lib():
         L.ik_reduced_size.restype = I64
         L.ik_set_requirement.argtypes = [P, P, D, I]
         L.ik_set_dirichlet_values.argtypes = [P, P, I]
+        L.ik_energy.argtypes = [P, P]
         L.ik_get_requirement.argtypes = [P, P, P, I]
         L.ik_vector.argtypes = [P, I, P, I]
         L.ik_matrix.argtypes = [P, I, I]

@@ -19,6 +19,7 @@ from ._capi import IkError, ptr
 
 STIFFNESS = "stiffness"
 FORCES = "forces"
+ENERGY = "mechanicalPotentialEnergy"
 
 
 class FERequirement:
@@ -114,6 +115,12 @@ class Device:
         (evaluateInhomogeneousBoundaryCondition, dirichletvalues.hh:214-257); None clears them."""
         self._ubar = None if ubar is None else np.ascontiguousarray(ubar, dtype=np.float64)
         self.check(capi.lib().ik_set_dirichlet_values(self.h, ptr(self._ubar), capi.HOST))
+
+    def energy(self):
+        """Potential energy at the bound requirement (plain element)."""
+        e = C.c_double()
+        self.check(capi.lib().ik_energy(self.h, C.byref(e)))
+        return e.value
 
     def get_requirement(self):
         u = np.zeros(self.ndof)
@@ -274,6 +281,13 @@ class SparseFlatAssembler:
 
     def createReducedVector(self, full):
         return np.asarray(full)[~self._fixed].copy()
+
+    def scalar(self, req=None, affordance=ENERGY):
+        """ScalarAssembler::scalar (interface.hh:283-330): the potential energy at the requirement."""
+        if affordance != ENERGY:
+            raise IkError(capi.IK_ENOTIMPL, f"ScalarAffordance not implemented: {affordance}")
+        self._sync(req)
+        return self.dev.energy()
 
     def vector(self, req=None, affordance=FORCES, dbc=None):
         if affordance != FORCES:

@@ -698,6 +698,118 @@ void launchElem(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32
   }
 }
 
+// Potential energy: the hyperelastic energy sum psi |det J| / 8 over the 2x2x2 Gauss points
+// (NonLinearElastic::energyFunction, nonlinearelastic.hh:403-408) plus the traction skill's
+// -t.u(face centre) |J_Gamma| on every loaded face (traction.hh:71-104, 1-point face rule). One thread per
+// element (grid-stride), block sums in a fixed order.
+template <int KIND>
+__global__ __launch_bounds__(256) void energyKernel(const double* __restrict__ X, const double* __restrict__ U,
+                                                   const int32_t* __restrict__ conn, int64_t ne, double lam,
+                                                   double mu, const uint8_t* __restrict__ fbits, double3 trac,
+                                                   double* __restrict__ partial,
+                                                   unsigned long long* __restrict__ err) {
+  const double gq = 0.28867513459481288225;
+  double acc      = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < ne; e += (int64_t)gridDim.x * 256) {
+    double xe[8][3], ue[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const int64_t nd = conn[e * 8 + a];
+      for (int k = 0; k < 3; ++k) {
+        xe[a][k] = X[nd * 3 + k];
+        ue[a][k] = U[nd * 3 + k];
+      }
+    }
+    for (int g = 0; g < 8; ++g) {
+      double xi[3];
+      for (int k = 0; k < 3; ++k) xi[k] = ((g >> k) & 1) ? 0.5 + gq : 0.5 - gq;
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int a = 0; a < 8; ++a) {
+        double fb[3], dN[3];
+        for (int k = 0; k < 3; ++k) fb[k] = ((a >> k) & 1) ? xi[k] : 1.0 - xi[k];
+        for (int k = 0; k < 3; ++k) dN[k] = (((a >> k) & 1) ? 1.0 : -1.0) * fb[(k + 1) % 3] * fb[(k + 2) % 3];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            J[i][j] += xe[a][i] * dN[j];
+            H[i][j] += ue[a][i] * dN[j];  // du/dxi
+          }
+      }
+      const double dJ = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      double Ji[3][3];
+      Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / dJ;
+      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / dJ;
+      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / dJ;
+      Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / dJ;
+      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / dJ;
+      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / dJ;
+      Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / dJ;
+      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / dJ;
+      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / dJ;
+      double F[3][3];  // I + (du/dxi) J^-1
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          F[i][j] = (i == j ? 1.0 : 0.0) + H[i][0] * Ji[0][j] + H[i][1] * Ji[1][j] + H[i][2] * Ji[2][j];
+      double C[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+      double psi;
+      if constexpr (KIND == IK_SVK) {
+        const double tr = 0.5 * (C[0][0] + C[1][1] + C[2][2] - 3.0);
+        double sq       = 0.0;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            const double Eij = 0.5 * (C[i][j] - (i == j ? 1.0 : 0.0));
+            sq += Eij * Eij;
+          }
+        psi = 0.5 * lam * tr * tr + mu * sq;
+      } else {
+        const double detC = C[0][0] * (C[1][1] * C[2][2] - C[1][2] * C[2][1]) -
+                            C[0][1] * (C[1][0] * C[2][2] - C[1][2] * C[2][0]) +
+                            C[0][2] * (C[1][0] * C[2][1] - C[1][1] * C[2][0]);
+        if (!(detC > 0.0)) {
+          atomicMin(err, (unsigned long long)e);
+          continue;
+        }
+        const double lnJ = 0.5 * log(detC);
+        psi = 0.5 * mu * (C[0][0] + C[1][1] + C[2][2] - 3.0 - 2.0 * lnJ) + 0.5 * lam * lnJ * lnJ;
+      }
+      acc += psi * fabs(dJ) * 0.125;
+    }
+    const unsigned bits = fbits ? fbits[e] : 0u;
+    for (int f = 0; f < 6 && bits; ++f) {
+      if (!((bits >> f) & 1)) continue;
+      const int ax = f >> 1, sd = f & 1;
+      double xi[3];
+      for (int k = 0; k < 3; ++k) xi[k] = k == ax ? (double)sd : 0.5;
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, uc[3] = {0, 0, 0};
+      for (int a = 0; a < 8; ++a) {
+        double fb[3], dN[3];
+        for (int k = 0; k < 3; ++k) fb[k] = ((a >> k) & 1) ? xi[k] : 1.0 - xi[k];
+        for (int k = 0; k < 3; ++k) dN[k] = (((a >> k) & 1) ? 1.0 : -1.0) * fb[(k + 1) % 3] * fb[(k + 2) % 3];
+        const double Na = fb[0] * fb[1] * fb[2];
+        for (int i = 0; i < 3; ++i) {
+          uc[i] += Na * ue[a][i];
+          for (int j = 0; j < 3; ++j) J[i][j] += xe[a][i] * dN[j];
+        }
+      }
+      double t0[3], t1[3];  // the two tangent columns of J (axes other than ax, in increasing order)
+      for (int i = 0; i < 3; ++i) {
+        t0[i] = ax == 0 ? J[i][1] : J[i][0];
+        t1[i] = ax == 2 ? J[i][1] : J[i][2];
+      }
+      const double c0 = t0[1] * t1[2] - t0[2] * t1[1], c1 = t0[2] * t1[0] - t0[0] * t1[2],
+                   c2 = t0[0] * t1[1] - t0[1] * t1[0];
+      acc -= (trac.x * uc[0] + trac.y * uc[1] + trac.z * uc[2]) * sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    }
+  }
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
 template <typename T, bool WK, bool WR>
 void launchElemMat(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32_t* rdest, T* Kebe,
                    double* Rebe, hipStream_t s = nullptr, T* K = nullptr, int mask = 0) {
@@ -729,6 +841,21 @@ static void checkErrFlag(ik_handle* h) {
     throw Error{IK_ENEGDET, "Determinant of right Cauchy Green tensor C must be greater than zero (element " +
                                 std::to_string(v) + ")"};
   }
+}
+
+double energy(ik_handle* h) {
+  require(h->neas == 0, IK_ENOTIMPL,
+          "EAS element do not support any scalar calculations, i.e. they are not derivable from a potential");
+  const int g     = (int)std::min<int64_t>((h->ne + 255) / 256, kReduceBlocks);
+  double* partial = partialsFor(h, g);
+  if (h->material == IK_SVK)
+    hipLaunchKernelGGL(energyKernel<IK_SVK>, dim3(g), dim3(256), 0, h->stream, h->X.p, h->u.p, h->conn.p, h->ne,
+                       h->lam, h->mu, h->faceBits.p, loadedTraction(h), partial, h->errFlag.p);
+  else
+    hipLaunchKernelGGL(energyKernel<IK_NEOHOOKE>, dim3(g), dim3(256), 0, h->stream, h->X.p, h->u.p, h->conn.p,
+                       h->ne, h->lam, h->mu, h->faceBits.p, loadedTraction(h), partial, h->errFlag.p);
+  checkErrFlag(h);
+  return foldPartials(h, g);
 }
 
 // staging units [u0, u1): elements (EAS, K_e and R_e) or tiles (plain element, K_e; blocks complete in a

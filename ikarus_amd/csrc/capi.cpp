@@ -394,6 +394,14 @@ int ik_set_dirichlet_values(ik_handle* h, const double* ubar, int where) {
   });
 }
 
+int ik_energy(ik_handle* h, double* energy) {
+  return guard(h, [&] {
+    require(energy != nullptr, IK_EINVAL, "null argument");
+    requireBound(h);
+    *energy = ik::energy(h);
+  });
+}
+
 int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where) {
   return guard(h, [&] {
     if (u) copyOut(h, u, h->u.p, h->ndof, where);

@@ -216,6 +216,10 @@ void spmv64(ik_handle* h, const double* x, double* y);
 void spmv32(ik_handle* h, const float* x, float* y);
 double dot(ik_handle* h, const double* a, const double* b, int64_t n);
 void axpy(ik_handle* h, double a, const double* x, double* y, int64_t n);  // y += a x
+// per-block partial sums (at most kReduceBlocks) folded in a fixed order
+double* partialsFor(ik_handle* h, int nb);
+double foldPartials(ik_handle* h, int nb);
+double energy(ik_handle* h);  // potential energy at the bound requirement (plain element)
 void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxIter, int* iters, double* relres);
 void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info);
 void residualBlock64(ik_handle* h, const double* b, const double* x, double* r);  // r = b - K x (FP64 K)

@@ -515,6 +515,19 @@ double dot(ik_handle* h, const double* a, const double* b, int64_t n) {
   return h->hred[0];
 }
 
+double* partialsFor(ik_handle* h, int nb) {
+  ensureWork(h);
+  require(nb >= 1 && nb <= kReduceBlocks, IK_EINVAL, "internal error: too many reduction partials");
+  return h->partial.p;
+}
+
+double foldPartials(ik_handle* h, int nb) {
+  hipLaunchKernelGGL(foldKernel, dim3(1), dim3(1024), 0, h->stream, h->partial.p, nb, h->partial.p + h->partial.n - 1);
+  IK_HIP(hipMemcpyAsync(h->hred, h->partial.p + h->partial.n - 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  IK_HIP(hipStreamSynchronize(h->stream));
+  return h->hred[0];
+}
+
 void axpy(ik_handle* h, double a, const double* x, double* y, int64_t n) {
   hipLaunchKernelGGL(axpyKernel, dim3(vecGrid(n)), dim3(256), 0, h->stream, a, x, y, n);
 }

@@ -97,6 +97,13 @@ int ik_set_dirichlet_values(ik_handle* h, const double* ubar, int where);
  * handle's device residual without copying it out. */
 int ik_vector(ik_handle* h, int dbc, double* R, int where);
 
+/* ScalarAssembler::scalar(req, mechanicalPotentialEnergy) (interface.hh:283-330): the sum over elements
+ * of NonLinearElastic's energy (nonlinearelastic.hh:403-408) and the traction skill's (traction.hh:71-104)
+ * at the bound requirement. EAS elements have no potential (enhancedassumedstrains.hh:302-309):
+ * IK_ENOTIMPL. Point loads are not part of it (as for the reference's assembler without a scalar
+ * manipulator). */
+int ik_energy(ik_handle* h, double* energy);
+
 /* MatrixAssembler::matrix(req, stiffness, dbc) (interface.hh:430-440; simpleassemblers.inl:121-204).
  * Assembles into library-owned 3x3-block CSR storage on the device (FP64, or FP32 for IK_FP32). */
 int ik_matrix(ik_handle* h, int dbc, int precision);

@@ -645,8 +645,8 @@ void elementCompute(const Problem& P, int e, const double* uFull, double lambda,
         out.R[r] -= s;
       }
   }
-  // Traction (traction.hh:106-138): 1-point face rule, R_a -= N_a t w |J_Gamma|
-  if (wantR && !P.faces.empty()) {
+  // Traction (traction.hh:71-138): 1-point face rule, R_a -= N_a t w |J_Gamma|, energy -= t.u w |J_Gamma|
+  if (!P.faces.empty()) {
     for (const auto& f : P.faces) {
       if (f.elem != e) continue;
       double xi[D];
@@ -669,8 +669,13 @@ void elementCompute(const Problem& P, int e, const double* uFull, double lambda,
         const int t = 1 - f.axis;
         meas = std::sqrt(gp.J[0][t] * gp.J[0][t] + gp.J[1][t] * gp.J[1][t]);
       }
+      double tu = 0.0;
       for (int a = 0; a < nn; ++a)
-        for (int k = 0; k < D; ++k) out.R[a * D + k] -= gp.N[a] * (P.t0[k] * lambda) * meas;
+        for (int k = 0; k < D; ++k) tu += gp.N[a] * ue[a * D + k] * (P.t0[k] * lambda);
+      out.energy -= tu * meas;
+      if (wantR)
+        for (int a = 0; a < nn; ++a)
+          for (int k = 0; k < D; ++k) out.R[a * D + k] -= gp.N[a] * (P.t0[k] * lambda) * meas;
     }
   }
 }

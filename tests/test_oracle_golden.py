@@ -287,3 +287,23 @@ def test_arc_length_two_dof_golden():
     assert abs(r["lam"] - 0.4877655071723067) < 1e-12
     # every step advances the load by the same arc length of (u, psi lambda): the constraint holds
     assert np.all(np.diff(r["lambdas"][1:]) > 0)
+
+
+@pytest.mark.parametrize("kind", [po.NEOHOOKE, po.SVK])
+def test_residual_is_derivative_of_energy_with_traction(kind):
+    """R = dPi/du with the traction term: Pi = sum psi dV - lambda sum t.u(face centre) |J_Gamma|
+    (traction.hh:71-104)."""
+    p = pb.hex_block((3, 2, 2), material=kind, jitter=0.08, seed=13, traction=(3.0, -2.0, 5.0))
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, 0, 2)
+    o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction)
+    rng = np.random.default_rng(6)
+    u = 0.02 * rng.standard_normal(p.ndof)
+    lam = 0.8
+    R = o.vector(u, lam, po.RAW)
+    h = 1e-6
+    top = np.nonzero(p.X[:, 2] > 1 - 1e-9)[0] * 3 + 2  # loaded-face DOFs carry the traction term
+    for j in np.concatenate([top[:3], rng.choice(p.ndof, 4, replace=False)]):
+        e = np.zeros(p.ndof)
+        e[j] = h
+        fd = (o.energy(u + e, lam) - o.energy(u - e, lam)) / (2 * h)
+        assert abs(fd - R[j]) <= 1e-6 * max(1.0, np.abs(R).max())
