@@ -17,12 +17,13 @@ FP64, FP32 = 0, 1
 HOST, DEVICE = 0, 1
 PCG_JACOBI_FP64, PCG_BJACOBI_FP32_IR = 0, 1
 PRED_CONSTANT, PRED_LINEAR, PRED_MLP = 0, 1, 2
+PK2, KIRCHHOFF, CAUCHY = 0, 1, 2
 KERNEL_ASSEMBLE_KR, KERNEL_ASSEMBLE_R, KERNEL_SPMV, KERNEL_MLP, KERNEL_GATHER_K = 0, 1, 2, 3, 4
 
 EXPORTED = [
     "ik_create", "ik_destroy", "ik_last_error", "ik_bad_element", "ik_size", "ik_reduced_size",
-    "ik_set_requirement", "ik_get_requirement", "ik_set_dirichlet_values", "ik_energy", "ik_vector", "ik_matrix",
-    "ik_assemble", "ik_matrix_csc", "ik_matrix_bsr",
+    "ik_set_requirement", "ik_get_requirement", "ik_set_dirichlet_values", "ik_energy", "ik_stress_at", "ik_vector",
+    "ik_matrix", "ik_assemble", "ik_matrix_csc", "ik_matrix_bsr",
     "ik_element", "ik_update_state", "ik_get_alpha", "ik_set_alpha", "ik_solve", "ik_newton_solve",
     "ik_load_control", "ik_path_following", "ik_mlp_num_params", "ik_mlp_set_params", "ik_mlp_get_params",
     "ik_mlp_set_load",
@@ -100,6 +101,7 @@ def lib():
         L.ik_set_requirement.argtypes = [P, P, D, I]
         L.ik_set_dirichlet_values.argtypes = [P, P, I]
         L.ik_energy.argtypes = [P, P]
+        L.ik_stress_at.argtypes = [P, I, P, P, I]
         L.ik_get_requirement.argtypes = [P, P, P, I]
         L.ik_vector.argtypes = [P, I, P, I]
         L.ik_matrix.argtypes = [P, I, I]

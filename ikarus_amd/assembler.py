@@ -122,6 +122,13 @@ class Device:
         self.check(capi.lib().ik_energy(self.h, C.byref(e)))
         return e.value
 
+    def stress_at(self, xi, kind=0):
+        """Per element, the PK2 (kind 0), Kirchhoff (1) or Cauchy (2) stress at local point xi, Voigt 6."""
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        out = np.zeros((self.problem.ne, 6))
+        self.check(capi.lib().ik_stress_at(self.h, int(kind), ptr(xi), ptr(out), capi.HOST))
+        return out
+
     def get_requirement(self):
         u = np.zeros(self.ndof)
         lam = C.c_double()

@@ -810,6 +810,99 @@ __global__ __launch_bounds__(256) void energyKernel(const double* __restrict__ X
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Stress at the local point xi of every element (NonLinearElastic::calculateAtImpl,
+// nonlinearelastic.hh:237-270): kind 0 PK2 S, 1 Kirchhoff F S F^T, 2 Cauchy F S F^T / det F, Voigt
+// [11,22,33,23,13,12] without shear factors. One thread per element.
+template <int KIND>
+__global__ __launch_bounds__(256) void stressKernel(const double* __restrict__ X, const double* __restrict__ U,
+                                                   const int32_t* __restrict__ conn, int64_t ne, double lam,
+                                                   double mu, double3 xi3, int kind, double* __restrict__ out,
+                                                   unsigned long long* __restrict__ err) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  const double xi[3] = {xi3.x, xi3.y, xi3.z};
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int a = 0; a < 8; ++a) {
+    const int64_t nd = conn[e * 8 + a];
+    double fb[3], dN[3];
+    for (int k = 0; k < 3; ++k) fb[k] = ((a >> k) & 1) ? xi[k] : 1.0 - xi[k];
+    for (int k = 0; k < 3; ++k) dN[k] = (((a >> k) & 1) ? 1.0 : -1.0) * fb[(k + 1) % 3] * fb[(k + 2) % 3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        J[i][j] += X[nd * 3 + i] * dN[j];
+        H[i][j] += U[nd * 3 + i] * dN[j];
+      }
+  }
+  const double dJ = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                    J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                    J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  double Ji[3][3];
+  Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / dJ;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / dJ;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / dJ;
+  Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / dJ;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / dJ;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / dJ;
+  Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / dJ;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / dJ;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / dJ;
+  double F[3][3], C[3][3], S[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      F[i][j] = (i == j ? 1.0 : 0.0) + H[i][0] * Ji[0][j] + H[i][1] * Ji[1][j] + H[i][2] * Ji[2][j];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+  if constexpr (KIND == IK_SVK) {
+    const double tr = 0.5 * (C[0][0] + C[1][1] + C[2][2] - 3.0);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) S[i][j] = (i == j ? lam * tr : 0.0) + mu * (C[i][j] - (i == j ? 1.0 : 0.0));
+  } else {
+    const double detC = C[0][0] * (C[1][1] * C[2][2] - C[1][2] * C[2][1]) -
+                        C[0][1] * (C[1][0] * C[2][2] - C[1][2] * C[2][0]) +
+                        C[0][2] * (C[1][0] * C[2][1] - C[1][1] * C[2][0]);
+    if (!(detC > 0.0)) {
+      atomicMin(err, (unsigned long long)e);
+      return;
+    }
+    double Ci[3][3];
+    Ci[0][0] = (C[1][1] * C[2][2] - C[1][2] * C[2][1]) / detC;
+    Ci[0][1] = (C[0][2] * C[2][1] - C[0][1] * C[2][2]) / detC;
+    Ci[0][2] = (C[0][1] * C[1][2] - C[0][2] * C[1][1]) / detC;
+    Ci[1][1] = (C[0][0] * C[2][2] - C[0][2] * C[2][0]) / detC;
+    Ci[1][2] = (C[0][2] * C[1][0] - C[0][0] * C[1][2]) / detC;
+    Ci[2][2] = (C[0][0] * C[1][1] - C[0][1] * C[1][0]) / detC;
+    Ci[1][0] = Ci[0][1];
+    Ci[2][0] = Ci[0][2];
+    Ci[2][1] = Ci[1][2];
+    const double lnJ = 0.5 * log(detC);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) S[i][j] = mu * ((i == j ? 1.0 : 0.0) - Ci[i][j]) + lam * lnJ * Ci[i][j];
+  }
+  double T[3][3];
+  if (kind == 0) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) T[i][j] = S[i][j];
+  } else {
+    const double dF = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+                      F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                      F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double t = 0.0;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) t += F[i][k] * S[k][l] * F[j][l];
+        T[i][j] = kind == 2 ? t / dF : t;
+      }
+  }
+  double* o = out + e * 6;
+  o[0] = T[0][0];
+  o[1] = T[1][1];
+  o[2] = T[2][2];
+  o[3] = T[1][2];
+  o[4] = T[0][2];
+  o[5] = T[0][1];
+}
+
 template <typename T, bool WK, bool WR>
 void launchElemMat(ik_handle* h, int64_t e0, int64_t count, bool tiled, const int32_t* rdest, T* Kebe,
                    double* Rebe, hipStream_t s = nullptr, T* K = nullptr, int mask = 0) {
@@ -841,6 +934,20 @@ static void checkErrFlag(ik_handle* h) {
     throw Error{IK_ENEGDET, "Determinant of right Cauchy Green tensor C must be greater than zero (element " +
                                 std::to_string(v) + ")"};
   }
+}
+
+void stressAt(ik_handle* h, int kind, const double xi[3], double* out) {
+  require(h->neas == 0, IK_ENOTIMPL, "stresses of EAS elements are not on the device");
+  require(kind >= 0 && kind <= 2, IK_EINVAL, "stress kind: IK_PK2, IK_KIRCHHOFF or IK_CAUCHY");
+  const double3 x = make_double3(xi[0], xi[1], xi[2]);
+  const unsigned g = (unsigned)((h->ne + 255) / 256);
+  if (h->material == IK_SVK)
+    hipLaunchKernelGGL(stressKernel<IK_SVK>, dim3(g), dim3(256), 0, h->stream, h->X.p, h->u.p, h->conn.p, h->ne,
+                       h->lam, h->mu, x, kind, out, h->errFlag.p);
+  else
+    hipLaunchKernelGGL(stressKernel<IK_NEOHOOKE>, dim3(g), dim3(256), 0, h->stream, h->X.p, h->u.p, h->conn.p,
+                       h->ne, h->lam, h->mu, x, kind, out, h->errFlag.p);
+  checkErrFlag(h);
 }
 
 double energy(ik_handle* h) {

@@ -402,6 +402,20 @@ int ik_energy(ik_handle* h, double* energy) {
   });
 }
 
+int ik_stress_at(ik_handle* h, int kind, const double xi[3], double* out, int where) {
+  return guard(h, [&] {
+    require(xi != nullptr && out != nullptr, IK_EINVAL, "null argument");
+    requireBound(h);
+    if (where == IK_DEVICE) {
+      ik::stressAt(h, kind, xi, out);
+    } else {
+      h->tmp.ensure((size_t)h->ne * 6);
+      ik::stressAt(h, kind, xi, h->tmp.p);
+      copyOut(h, out, h->tmp.p, h->ne * 6, IK_HOST);
+    }
+  });
+}
+
 int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where) {
   return guard(h, [&] {
     if (u) copyOut(h, u, h->u.p, h->ndof, where);

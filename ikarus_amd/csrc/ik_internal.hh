@@ -220,6 +220,7 @@ void axpy(ik_handle* h, double a, const double* x, double* y, int64_t n);  // y 
 double* partialsFor(ik_handle* h, int nb);
 double foldPartials(ik_handle* h, int nb);
 double energy(ik_handle* h);  // potential energy at the bound requirement (plain element)
+void stressAt(ik_handle* h, int kind, const double xi[3], double* out);  // out: device, ne x 6
 void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxIter, int* iters, double* relres);
 void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info);
 void residualBlock64(ik_handle* h, const double* b, const double* x, double* r);  // r = b - K x (FP64 K)

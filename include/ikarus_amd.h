@@ -42,6 +42,8 @@ enum { IK_PCG_JACOBI_FP64 = 0, IK_PCG_BJACOBI_FP32_IR = 1 };
 /* load-step predictors: constant (loadcontrol.inl:55-57), secant/linear extrapolation
  * (pathfollowingfunctions.hh:188-195 semantics), learned per-node MLP (no reference counterpart) */
 enum { IK_PRED_CONSTANT = 0, IK_PRED_LINEAR = 1, IK_PRED_MLP = 2 };
+/* stress measures of ik_stress_at (ResultTypes PK2Stress, kirchhoffStress, cauchyStress) */
+enum { IK_PK2 = 0, IK_KIRCHHOFF = 1, IK_CAUCHY = 2 };
 
 /* ---- problem description (replaces makeBasis + makeFE(skills(nonLinearElastic, eas, traction))
  *      + DirichletValues + makeSparseFlatAssembler: tests/src/testcantileverbeam.hh:102-151) ---- */
@@ -103,6 +105,12 @@ int ik_vector(ik_handle* h, int dbc, double* R, int where);
  * IK_ENOTIMPL. Point loads are not part of it (as for the reference's assembler without a scalar
  * manipulator). */
 int ik_energy(ik_handle* h, double* energy);
+
+/* NonLinearElastic::calculateAt<PK2Stress | kirchhoffStress | cauchyStress>(req, xi)
+ * (nonlinearelastic.hh:237-270) for every element at the bound requirement: out[e*6 + v], Voigt
+ * [11,22,33,23,13,12] without shear factors, xi in the reference cube [0,1]^3. Plain elements only
+ * (IK_ENOTIMPL for EAS). */
+int ik_stress_at(ik_handle* h, int kind, const double xi[3], double* out, int where);
 
 /* MatrixAssembler::matrix(req, stiffness, dbc) (interface.hh:430-440; simpleassemblers.inl:121-204).
  * Assembles into library-owned 3x3-block CSR storage on the device (FP64, or FP32 for IK_FP32). */

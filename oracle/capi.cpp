@@ -157,6 +157,15 @@ int ikor_vector(void* hp, const double* u, double lambda, int dbc, double* R) {
   });
 }
 
+// stresses at the local point xi of every element (3D plain elements), out[ne * 6]
+int ikor_stress_at(void* hp, const double* u, const double* xi, int kind, double* out) {
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] {
+    if (h->P.dim != 3 || h->P.neas != 0) throw std::runtime_error("stress_at: 3D plain elements only");
+    for (int e = 0; e < h->P.ne; ++e) stressAt3(h->P, e, u, xi, kind, out + (size_t)e * 6);
+  });
+}
+
 int ikor_energy(void* hp, const double* u, double lambda, double* energy) {
   auto* h = static_cast<Handle*>(hp);
   return guard([&] { *energy = solver(h).asmb.energy(u, lambda); });

@@ -680,6 +680,42 @@ void elementCompute(const Problem& P, int e, const double* uFull, double lambda,
   }
 }
 
+// Stress at the local point xi of a 3D plain element (NonLinearElastic::calculateAtImpl,
+// nonlinearelastic.hh:237-270): kind 0 PK2 S, 1 Kirchhoff F S F^T, 2 Cauchy F S F^T / det F; Voigt
+// [11,22,33,23,13,12] without shear factors (toVoigt(., false))
+inline void stressAt3(const Problem& P, int e, const double* uFull, const double xi[3], int kind, double out[6]) {
+  double Xe[24], ue[24];
+  for (int a = 0; a < 8; ++a) {
+    const int v = P.conn[(size_t)e * 8 + a];
+    for (int k = 0; k < 3; ++k) {
+      Xe[a * 3 + k] = P.X[(size_t)v * 3 + k];
+      ue[a * 3 + k] = uFull[(size_t)v * 3 + k];
+    }
+  }
+  GeoPoint<3> gp;
+  geoAt<3>(Xe, xi, gp);
+  double F[3][3], Ev[6], S[6];
+  glStrain<3>(gp, ue, F, Ev);
+  material3d(P.mat, Ev, S, nullptr, nullptr);
+  if (kind == 0) {
+    for (int v = 0; v < 6; ++v) out[v] = S[v];
+    return;
+  }
+  double Sm[3][3], T[3][3];
+  for (int v = 0; v < 6; ++v) Sm[kVoigt3[v][0]][kVoigt3[v][1]] = Sm[kVoigt3[v][1]][kVoigt3[v][0]] = S[v];
+  const double detF = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+                      F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                      F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) t += F[i][k] * Sm[k][l] * F[j][l];
+      T[i][j] = kind == 2 ? t / detF : t;
+    }
+  for (int v = 0; v < 6; ++v) out[v] = T[kVoigt3[v][0]][kVoigt3[v][1]];
+}
+
 inline void elementComputeDispatch(const Problem& P, int e, const double* u, double lambda, bool wantK, bool wantR,
                                    ElementOut& out, const double* alphaOverride = nullptr) {
   if (P.dim == 3)

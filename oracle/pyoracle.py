@@ -51,6 +51,7 @@ def lib():
         L.ikor_matrix.argtypes = [P, P, D, I, P]
         L.ikor_vector.argtypes = [P, P, D, I, P]
         L.ikor_energy.argtypes = [P, P, D, P]
+        L.ikor_stress_at.argtypes = [P, P, P, I, P]
         L.ikor_element.argtypes = [P, I, P, D, P, P, P, P, P]
         L.ikor_update_state.argtypes = [P, P, D, P]
         L.ikor_cg.argtypes = [P, P, D, P, P, D, I, I, P, P]
@@ -177,6 +178,14 @@ class Oracle:
         R = np.zeros(n)
         _check(lib().ikor_vector(self.h, _p(u), float(lam), dbc, _p(R)))
         return R
+
+    def stress_at(self, u, xi, kind=0):
+        """Per element, the PK2 (kind 0), Kirchhoff (1) or Cauchy (2) stress at local point xi, Voigt 6."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        out = np.zeros((self.ne, 6))
+        _check(lib().ikor_stress_at(self.h, _p(u), _p(xi), int(kind), _p(out)))
+        return out
 
     def energy(self, u, lam):
         u = np.ascontiguousarray(u, dtype=np.float64)

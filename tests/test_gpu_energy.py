@@ -50,3 +50,31 @@ def test_energy_rejects_eas():
     with pytest.raises(RuntimeError):
         dev.energy()
     dev.close()
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_stress_at_points_matches_oracle(material):
+    """PK2, Kirchhoff and Cauchy stresses at local points (nonlinearelastic.hh:237-270) against the oracle,
+    whose stresses are pinned by the closed form of a homogeneous deformation. Bar 1e-12 relative."""
+    p = pb.hex_block((4, 3, 3), material=material, jitter=0.08, seed=53)
+    print(f"case: stresses material={material}", flush=True)
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, 0, 2)
+    u = 0.03 * np.random.default_rng(4).standard_normal(p.ndof)
+    dev = ia.Device(p)
+    dev.set_requirement(u, 0.5)
+    for xi in ([0.5, 0.5, 0.5], [0.2113248654051871, 0.7886751345948129, 0.5], [0.0, 1.0, 0.3]):
+        for kind in (capi.PK2, capi.KIRCHHOFF, capi.CAUCHY):
+            s = dev.stress_at(xi, kind)
+            so = o.stress_at(u, xi, kind)
+            assert np.abs(s - so).max() <= 1e-12 * np.abs(so).max()
+    dev.close()
+
+
+def test_stress_at_rejects_eas():
+    p = pb.hex_block((2, 2, 2), eas=21, seed=3)
+    print("case: stresses EAS", flush=True)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    with pytest.raises(RuntimeError):
+        dev.stress_at([0.5, 0.5, 0.5])
+    dev.close()

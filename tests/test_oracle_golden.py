@@ -307,3 +307,29 @@ def test_residual_is_derivative_of_energy_with_traction(kind):
         e[j] = h
         fd = (o.energy(u + e, lam) - o.energy(u - e, lam)) / (2 * h)
         assert abs(fd - R[j]) <= 1e-6 * max(1.0, np.abs(R).max())
+
+
+@pytest.mark.parametrize("kind", [po.NEOHOOKE, po.SVK])
+def test_stress_at_points_closed_form(kind):
+    """Under a homogeneous deformation u = (F - I) X every point of a jittered mesh carries the same F, so
+    the PK2 stress is the material law's closed form (neohooke.hh: S = mu (I - C^-1) + lambda ln J C^-1;
+    svk.hh: S = lambda tr E I + 2 mu E) and Kirchhoff / Cauchy are F S F^T and F S F^T / det F
+    (nonlinearelastic.hh:237-270)."""
+    X, conn = po.structured_mesh([3, 2, 2], [1.0, 1.0, 1.0], jitter=0.1, seed=3)
+    lam, mu = po.lame(1000.0, 0.3)
+    o = po.Oracle(X, conn, kind, lam, mu, 0, 2)
+    F = np.array([[1.05, 0.02, -0.03], [0.01, 0.97, 0.04], [-0.02, 0.03, 1.02]])
+    u = (X @ (F - np.eye(3)).T).ravel()
+    C = F.T @ F
+    if kind == po.NEOHOOKE:
+        Ci = np.linalg.inv(C)
+        S = mu * (np.eye(3) - Ci) + lam * np.log(np.sqrt(np.linalg.det(C))) * Ci
+    else:
+        E = 0.5 * (C - np.eye(3))
+        S = lam * np.trace(E) * np.eye(3) + 2 * mu * E
+    tau = F @ S @ F.T
+    voigt = lambda A: np.array([A[0, 0], A[1, 1], A[2, 2], A[1, 2], A[0, 2], A[0, 1]])
+    for xi in ([0.5, 0.5, 0.5], [0.1, 0.8, 0.3], [1.0, 0.0, 0.5]):
+        for k, ref in ((0, S), (1, tau), (2, tau / np.linalg.det(F))):
+            out = o.stress_at(u, xi, k)
+            assert np.allclose(out, voigt(ref)[None, :], rtol=1e-12, atol=1e-12 * np.abs(ref).max())
