@@ -310,3 +310,50 @@ def test_pattern_build_independent_of_thread_count(monkeypatch):
     (c1, R1, l1), (c5, R5, l5) = out
     assert all(np.array_equal(a, b) for a, b in zip(c1, c5))
     assert np.array_equal(R1, R5) and l1 == l5
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_tangent_is_derivative_of_residual(material):
+    """The device's own consistency, independent of the oracle (checkfebyautodiff.hh:19-115,
+    testnonlinearelasticity.hh:194-246): the RAW tangent equals the central difference of the RAW residual,
+    is symmetric, and at u = 0 has the 6 rigid-body modes of the free body in its null space."""
+    import scipy.sparse as sp
+    p = pb.hex_block((3, 3, 2), material=material, jitter=0.08, seed=61, traction=(2.0, -1.0, 3.0))
+    print(f"case: tangent consistency material={material}", flush=True)
+    dev = ia.Device(p)
+    n = p.ndof
+    rng = np.random.default_rng(8)
+    u = 0.02 * rng.standard_normal(n)
+    lam = 0.6
+
+    def tangent(uu):
+        dev.set_requirement(uu, lam)
+        dev.matrix(capi.DBC_RAW)
+        outer, inner, vals = dev.csc(capi.DBC_RAW)
+        return sp.csc_matrix((vals, inner, outer), shape=(n, n)).toarray()
+
+    K = tangent(u)
+    assert np.abs(K - K.T).max() <= 1e-13 * np.abs(K).max()
+    h = 1e-6
+    for j in rng.choice(n, 8, replace=False):
+        e = np.zeros(n)
+        e[j] = h
+        dev.set_requirement(u + e, lam)
+        Rp = dev.vector(capi.DBC_RAW)
+        dev.set_requirement(u - e, lam)
+        Rm = dev.vector(capi.DBC_RAW)
+        assert np.abs((Rp - Rm) / (2 * h) - K[:, j]).max() <= 1e-6 * np.abs(K).max()
+    K0 = tangent(np.zeros(n))
+    X = p.X
+    modes = []
+    for k in range(3):
+        t = np.zeros((X.shape[0], 3))
+        t[:, k] = 1.0
+        modes.append(t.ravel())
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        r = np.zeros((X.shape[0], 3))
+        r[:, a], r[:, b] = -X[:, b], X[:, a]
+        modes.append(r.ravel())
+    for m in modes:
+        assert np.linalg.norm(K0 @ m) <= 1e-12 * np.abs(K0).max() * np.linalg.norm(m)
+    dev.close()
