@@ -57,6 +57,28 @@ def test_inhomogeneous_dirichlet_load_control(material, eas):
     assert np.linalg.norm(ug - r["u"]) / np.linalg.norm(r["u"]) < 1e-8
 
 
+def test_inhomogeneous_dirichlet_mixed_precision():
+    """The mixed form (FP32 tangent and PCG, FP64 refinement to 1e-10) on the same path: same Newton
+    iterations as the oracle and displacements within 1e-8."""
+    p, ubar = _stretch_problem(capi.NEOHOOKE, 0)
+    print("case: IDBC load control, mixed precision", flush=True)
+    o = _oracle_for(p)
+    o.set_ubar(ubar)
+    r = o.load_control(4, 0.0, 1.0, tol=1e-8)
+    dev = ia.Device(p)
+    dev.set_dirichlet_values(ubar)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.LCConfig()
+    cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 4, 0.0, 1.0, capi.PRED_CONSTANT
+    s = ia.nr_settings(tol=1e-8, precision=capi.FP32,
+                       linear=ia.solver_opts(kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+    info, iters, res = dev.load_control(cfg, s)
+    ug, _ = dev.get_requirement()
+    dev.close()
+    assert info.success and list(iters) == list(r["iterations"])
+    assert np.linalg.norm(ug - r["u"]) / np.linalg.norm(r["u"]) < 1e-8
+
+
 def test_inhomogeneous_dirichlet_rejects_free_dofs():
     p, ubar = _stretch_problem(capi.NEOHOOKE, 0, n=(2, 2, 2))
     print("case: IDBC on a free DOF", flush=True)

@@ -51,6 +51,37 @@ def test_arc_length_matches_oracle(material):
     assert np.all(rn[1:] <= 1e-8)
 
 
+def test_arc_length_mixed_precision():
+    """The bordered solves by the mixed form (FP32 tangent and PCG, FP64 refinement to 1e-10): same
+    iterations as the oracle, the final state in FP64 equilibrium (oracle residual <= 2e-8), load factors
+    and displacements within 1e-6 (the tolerance region's extent, see below)."""
+    p = pb.hex_block((4, 3, 3), jitter=0.05, seed=41, mag_range=(20.0, 30.0))
+    print("case: arc length, mixed precision", flush=True)
+    o = _oracle_for(p)
+    res = lambda u, lam: o.vector(u, lam, po.FULL)
+    jac = lambda u, lam: o.matrix_dense(u, lam, po.FULL)
+    z = np.zeros(p.ndof)
+    s = 0.25 * np.sqrt(2.0) * np.linalg.norm(np.linalg.solve(jac(z, 1.0), -res(z, 1.0)))
+    r = pf.arc_length(res, jac, z, steps=4, step_size=s)
+    dev = ia.Device(p)
+    dev.set_requirement(z, 0.0)
+    cfg = capi.PFConfig()
+    cfg.steps, cfg.step_size = 4, s
+    st = ia.nr_settings(tol=1e-8, precision=capi.FP32,
+                        linear=ia.solver_opts(kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+    info, iters, rn, lams = dev.path_following(cfg, st)
+    ug, lam = dev.get_requirement()
+    dev.close()
+    assert info.success and list(iters) == list(r["iterations"])
+    # The FP32 tangent takes slightly different Newton steps, so the two runs stop at different points of the
+    # region sqrt(|R|^2 + f^2) <= tol. There lambda is fixed only to about tol / (df/dDl) (df/dDl = psi^2 Dl
+    # / root ~ 0.05 here): compare the equilibrium itself, then the states at that scale.
+    assert np.linalg.norm(res(ug, lam)) <= 2e-8
+    dl = np.abs(lams - np.array(r["lambdas"])).max()
+    assert dl < 1e-6, dl
+    assert np.linalg.norm(ug - r["u"]) / np.linalg.norm(r["u"]) < 1e-6  # the same shift along the path
+
+
 def test_arc_length_rejects_eas():
     p = pb.hex_block((2, 2, 2), eas=21, seed=3)
     print("case: arc length EAS", flush=True)
