@@ -27,6 +27,8 @@
 // each chunk completes.
 #include "ik_internal.hh"
 
+#include <type_traits>
+
 namespace ik {
 namespace {
 
@@ -136,6 +138,16 @@ __device__ inline double dpp(double v) {
 }
 constexpr int DPP_XOR1 = 0xB1;   // quad_perm [1,0,3,2]
 constexpr int DPP_XOR2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int DPP_XOR3 = 0x1B;   // quad_perm [3,2,1,0]
+constexpr int DPP_HALF_MIRROR = 0x141;  // row_half_mirror: lane q of each group of 8 reads lane 7 - q
+
+// value of lane (lane ^ BIT), BIT = 1, 2 or 4, by DPP moves (q ^ 4 = (7 - q) ^ 3 within a group of 8)
+template <int BIT>
+__device__ inline double laneXor(double v) {
+  if constexpr (BIT == 1) return dpp<DPP_XOR1>(v);
+  else if constexpr (BIT == 2) return dpp<DPP_XOR2>(v);
+  else return dpp<DPP_XOR3>(dpp<DPP_HALF_MIRROR>(v));
+}
 
 // upper-block index of (a, b), a <= b: row-major over the upper triangle of the 8x8 node pairs
 __host__ __device__ constexpr int upperBlock(int a, int b) { return 8 * a - (a * (a - 1)) / 2 + (b - a); }
@@ -460,14 +472,16 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
         if (d >= 0) {
           Kebe[(int64_t)d * 9 + tt] = (TK)v;
         } else {
-          // complete K block: the FULL treatment as in the gather, then the block and its mirror
+          // complete K block: the FULL treatment as in the gather, then the block and its mirror, stored
+          // non-temporally (K is read back only by later passes, long after it has left the caches; the
+          // staged sums above stay cached for the gather)
           const int32_t p = -d - 1, pt = gt[g];
           if (maskDbc) {
             const unsigned m = gm[g];
             if ((m >> tt) & 1u) v = (k == l && ((m >> (9 + k)) & 1u)) ? 1.0 : 0.0;
           }
-          K[(int64_t)p * 9 + tt] = (TK)v;
-          if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = (TK)v;
+          __builtin_nontemporal_store((TK)v, &K[(int64_t)p * 9 + tt]);
+          if (pt != p) __builtin_nontemporal_store((TK)v, &K[(int64_t)pt * 9 + 3 * l + k]);
         }
       }
     }
@@ -475,18 +489,23 @@ __global__ __launch_bounds__(kTileThreads, 3) void hex8Element(const double* __r
 }
 
 // Residual-only element kernel (R_e[3a+k] = sum_gp (F S dV g_a)_k): lane (el, gp) = 8 elements x 8 Gauss
-// points per wavefront, nodes staged in LDS. With dN_a the reference-cube gradients, J = sum_a x_a dN_a^T and
-// H = sum_a u_a dN_a^T, the point needs F = I + H J^-1 and, per corner, F S dV g_a = Q dN_a with
-// Q = F S dV J^-T (g_a = J^-T dN_a is never formed). The eight Gauss-point lanes of an element then
-// reduce-scatter their 8 x 3 partials (exchanges at lane distance 4, 2, 1, a fixed order), which leaves
-// corner q's three sums on lane q.
+// points per wavefront. The trilinear maps x(xi), u(xi) are written in monomials, x = sum_S c_S m_S(xi) with
+// m_S = prod_{d in S} xi_d over the subsets S of {0,1,2} (bit d of S = dimension d, as for the corners);
+// the coefficients follow from the corner values by the Moebius transform over the corner bits,
+// c_S = sum_{T subset S} (-1)^|S\T| x_T: the element's lane q loads corner q and, after one exchange step
+// per dimension, holds c_q. J = dx/dxi and H = du/dxi at a point then cost three FMAs per entry, and the
+// point needs F = I + H J^-1 and, per corner, F S dV g_a = Q dN_a with Q = F S dV J^-T (g_a = J^-T dN_a is
+// never formed). The corner forces use the same basis backwards: N_a = sum_{S superset of a} (-1)^|S\a| m_S,
+// so with G_S = Q grad m_S summed over the element's points, R_a = sum_{S superset of a} (-1)^|S\a| G_S.
+// The eight Gauss-point lanes reduce-scatter their G_S (exchanges at lane distance 4, 2, 1, a fixed order),
+// which leaves G_q on lane q, and three more exchange steps turn G_q into corner q's three sums.
 template <int KIND>
 __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X, const double* __restrict__ U,
                                                    const int32_t* __restrict__ conn, int64_t e0, int64_t count,
                                                    double lam, double mu, const int32_t* __restrict__ rdest,
                                                    double* __restrict__ Rebe, const uint8_t* __restrict__ fbits,
                                                    double3 trac, unsigned long long* __restrict__ err) {
-  __shared__ double XU[8][8][6];
+  __shared__ double CF[8][8][6];  // monomial coefficients of x (0..2) and u (3..5) per element
   const int lane     = threadIdx.x;
   const int el       = lane >> 3, q = lane & 7;
   const int64_t base = (int64_t)blockIdx.x * 8;
@@ -494,28 +513,42 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
   const bool valid   = el < nvalid;
   const int64_t e    = e0 + base + (valid ? el : nvalid - 1);
   {
+    double c[6];
     const int64_t nd = conn[e * 8 + q];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      XU[el][q][k]     = X[nd * 3 + k];
-      XU[el][q][3 + k] = U[nd * 3 + k];
+      c[k]     = X[nd * 3 + k];
+      c[3 + k] = U[nd * 3 + k];
     }
+    // Moebius transform: the exchange partners (lane ^ 1, 2, 4) are lanes of the same element
+    auto step = [&](auto bitc) {
+      constexpr int B   = decltype(bitc)::value;
+      const double sub  = (q & B) ? 1.0 : 0.0;  // c - 1 * o exactly as c - o, c - 0 * o = c
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[k] = fma(-sub, laneXor<B>(c[k]), c[k]);
+    };
+    step(std::integral_constant<int, 1>{});
+    step(std::integral_constant<int, 2>{});
+    step(std::integral_constant<int, 4>{});
+#pragma unroll
+    for (int k = 0; k < 6; ++k) CF[el][q][k] = c[k];
   }
   __syncthreads();
   double xi[3];
   gaussPoint(q, xi);
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  // d/dxi0 = c1 + c3 xi1 + (c5 + c7 xi1) xi2, d/dxi1 = c2 + c3 xi0 + (c6 + c7 xi0) xi2,
+  // d/dxi2 = c4 + c5 xi0 + (c6 + c7 xi0) xi1
+  double J[3][3], H[3][3];
+  {
+    const double(*cf)[6] = CF[el];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    double dN[3];
-    dShape(a, xi, dN);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        J[i][j] += XU[el][a][i] * dN[j];
-        H[i][j] += XU[el][a][3 + i] * dN[j];
-      }
+    for (int k = 0; k < 6; ++k) {
+      const double c70 = fma(cf[7][k], xi[0], cf[6][k]);
+      double* row      = k < 3 ? J[k] : H[k - 3];
+      row[0]           = fma(fma(cf[7][k], xi[1], cf[5][k]), xi[2], fma(cf[3][k], xi[1], cf[1][k]));
+      row[1]           = fma(c70, xi[2], fma(cf[3][k], xi[0], cf[2][k]));
+      row[2]           = fma(c70, xi[1], fma(cf[5][k], xi[0], cf[4][k]));
+    }
   }
   const double detJ = det3(J);
   double Ji[3][3];
@@ -541,31 +574,47 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
 #pragma unroll
     for (int j = 0; j < 3; ++j) Q[k][j] = T[0] * Ji[j][0] + T[1] * Ji[j][1] + T[2] * Ji[j][2];
   }
-  // reduce-scatter over the element's eight lanes: keep the half of the corners selected by this lane's bit,
-  // add the partner's partials of that half
-  const bool b2 = (q >> 2) & 1, b1 = (q >> 1) & 1, b0 = q & 1;
-  double n4[4][3];  // corners j + 4 b2
+  // G_S = Q grad m_S at this point (G_0 = 0)
+  const double x01 = xi[0] * xi[1], x02 = xi[0] * xi[2], x12 = xi[1] * xi[2];
+  double G[8][3];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    double dlo[3], dhi[3];
-    dShape(j, xi, dlo);
-    dShape(j + 4, xi, dhi);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double rlo = Q[k][0] * dlo[0] + Q[k][1] * dlo[1] + Q[k][2] * dlo[2];
-      const double rhi = Q[k][0] * dhi[0] + Q[k][1] * dhi[1] + Q[k][2] * dhi[2];
-      n4[j][k]         = (b2 ? rhi : rlo) + __shfl_xor(b2 ? rlo : rhi, 4);
-    }
+  for (int k = 0; k < 3; ++k) {
+    G[0][k] = 0.0;
+    G[1][k] = Q[k][0];
+    G[2][k] = Q[k][1];
+    G[4][k] = Q[k][2];
+    G[3][k] = Q[k][0] * xi[1] + Q[k][1] * xi[0];
+    G[5][k] = Q[k][0] * xi[2] + Q[k][2] * xi[0];
+    G[6][k] = Q[k][1] * xi[2] + Q[k][2] * xi[1];
+    G[7][k] = Q[k][0] * x12 + Q[k][1] * x02 + Q[k][2] * x01;
   }
-  double n2[2][3];  // corners j + 2 b1 + 4 b2
+  // reduce-scatter over the element's eight lanes: keep the half of the monomials selected by this lane's
+  // bit, add the partner's partials of that half
+  const bool b2 = (q >> 2) & 1, b1 = (q >> 1) & 1, b0 = q & 1;
+  double n4[4][3];  // monomials j + 4 b2
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      n4[j][k] = (b2 ? G[j + 4][k] : G[j][k]) + laneXor<4>(b2 ? G[j][k] : G[j + 4][k]);
+  double n2[2][3];  // monomials j + 2 b1 + 4 b2
 #pragma unroll
   for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      n2[j][k] = (b1 ? n4[j + 2][k] : n4[j][k]) + __shfl_xor(b1 ? n4[j][k] : n4[j + 2][k], 2);
-  double r[3];  // corner q
+      n2[j][k] = (b1 ? n4[j + 2][k] : n4[j][k]) + laneXor<2>(b1 ? n4[j][k] : n4[j + 2][k]);
+  double r[3];  // monomial q, then corner q
 #pragma unroll
-  for (int k = 0; k < 3; ++k) r[k] = (b0 ? n2[1][k] : n2[0][k]) + __shfl_xor(b0 ? n2[0][k] : n2[1][k], 1);
+  for (int k = 0; k < 3; ++k) r[k] = (b0 ? n2[1][k] : n2[0][k]) + laneXor<1>(b0 ? n2[0][k] : n2[1][k]);
+  auto back = [&](auto bitc) {
+    constexpr int B  = decltype(bitc)::value;
+    const double sub = (q & B) ? 0.0 : 1.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = fma(-sub, laneXor<B>(r[k]), r[k]);
+  };
+  back(std::integral_constant<int, 1>{});
+  back(std::integral_constant<int, 2>{});
+  back(std::integral_constant<int, 4>{});
   if (valid) {
     const unsigned bits = fbits ? fbits[e] : 0u;
     const double tk[3]  = {trac.x, trac.y, trac.z};
@@ -583,7 +632,11 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
 // tile), so consecutive threads read consecutive values. The entry's sum is treated with FULL (rows/columns
 // of constrained DOFs zero, unit diagonal; simpleassemblers.inl:152-169) and written to the block and,
 // transposed, to its mirror block below the diagonal (K and the Dirichlet treatment are symmetric).
-template <typename T>
+// NT: the staged values are read once and K is written once, both non-temporally. For the tiled plain
+// element at 64^3 this took the gather from 149 to 139 us and the element and residual passes that
+// follow it by 12 and 7 us (no dirty K lines left behind in the caches); the EAS element's per-element
+// staging gathered that way took 632 instead of 519 us at 96x96x48, so that path keeps cached accesses.
+template <typename T, bool NT>
 __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segptr, const int32_t* __restrict__ upBlk,
                                                const int32_t* __restrict__ upT, const uint16_t* __restrict__ upMask,
                                                int maskDbc, const T* __restrict__ Kst, int64_t q0, int64_t q1,
@@ -602,7 +655,10 @@ __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segpt
   for (; c < c1; c += 8) {
     T v[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = c + j < c1 ? Kst[(int64_t)(c + j) * 9 + t] : T(0);
+    for (int j = 0; j < 8; ++j) {
+      const T* src = &Kst[(int64_t)(c + j) * 9 + t];
+      v[j]         = c + j < c1 ? (NT ? __builtin_nontemporal_load(src) : *src) : T(0);
+    }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
       if (c + j < c1) acc += v[j];
@@ -612,9 +668,14 @@ __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segpt
     const unsigned m = upMask[q];
     if ((m >> t) & 1u) acc = (k == l && ((m >> (9 + k)) & 1u)) ? T(1) : T(0);
   }
-  K[(int64_t)p * 9 + t] = acc;
-  const int32_t pt      = upT[q];
-  if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = acc;
+  const int32_t pt = upT[q];
+  if constexpr (NT) {
+    __builtin_nontemporal_store(acc, &K[(int64_t)p * 9 + t]);
+    if (pt != p) __builtin_nontemporal_store(acc, &K[(int64_t)pt * 9 + 3 * l + k]);
+  } else {
+    K[(int64_t)p * 9 + t] = acc;
+    if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = acc;
+  }
 }
 
 // R gather: thread per DOF over node rows [r0, r1); the node's staged corner contributions occupy the
@@ -982,8 +1043,13 @@ template <typename T>
 static void launchGatherK(ik_handle* h, int64_t r0, int64_t r1, int mask, const T* Kebe, T* K, hipStream_t s) {
   const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
   if (q1 <= q0) return;  // every block of these rows was written by the element kernel
-  hipLaunchKernelGGL(gatherK<T>, dim3((unsigned)(((q1 - q0) * 9 + 255) / 256)), dim3(256), 0, s, h->segptr.p,
-                     h->upBlk.p, h->upT.p, h->upMask.p, mask, Kebe, q0, q1, K);
+  const dim3 grid((unsigned)(((q1 - q0) * 9 + 255) / 256));
+  if (h->neas)
+    hipLaunchKernelGGL((gatherK<T, false>), grid, dim3(256), 0, s, h->segptr.p, h->upBlk.p, h->upT.p, h->upMask.p,
+                       mask, Kebe, q0, q1, K);
+  else
+    hipLaunchKernelGGL((gatherK<T, true>), grid, dim3(256), 0, s, h->segptr.p, h->upBlk.p, h->upT.p, h->upMask.p,
+                       mask, Kebe, q0, q1, K);
 }
 
 // residual-only element pass over the mesh on stream s (the EAS pass always runs on the main stream)
