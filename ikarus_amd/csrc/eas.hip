@@ -104,13 +104,21 @@ __device__ inline void jacobianAt(const double (*X)[3], const double xi[3], doub
   }
 }
 
+// value of lane `lane` (uniform) of a 64-bit register, as two 32-bit lane reads into scalar registers
+__device__ inline double readLane(double x, int lane) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo          = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi          = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Voigt index of the symmetric pair (i, j): 00 11 22 12 02 01
 __host__ __device__ constexpr int v6(int i, int j) {
   return i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5));
 }
 
 // One 64-lane workgroup (one wave) per element. The LDS (13.6 KB; 11.2 KB for the residual alone) holds two
-// unions: the K_uu inputs (later the parked K_uu blocks and the elimination's pivot column) and the point data of
+// unions: the K_uu inputs (later the parked K_uu blocks) and the point data of
 // the D / L / Rt loop (later the condensation operands).
 template <bool WANT_K>
 struct EasLds {
@@ -129,7 +137,6 @@ struct EasLds {
     } k;
     struct {
       double Kuu[WANT_K ? 36 : 1][9];  // K_uu upper blocks while the elimination runs
-      double piv[NE];     // pivot column of the elimination step
     } q;
   } uk;
   union {
@@ -498,23 +505,23 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
 
   // (3) Y = D^-1 [L | Rt] by Gauss-Jordan elimination on the augmented 21 x 46 matrix (D is SPD: no pivoting,
   // like the LDL^T it replaces; enhancedassumedstrains.hh inverts D, :243,294,344). Lane c < 46 keeps column
-  // c in registers; each step the pivot column is broadcast through LDS.
+  // c in registers; each step every lane reads the pivot column from lane j's registers (readlane). The
+  // LDS broadcast this replaced (lane j stores, barrier, every lane loads) made the kernel LDS-issue bound:
+  // 7.98 -> 5.69 ms per R+K pass at 96x96x48, results bitwise the same.
   double col[NE];
 #pragma unroll
   for (int i = 0; i < NE; ++i)
     col[i] = t < NE ? sc.A[i][t] : (t < NE + 24 ? sc.Lm[i][t - NE] : (t == NE + 24 ? sc.rt[i] : 0.0));
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
-    if (t == j)
+    double piv[NE];  // lane j's column, read into scalar registers (no LDS round trip, no barrier)
 #pragma unroll
-      for (int i = 0; i < NE; ++i) sq.piv[i] = col[i];
-    __syncthreads();
-    const double f = col[j] / sq.piv[j];
+    for (int i = 0; i < NE; ++i) piv[i] = readLane(col[i], j);
+    const double f = col[j] / piv[j];
 #pragma unroll
     for (int i = 0; i < NE; ++i)
-      if (i != j) col[i] -= sq.piv[i] * f;
+      if (i != j) col[i] -= piv[i] * f;
     col[j] = f;
-    __syncthreads();
   }
   // lanes NE..NE+23 hold Y = D^-1 L (column c = t - NE), lane NE+24 holds y = D^-1 Rt; cache them for the
   // alpha update (layout per element: Y 21 x 24 row-major, then y)
