@@ -393,12 +393,21 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   const int lc = t % 24, lh = t / 24;
   double lacc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int rK = kRow[t < NE ? t : 0];
+  // the one column of T0 each lane's role reads at every point: w (C T0^-1), p (T0^-T C T0^-1), w (T0^-T S)
+  int chP = 0, chQ = 0;
+  if (t >= 43) triDecode(t >= 48 ? t - 48 : t - 27, 6, chP, chQ);
+  double t0c[6];
+  {
+    const int tc = t < 36 ? t % 6 : (t >= 43 ? chP : (t >= 37 ? t - 37 : 0));
+#pragma unroll
+    for (int u = 0; u < 6; ++u) t0c[u] = s.T0[u][tc];
+  }
   for (int g = 0; g < 8; ++g) {
     if (t < 36) {
       const int v = t / 6, w = t % 6;
       double acc = 0.0;
 #pragma unroll
-      for (int u = 0; u < 6; ++u) acc += sg.C[g][sym6(v, u)] * s.T0[u][w];
+      for (int u = 0; u < 6; ++u) acc += sg.C[g][sym6(v, u)] * t0c[u];
       sg.CT[v][w] = acc;
     }
     __syncthreads();
@@ -413,20 +422,18 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
       for (int l = 0; l < 3; ++l)
         sg.Z[w][3 * b + l] = sg.F[g][3 * l] * h[0] + sg.F[g][3 * l + 1] * h[1] + sg.F[g][3 * l + 2] * h[2];
     }
-    if (t >= 48 || (t >= 43 && t < 48)) {
-      int p, q;
-      triDecode(t >= 48 ? t - 48 : t - 27, 6, p, q);
+    if (t >= 43) {
       double acc = 0.0;
 #pragma unroll
-      for (int v = 0; v < 6; ++v) acc += s.T0[v][p] * sg.CT[v][q];
-      sg.Ch[p][q] = acc;
-      sg.Ch[q][p] = acc;
+      for (int v = 0; v < 6; ++v) acc += t0c[v] * sg.CT[v][chQ];
+      sg.Ch[chP][chQ] = acc;
+      sg.Ch[chQ][chP] = acc;
     }
     if (t >= 37 && t < 43) {
       const int w = t - 37;
       double acc = 0.0;
 #pragma unroll
-      for (int v = 0; v < 6; ++v) acc += s.T0[v][w] * sk.S[g][v];
+      for (int v = 0; v < 6; ++v) acc += t0c[v] * sk.S[g][v];
       sg.TS[w] = acc;
     }
     __syncthreads();
