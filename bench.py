@@ -389,9 +389,17 @@ def config_eas(args):
            "assembly": {"ms_per_step": round(dt / args.c4_steps * 1e3, 3),
                         "Melem_per_s": round(p.ne * args.c4_steps / dt / 1e6, 3)}}
     if args.c4_load_steps > 0:
-        r, _ = newton_path(p, "fp64", args.c4_load_steps)
+        r, u64 = newton_path(p, "fp64", args.c4_load_steps)
         r["seconds_per_load_step"] = round(r["seconds"] / args.c4_load_steps, 4)
         out["newton_fp64"] = r
+        # the mixed form on the EAS element: FP32 condensed tangent (gathered from the FP64 element pass),
+        # FP32 block-Jacobi PCG with FP64 refinement, inexact Newton, secant predictor
+        log("[bench] c4 newton mixed_secant ...")
+        r, u32 = newton_path(p, "mixed_secant", args.c4_load_steps)
+        r["seconds_per_load_step"] = round(r["seconds"] / args.c4_load_steps, 4)
+        r["speedup_vs_fp64"] = round(out["newton_fp64"]["seconds"] / r["seconds"], 3)
+        r["rel_diff_u_vs_fp64"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+        out["newton_mixed_secant"] = r
     return out
 
 

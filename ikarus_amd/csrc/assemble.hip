@@ -636,10 +636,11 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
 // element at 64^3 this took the gather from 149 to 139 us and the element and residual passes that
 // follow it by 12 and 7 us (no dirty K lines left behind in the caches); the EAS element's per-element
 // staging gathered that way took 632 instead of 519 us at 96x96x48, so that path keeps cached accesses.
-template <typename T, bool NT>
+// TS: staging type, summed in TS; T: matrix type (the EAS element stages FP64 also for an FP32 matrix).
+template <typename TS, typename T, bool NT>
 __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segptr, const int32_t* __restrict__ upBlk,
                                                const int32_t* __restrict__ upT, const uint16_t* __restrict__ upMask,
-                                               int maskDbc, const T* __restrict__ Kst, int64_t q0, int64_t q1,
+                                               int maskDbc, const TS* __restrict__ Kst, int64_t q0, int64_t q1,
                                                T* __restrict__ K) {
   const uint32_t loc = blockIdx.x * 256u + threadIdx.x;  // (q - q0) * 9 + t
   const uint32_t ql   = loc / 9u;
@@ -649,15 +650,15 @@ __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segpt
   const int k      = t / 3, l = t - 3 * (t / 3);
   int32_t c        = segptr[q];
   const int32_t c1 = segptr[q + 1];
-  T acc            = T(0);
+  TS acc           = TS(0);
   // segments hold one block per element (or tile) sharing the two nodes (8 at most on hexahedral meshes
   // whose nodes have 8 elements); the loads of a group of 8 are issued together, then summed in order
   for (; c < c1; c += 8) {
-    T v[8];
+    TS v[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const T* src = &Kst[(int64_t)(c + j) * 9 + t];
-      v[j]         = c + j < c1 ? (NT ? __builtin_nontemporal_load(src) : *src) : T(0);
+      const TS* src = &Kst[(int64_t)(c + j) * 9 + t];
+      v[j]          = c + j < c1 ? (NT ? __builtin_nontemporal_load(src) : *src) : TS(0);
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
@@ -666,15 +667,16 @@ __global__ __launch_bounds__(256) void gatherK(const int32_t* __restrict__ segpt
   const int32_t p = upBlk[q];
   if (maskDbc) {
     const unsigned m = upMask[q];
-    if ((m >> t) & 1u) acc = (k == l && ((m >> (9 + k)) & 1u)) ? T(1) : T(0);
+    if ((m >> t) & 1u) acc = (k == l && ((m >> (9 + k)) & 1u)) ? TS(1) : TS(0);
   }
   const int32_t pt = upT[q];
+  const T val      = (T)acc;
   if constexpr (NT) {
-    __builtin_nontemporal_store(acc, &K[(int64_t)p * 9 + t]);
-    if (pt != p) __builtin_nontemporal_store(acc, &K[(int64_t)pt * 9 + 3 * l + k]);
+    __builtin_nontemporal_store(val, &K[(int64_t)p * 9 + t]);
+    if (pt != p) __builtin_nontemporal_store(val, &K[(int64_t)pt * 9 + 3 * l + k]);
   } else {
-    K[(int64_t)p * 9 + t] = acc;
-    if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = acc;
+    K[(int64_t)p * 9 + t] = val;
+    if (pt != p) K[(int64_t)pt * 9 + 3 * l + k] = val;
   }
 }
 
@@ -1044,11 +1046,11 @@ static void launchGatherK(ik_handle* h, int64_t r0, int64_t r1, int mask, const 
   const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
   if (q1 <= q0) return;  // every block of these rows was written by the element kernel
   const dim3 grid((unsigned)(((q1 - q0) * 9 + 255) / 256));
-  if (h->neas)
-    hipLaunchKernelGGL((gatherK<T, false>), grid, dim3(256), 0, s, h->segptr.p, h->upBlk.p, h->upT.p, h->upMask.p,
-                       mask, Kebe, q0, q1, K);
+  if (h->neas)  // the EAS element stages FP64 blocks (h->ebeK64) whatever the matrix precision
+    hipLaunchKernelGGL((gatherK<double, T, false>), grid, dim3(256), 0, s, h->segptr.p, h->upBlk.p, h->upT.p,
+                       h->upMask.p, mask, h->ebeK64.p, q0, q1, K);
   else
-    hipLaunchKernelGGL((gatherK<T, true>), grid, dim3(256), 0, s, h->segptr.p, h->upBlk.p, h->upT.p, h->upMask.p,
+    hipLaunchKernelGGL((gatherK<T, T, true>), grid, dim3(256), 0, s, h->segptr.p, h->upBlk.p, h->upT.p, h->upMask.p,
                        mask, Kebe, q0, q1, K);
 }
 
@@ -1125,8 +1127,8 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
   const int mask = dbc == IK_DBC_RAW ? 0 : 1;
   h->ebeR.ensure((size_t)h->ne * 24);  // EAS elements always produce R_e
   if (h->neas) {
-    require(precision == IK_FP64, IK_ENOTIMPL, "the EAS element is FP64 only");
-    // the EAS kernel computes the condensed R_e with K_e; both gathers run when K is wanted
+    // the EAS kernel computes the condensed R_e with K_e (in FP64; an FP32 matrix is gathered from the FP64
+    // staging); both gathers run when K is wanted
     wantR = true;
   }
   // EAS: one element pass yields K_e and R_e. Plain element: the register-bound tangent kernel and the
@@ -1140,8 +1142,13 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
     runChunks<double>(h, dbc, easBoth, sideR, h->ebeK64.p, h->K64.p);
   } else if (wantK) {
     h->K32.ensure((size_t)h->nnzb * 9);
-    h->ebeK32.ensure((size_t)h->nstage * 9);
-    runChunks<float>(h, dbc, false, sideR, h->ebeK32.p, h->K32.p);
+    if (h->neas) {
+      h->ebeK64.ensure((size_t)h->nstage * 9);
+      runChunks<float>(h, dbc, easBoth, sideR, nullptr, h->K32.p);
+    } else {
+      h->ebeK32.ensure((size_t)h->nstage * 9);
+      runChunks<float>(h, dbc, false, sideR, h->ebeK32.p, h->K32.p);
+    }
   }
   if (wantR && !wantK) {
     // residual alone: one element launch over the mesh, then one gather (a few MB; no staging overlap needed)

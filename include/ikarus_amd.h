@@ -113,7 +113,8 @@ int ik_energy(ik_handle* h, double* energy);
 int ik_stress_at(ik_handle* h, int kind, const double xi[3], double* out, int where);
 
 /* MatrixAssembler::matrix(req, stiffness, dbc) (interface.hh:430-440; simpleassemblers.inl:121-204).
- * Assembles into library-owned 3x3-block CSR storage on the device (FP64, or FP32 for IK_FP32). */
+ * Assembles into library-owned 3x3-block CSR storage on the device (FP64, or FP32 for IK_FP32; the EAS
+ * element computes in FP64 and its FP32 matrix is the FP64 one rounded entry by entry). */
 int ik_matrix(ik_handle* h, int dbc, int precision);
 
 /* Residual and tangent at the bound requirement in one call: the pair residualFunction_(x),

@@ -79,6 +79,47 @@ def test_eas_assembly_and_alpha_update():
     assert _rel(dev.alpha(), o.alpha()) < 1e-10
 
 
+def test_eas_fp32_matrix_is_the_rounded_fp64_matrix():
+    """The EAS element computes and stages FP64 blocks; the FP32 matrix of the mixed solver is gathered from
+    the same staging in the same order, so every entry is the FP64 entry rounded once."""
+    p = pb.hex_block((4, 3, 3), jitter=0.05, seed=17, eas=21)
+    print("case: eas fp32 matrix", flush=True)
+    dev = ia.Device(p)
+    rng = np.random.default_rng(5)
+    dev.set_alpha(1e-6 * rng.standard_normal((p.ne, 21)))
+    dev.set_requirement(0.01 * rng.standard_normal(p.ndof) * (1 - p.fixed), 0.8)
+    dev.matrix(capi.DBC_FULL, precision=capi.FP64)
+    _, _, v64 = dev.csc()
+    dev.matrix(capi.DBC_FULL, precision=capi.FP32)
+    _, _, v32 = dev.csc()
+    assert np.array_equal(v32, v64.astype(np.float32).astype(np.float64))
+
+
+def test_eas_mixed_precision_newton_matches_fp64():
+    """Load control with the mixed solver (FP32 condensed tangent, FP32 block-Jacobi PCG, FP64 refinement)
+    on an EAS-21 bending block reaches the FP64 residual tolerance and the plain FP64 path's solution. Both
+    stop at ||R|| <= 1e-8 (FP64 residual), so the displacements differ by about K^-1 times that residual."""
+    p = pb.bending_block((6, 4, 3), seed=3)
+    print("case: eas mixed newton", flush=True)
+    out = {}
+    for prec in (capi.FP64, capi.FP32):
+        dev = ia.Device(p)
+        dev.set_requirement(np.zeros(p.ndof), 0.0)
+        cfg = capi.LCConfig()
+        cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 3, 0.0, 1.0, capi.PRED_LINEAR
+        kind = capi.PCG_JACOBI_FP64 if prec == capi.FP64 else capi.PCG_BJACOBI_FP32_IR
+        s = ia.nr_settings(tol=1e-8, precision=prec,
+                           linear=ia.solver_opts(kind=kind, rtol=1e-12, inner_rtol=1e-3, max_iter=20000))
+        info, iters, res = dev.load_control(cfg, s)
+        assert info.success and res[info.solves - 1] <= 1e-8
+        out[prec] = (dev.get_requirement()[0], dev.alpha())
+        dev.close()
+    print("rel u", _rel(out[capi.FP32][0], out[capi.FP64][0]), "rel alpha",
+          _rel(out[capi.FP32][1], out[capi.FP64][1]), flush=True)
+    assert _rel(out[capi.FP32][0], out[capi.FP64][0]) < 1e-8
+    assert _rel(out[capi.FP32][1], out[capi.FP64][1]) < 1e-6
+
+
 # tests/src/testcantileverbeamEAS.cpp:61-67 with tests/src/testcantileverbeam.hh:83-198
 @pytest.mark.parametrize("material,maxu", [(capi.NEOHOOKE, 4.781820664768682), (capi.SVK, 4.7692391315649365)])
 def test_cantilever_eas21_golden_on_gpu(material, maxu):
