@@ -477,7 +477,7 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   // (2b) K_uu, lane (ka, kb) per 3x3 block (the K_uu inputs are untouched by the spill above)
   if constexpr (WANT_K) {
     double kb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 2  // fully unrolled, the hoisted loads would take every VGPR
+#pragma unroll 1  // unrolled, the hoisted loads take registers (2 or 4: 166 / 154 VGPRs, no faster)
     for (int g = 0; g < 8; ++g) {
       double pa[3], pb[3], ga[3], gb[3], Ci6[6], S6[6], Ph[6];
       for (int i = 0; i < 3; ++i) {
