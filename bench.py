@@ -199,8 +199,8 @@ def main():
     ap.add_argument("--c4-load-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding design is "
-              "described in DESIGN.md, nothing here uses a second GPU.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hot path is measured on one GPU's "
+              "share of the mesh; nothing here uses a second GPU.", file=sys.stderr)
         sys.exit(2)
 
     if args.config:

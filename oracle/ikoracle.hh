@@ -333,7 +333,8 @@ inline void glStrain(const GeoPoint<D>& gp, const double* ue, double F[D][D], do
   }
 }
 
-// B_a[v][k] = dEv_v / du_{a,k}   (survey Appendix A.4)
+// B_a[v][k] = dEv_v / du_{a,k}: the B-operator of the Green-Lagrange strain in Voigt form, engineering shears
+// (nonlinearelastic.hh:313-322 contracts it with the material tangent; strain from Dune::greenLagrangeStrains)
 template <int D>
 inline void bop(const double F[D][D], const double g[D], double* B /* ns x D */) {
   if constexpr (D == 3) {
