@@ -10,10 +10,16 @@ host synchronisation for the pair).
 value = elements assembled per second (Melem/s) over the timed steps. The same state assembled by the two
 separate calls (ik_vector, then ik_matrix) is timed beside it.
 
+Untimed before the timed steps: a clock-settling pre-pass (assembly steps for --settle-ms of wall time, reported
+as clock_settle), then --warmup steps.
+
 Also reported (same run): the element kernel's roofline figures (HIP events on the library stream; HBM
-traffic per launch from the committed rocprofv3 counter summary), the gather and SpMV bandwidths, the CPU
-baseline (oracle restatement timed on a bounded sample on the host cores) and the Newton load path
-(40 load steps to ||R|| <= 1e-8) in the plain FP64 form and the mixed / learned forms.
+traffic per launch from the committed rocprofv3 counter summary) and the whole step's FP64 fraction, the gather and
+SpMV bandwidths, config c2's 5-step FP64 load path, the Newton comparison (40 load steps to ||R|| <= 1e-8: the
+plain FP64 form on the first evaluation seed, the mixed form with the constant, secant and learned predictors on
+8 evaluation seeds never used for training or tuning) and the CPU baseline (the oracle restatement, faithful and
+optimized, on bounded samples on the host cores, plus config c1 in full).
+--tune-eta chooses the forcing-term cap of each mixed predictor on training seeds.
 """
 import argparse
 import json
@@ -40,30 +46,100 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds):
-    """Oracle (reference-cost port: tangent per node pair, serial scatter) K+R assembly on a 16^3 sample,
-    repeated until `seconds` of CPU time, single thread."""
+def cpu_info():
+    """Host CPU model, its logical cores and the threads the optimized CPU variant uses (the GPU box's share of
+    host cores is 16, whatever the machine reports)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+    return model, nproc, max(1, min(threads, nproc, 16))
+
+
+def _oracle(p, faithful, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    p = pb.hex_block((16, 16, 16), seed=0)
-    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, 0, 2)
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, p.eas, 2)
     o.set_fixed(p.fixed)
-    o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction)
-    o.set_faithful(True)
+    if len(p.face_elem):
+        o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction[:p.dim])
+    o.set_faithful(faithful)
+    o.set_threads(threads)
+    return po, o
+
+
+def cpu_assembly(seconds, faithful, threads, n=16):
+    """K+R assembly rate of the CPU oracle on an n^3 sample, repeated until `seconds` have passed."""
+    p = pb.hex_block((n, n, n), seed=0)
+    _, o = _oracle(p, faithful, threads)
     u = 0.005 * np.random.default_rng(0).standard_normal(p.ndof) * (1 - p.fixed)
     o.pattern()
-    n = 0
+    reps = 0
     t0 = time.perf_counter()
     while True:
         o.matrix(u, 1.0)
         o.vector(u, 1.0)
-        n += 1
+        reps += 1
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    return dict(value=round(n * p.ne / dt / 1e6, 6), unit="Melem/s", cores=1, kind="port",
-                sample=f"{n}x K+R assembly of a 16^3 hex8 NH mesh (4096 elements), reference cost structure "
-                       f"(tangent per node pair), serial, {dt:.1f} s")
+    return p.ne * reps / dt / 1e6, reps, dt
+
+
+def cpu_load_path(p, steps, faithful, threads, direct):
+    """One LoadControl run of the CPU oracle (Newton to ||R|| <= 1e-8 with the banded direct solve, or Jacobi-CG
+    to 1e-10 relative as the plain FP64 device form does)."""
+    po, o = _oracle(p, faithful, threads)
+    t0 = time.perf_counter()
+    r = o.load_control(steps, 0.0, 1.0, tol=1e-8, solver=po.DIRECT if direct else po.JACOBI_CG, cg_tol=1e-10,
+                       threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(seconds=round(dt, 3), seconds_per_step=round(dt / steps, 4), success=bool(r["success"]),
+                newton_iterations=int(r["total"]), cg_iterations=int(r["cg_iterations"]))
+
+
+def cpu_baseline(seconds, c2_sample=12):
+    """The CPU oracle (the reference itself cannot be built here: it needs DUNE, Eigen >= 3.4.90 and SuiteSparse,
+    none of which is installed), timed on the host cores in two variants:
+    * faithful: the reference's cost structure, one thread (material tangent re-evaluated for every node pair of
+      the element as nonlinearelastic.hh:390-398 does, serial element loop and scatter, single-thread CG);
+    * optimized: material tangent once per Gauss point, elements in a node-conflict coloring with one OpenMP loop
+      per color, threaded Jacobi-CG, on `threads` host threads.
+    Samples: K+R assembly of a 16^3 hex8 NH mesh (the headline unit, Melem/s); config c1 in full (2D plane-strain
+    NH cantilever 64x16, 10 load steps, banded LDL^T direct solve standing in for the reference's sparse direct
+    solver); config c2's load path (5 load steps, Jacobi-CG) on a c2_sample^3 mesh."""
+    model, nproc, threads = cpu_info()
+    log(f"[bench] cpu: {model}, {nproc} logical cores, optimized variant on {threads} threads")
+    f_rate, f_reps, f_dt = cpu_assembly(seconds, True, 1)
+    o_rate, o_reps, o_dt = cpu_assembly(seconds / 2, False, threads)
+    log(f"[bench] cpu assembly: faithful {f_rate:.4f} Melem/s, optimized {o_rate:.4f} Melem/s")
+    c1 = pb.cantilever2d(seed=0)
+    c2 = pb.hex_block((c2_sample,) * 3, seed=0)
+    out = dict(value=round(o_rate, 6), unit="Melem/s", cores=threads, kind="port",
+               sample=f"{o_reps}x K+R assembly of a 16^3 hex8 NH mesh (4096 elements), optimized CPU oracle (tangent "
+                      f"once per Gauss point, colored elements, OpenMP) on {threads} threads, {o_dt:.1f} s",
+               cpu_model=model, logical_cores=nproc,
+               reference_build="not available: the reference needs DUNE, dune-localfefunctions, Eigen >= 3.4.90 and "
+                               "SuiteSparse, none installed; the oracle restates its arithmetic",
+               faithful={"value": round(f_rate, 6), "unit": "Melem/s", "cores": 1,
+                         "sample": f"{f_reps}x K+R assembly of the same mesh, reference cost structure (tangent per "
+                                   f"node pair), serial, {f_dt:.1f} s"},
+               c1={"workload": f"2D plane-strain NH cantilever 64x16 Q1 over 4x1, E=1000, nu=0.3, traction "
+                               f"{[round(t, 4) for t in c1.traction[:2]]}, 10 load steps to ||R|| <= 1e-8, banded LDL^T",
+                   "faithful_1_thread": cpu_load_path(c1, 10, True, 1, True),
+                   "optimized": cpu_load_path(c1, 10, False, threads, True)},
+               c2_sample={"workload": f"c2 load path on a {c2_sample}^3 hex8 NH sample (5 load steps, Jacobi-CG to "
+                                      "1e-10 per Newton iteration, ||R|| <= 1e-8)",
+                          "faithful_1_thread": cpu_load_path(c2, 5, True, 1, False),
+                          "optimized": cpu_load_path(c2, 5, False, threads, False)})
+    log(f"[bench] cpu c1: {out['c1']}; c2 sample: {out['c2_sample']}")
+    return out
 
 
 NEWTON_MODES = {
@@ -72,21 +148,31 @@ NEWTON_MODES = {
     # same precision, inexact Newton + secant predictor (isolates the algorithmic part of the gain)
     "fp64_inexact_secant": dict(precision=capi.FP64, kind=capi.PCG_JACOBI_FP64, rtol=1e-10, forcing=0.9,
                                 pred=capi.PRED_LINEAR),
-    # mixed form: FP32 tangent + FP32 block-Jacobi PCG with FP64 refinement, inexact Newton, secant predictor
+    # mixed form: FP32 tangent storage + FP32 block-Jacobi PCG with FP64 refinement, inexact Newton; constant,
+    # secant and learned predictors. eta_max (the forcing term's cap) of each predictor is the best of
+    # ETA_CANDIDATES on the training seeds TUNE_SEEDS (bench.py --tune-eta), never on an evaluation seed.
+    # (64^3, 40 load steps, seeds 1000-1003: constant 0.03, secant 0.1, learned 0.01)
+    "mixed_constant": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
+                           pred=capi.PRED_CONSTANT, eta_max=0.03),
     "mixed_secant": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
-                         pred=capi.PRED_LINEAR),
-    # mixed form + learned predictor: Newton starts near the solution, where a loose first linear solve costs a
-    # Newton iteration, so the forcing term is capped at 1e-3 (best of 0.1 / 0.01 / 1e-3 on seeds 2000-2002)
+                         pred=capi.PRED_LINEAR, eta_max=0.1),
     "mixed_mlp": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
-                      pred=capi.PRED_MLP, eta_max=1e-3),
+                      pred=capi.PRED_MLP, eta_max=0.01),
     # precision alone: the plain form's Newton and predictor with the FP32 tangent and refined FP32 PCG
     "mixed_exact": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.0,
                         pred=capi.PRED_CONSTANT),
 }
 
 
-def newton_path(p, mode, steps, mlp_params=None):
-    m = NEWTON_MODES[mode]
+TUNE_SEEDS = (1000, 1001, 1002, 1003)     # training seeds (the network was trained on paths of 1000-1255)
+EVAL_SEEDS = tuple(range(2003, 2011))      # evaluation seeds: never used for training or tuning
+ETA_CANDIDATES = (0.1, 0.03, 0.01, 1e-3)
+
+
+def newton_path(p, mode, steps, mlp_params=None, eta_max=None):
+    m = dict(NEWTON_MODES[mode])
+    if eta_max is not None:
+        m["eta_max"] = eta_max
     dev = ia.Device(p)
     if m["pred"] == capi.PRED_MLP:
         dev.check(capi.lib().ik_mlp_set_params(dev.h, capi.ptr(mlp_params), capi.HOST))
@@ -105,7 +191,80 @@ def newton_path(p, mode, steps, mlp_params=None):
     dev.close()
     return dict(seconds=round(dt, 4), success=bool(info.success), newton_iterations=int(info.total_iterations),
                 per_step=[int(i) for i in iters], linear_iterations=int(info.linear_iterations),
-                final_residual=float(res[info.solves - 1])), u
+                final_residual=float(res[info.solves - 1]), max_step_residual=float(np.max(res[:info.solves])),
+                fallbacks=int(info.fallbacks)), u
+
+
+MIXED_MODES = ("mixed_constant", "mixed_secant", "mixed_mlp")
+
+
+def tune_eta(n, steps, mlp):
+    """eta_max of each mixed predictor chosen on the training seeds: total seconds over TUNE_SEEDS per candidate."""
+    table = {}
+    for mode in MIXED_MODES:
+        table[mode] = {}
+        for eta in ETA_CANDIDATES:
+            tot, its = 0.0, 0
+            for seed in TUNE_SEEDS:
+                r, _ = newton_path(pb.hex_block((n, n, n), seed=seed), mode, steps, mlp, eta_max=eta)
+                assert r["success"], (mode, eta, seed)
+                tot += r["seconds"]
+                its += r["newton_iterations"]
+            table[mode][str(eta)] = {"seconds": round(tot, 3), "newton_iterations": its}
+            log(f"[bench] tune {mode} eta_max={eta}: {tot:.3f} s, {its} Newton over seeds {TUNE_SEEDS}")
+    best = {m: float(min(t, key=lambda k: t[k]["seconds"])) for m, t in table.items()}
+    return {"seeds": list(TUNE_SEEDS), "mesh": f"{n}^3", "load_steps": steps, "table": table, "best_eta_max": best}
+
+
+def evaluate_newton(n, steps, mlp, seeds):
+    """The c5 comparison: the plain FP64 form and FP64 inexact + secant on the first evaluation seed, then the mixed
+    form with the constant, secant and learned predictors on every evaluation seed; per-seed and median figures."""
+    first = pb.hex_block((n, n, n), seed=seeds[0])
+    plain, us = {}, {}
+    for mode in ("fp64", "fp64_inexact_secant"):
+        log(f"[bench] newton {mode} seed {seeds[0]} ...")
+        plain[mode], us[mode] = newton_path(first, mode, steps)
+        log(f"[bench] newton {mode}: {plain[mode]['seconds']} s, {plain[mode]['newton_iterations']} Newton, "
+            f"{plain[mode]['linear_iterations']} PCG")
+    per_seed = {}
+    for seed in seeds:
+        p = pb.hex_block((n, n, n), seed=seed)
+        row = {"traction": [round(t, 3) for t in p.traction]}
+        for mode in MIXED_MODES:
+            r, u = newton_path(p, mode, steps, mlp)
+            row[mode] = {k: r[k] for k in ("seconds", "newton_iterations", "linear_iterations", "success",
+                                           "max_step_residual", "fallbacks")}
+            if seed == seeds[0]:
+                row[mode]["rel_diff_u_vs_fp64"] = float(np.linalg.norm(u - us["fp64"]) / np.linalg.norm(us["fp64"]))
+        log(f"[bench] seed {seed}: " + ", ".join(f"{m} {row[m]['seconds']} s / {row[m]['newton_iterations']} Newton"
+                                                  for m in MIXED_MODES))
+        per_seed[str(seed)] = row
+    med = {m: {"seconds": float(np.median([per_seed[str(s)][m]["seconds"] for s in seeds])),
+               "newton_iterations": float(np.median([per_seed[str(s)][m]["newton_iterations"] for s in seeds]))}
+           for m in MIXED_MODES}
+    ratio = {f"{a}_over_mlp": float(np.median([per_seed[str(s)][a]["seconds"] / per_seed[str(s)]["mixed_mlp"]["seconds"]
+                                              for s in seeds])) for a in ("mixed_constant", "mixed_secant")}
+    first_row = per_seed[str(seeds[0])]
+    speed = {m: round(plain["fp64"]["seconds"] / first_row[m]["seconds"], 3) for m in MIXED_MODES}
+    speed["fp64_inexact_secant"] = round(plain["fp64"]["seconds"] / plain["fp64_inexact_secant"]["seconds"], 3)
+    return {
+        "workload": f"hex8 Neo-Hooke {n}^3, {steps} load steps to ||R||<=1e-8 at every step, random traction per "
+                    f"seed; evaluation seeds {seeds[0]}-{seeds[-1]} (not used for training or tuning); every time "
+                    "includes the one-time pattern build of the first matrix call",
+        "eta_max": {m: NEWTON_MODES[m]["eta_max"] for m in MIXED_MODES},
+        "plain_fp64_seed": seeds[0], "plain": plain,
+        "per_seed": per_seed, "median": med, "median_time_ratio_vs_mlp": ratio,
+        "speedup_vs_fp64_on_first_seed": speed,
+        "headline": f"mixed_mlp vs fp64 on seed {seeds[0]}; medians over {len(seeds)} seeds for the mixed forms",
+    }
+
+
+def c2_load_path(n, steps):
+    """Config c2's load path on the device: plain FP64 form (FP64 tangent, Jacobi-PCG to 1e-10 per Newton
+    iteration, constant predictor), seed 0, per-step time."""
+    r, _ = newton_path(pb.hex_block((n, n, n), seed=0), "fp64", steps)
+    r["seconds_per_step"] = round(r["seconds"] / steps, 4)
+    return r
 
 
 def assembly_step(dev):
@@ -181,12 +340,18 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=30, help="untimed steps (the clocks settle over ~20 ms)")
+    ap.add_argument("--warmup", type=int, default=30, help="untimed steps after the clock-settling pre-pass")
+    ap.add_argument("--settle-ms", type=float, default=300.0,
+                    help="untimed assembly steps until this much wall time has passed, before the warmup steps: the "
+                         "GPU clocks ramp over the first ~20-100 ms of load (reported as clock_settle)")
     ap.add_argument("--n", type=int, default=64, help="elements per direction")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=40, help="load steps of the Newton comparison")
-    ap.add_argument("--newton-seed", type=int, default=2000, help="unseen seed of the Newton comparison")
+    ap.add_argument("--eval-seeds", type=int, default=len(EVAL_SEEDS), help="evaluation seeds of the c5 comparison")
+    ap.add_argument("--tune-eta", action="store_true",
+                    help="only choose eta_max per mixed predictor on the training seeds (prints the table)")
+    ap.add_argument("--c2-steps", type=int, default=5, help="load steps of config c2's FP64 path on the device")
     ap.add_argument("--mlp", default=os.path.join(ROOT, "ikarus_amd", "data", "predictor_mlp_32cubed.f32"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-pass", action="store_true",
@@ -206,6 +371,10 @@ def main():
     if args.config:
         out = config_precision(args) if args.config == "c3" else config_eas(args)
         print(json.dumps(out), flush=True)
+        return
+    if args.tune_eta:
+        mlp = np.fromfile(args.mlp, dtype=np.float32)
+        print(json.dumps({"tune_eta": tune_eta(args.n, args.newton_steps, mlp)}), flush=True)
         return
 
     n = args.n
@@ -236,10 +405,18 @@ def main():
                                       "fp32": round(s32[1] / max(s32[0], 1), 4)}}), flush=True)
         return
 
+    # clock-settling pre-pass (untimed, reported): steps until settle_ms of wall time, then the warmup steps
+    settle_steps = 0
+    t0 = time.perf_counter()
+    while (time.perf_counter() - t0) * 1e3 < args.settle_ms:
+        assembly_step(dev)
+        dev.synchronize()
+        settle_steps += 1
+    settle_s = time.perf_counter() - t0
     for _ in range(args.warmup):
         assembly_step(dev)
     dev.synchronize()
-    log("[bench] warmup done")
+    log(f"[bench] clock settle {settle_steps} steps / {settle_s * 1e3:.0f} ms, then {args.warmup} warmup steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         assembly_step(dev)
@@ -247,6 +424,7 @@ def main():
     dt = time.perf_counter() - t0
     ms_per_step = dt / args.steps * 1e3
     value = p.ne * args.steps / dt / 1e6
+    step_tflops = FLOP_PER_ELEM_KR * p.ne / (ms_per_step * 1e-3) / 1e12
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} Melem/s")
     for _ in range(args.warmup):
         separate_step(dev)
@@ -303,7 +481,12 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                      "kernel": "hex8Element<NeoHooke,double,K>", "kernel_ms_per_launch": round(k_ms, 4),
-                     "elements_per_launch": int(elems_per_launch), "flop_per_elem": FLOP_PER_ELEM_K},
+                     "elements_per_launch": int(elems_per_launch), "flop_per_elem": FLOP_PER_ELEM_K,
+                     "step_frac": round(step_tflops / PEAK_FP64_TFLOPS, 4), "step_tflops": round(step_tflops, 3),
+                     "step_note": "whole R+K step (element, gather and residual passes) at 86,176 flop/element"},
+        "clock_settle": {"steps": settle_steps, "ms": round(settle_s * 1e3, 1),
+                         "note": "untimed assembly steps before the warmup steps, so the timed loop runs at settled "
+                                 "clocks"},
         "assembly_breakdown_ms_per_step": {
             "element_K": round(kp["element"][1] / args.steps, 4), "gather_K": round(g_ms_tot / args.steps, 4),
             "residual": round(r_ms, 4)},
@@ -319,27 +502,17 @@ def main():
                           "frac_hbm": round(spmv32_gbs / PEAK_HBM_GBS, 4), "bytes": spmv32_bytes}},
     }
     if not args.no_newton:
-        pn = pb.hex_block((n, n, n), seed=args.newton_seed)
-        mlp = np.fromfile(args.mlp, dtype=np.float32) if os.path.exists(args.mlp) else None
-        res, us = {}, {}
-        for mode in ("fp64", "fp64_inexact_secant", "mixed_secant", "mixed_mlp"):
-            if mode == "mixed_mlp" and mlp is None:
-                continue
-            log(f"[bench] newton {mode} ...")
-            res[mode], us[mode] = newton_path(pn, mode, args.newton_steps, mlp)
-            log(f"[bench] newton {mode}: {res[mode]['seconds']} s, {res[mode]['newton_iterations']} Newton, "
-                f"{res[mode]['linear_iterations']} PCG iterations")
-        best = "mixed_mlp" if "mixed_mlp" in res else "mixed_secant"
-        ref = us["fp64"]
-        out["newton"] = {
-            "workload": f"hex8 Neo-Hooke {n}^3, {args.newton_steps} load steps to ||R||<=1e-8, unseen seed "
-                        f"{args.newton_seed}, traction {[round(t, 3) for t in pn.traction]}; each time includes "
-                        "the one-time pattern build of the first matrix call (pattern_build_s)",
-            "modes": res,
-            "speedup_vs_fp64": {k: round(res["fp64"]["seconds"] / v["seconds"], 3) for k, v in res.items()},
-            "rel_diff_u_vs_fp64": {k: float(np.linalg.norm(us[k] - ref) / np.linalg.norm(ref)) for k in res},
-            "headline": f"{best} vs fp64",
-        }
+        log(f"[bench] c2 load path ({args.c2_steps} steps, plain FP64) ...")
+        out["c2_load_path"] = c2_load_path(n, args.c2_steps)
+        out["c2_load_path"]["workload"] = (f"hex8 Neo-Hooke {n}^3 seed 0, {args.c2_steps} load steps, FP64 tangent, "
+                                           "Jacobi-PCG to 1e-10 per Newton iteration, ||R|| <= 1e-8")
+        log(f"[bench] c2: {out['c2_load_path']['seconds_per_step']} s per load step")
+        if args.c2_steps > 0:
+            r, _ = newton_path(pb.hex_block((12, 12, 12), seed=0), "fp64", args.c2_steps)
+            out["c2_load_path"]["device_on_cpu_sample_12cubed"] = {k: r[k] for k in ("seconds", "newton_iterations",
+                                                                                     "linear_iterations")}
+        mlp = np.fromfile(args.mlp, dtype=np.float32)
+        out["newton"] = evaluate_newton(n, args.newton_steps, mlp, EVAL_SEEDS[:max(1, args.eval_seeds)])
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

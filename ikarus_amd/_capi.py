@@ -10,7 +10,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libikarus_amd.so")
 
-IK_OK, IK_EINVAL, IK_EHIP, IK_ENEGDET, IK_ENOTIMPL, IK_ENOTBOUND, IK_ENOCONV = 0, -1, -2, -3, -4, -5, -6
+IK_OK, IK_EINVAL, IK_EHIP, IK_ENEGDET, IK_ENOTIMPL, IK_ENOTBOUND, IK_ENOCONV, IK_ESINGULAR = 0, -1, -2, -3, -4, -5, -6, -7
 DBC_RAW, DBC_FULL, DBC_REDUCED = 0, 1, 2
 NEOHOOKE, SVK = 0, 1
 FP64, FP32 = 0, 1
@@ -74,7 +74,7 @@ class PFConfig(C.Structure):
 
 class ControlInfo(C.Structure):
     _fields_ = [("success", C.c_int32), ("total_iterations", C.c_int32), ("linear_iterations", C.c_int64),
-                ("solves", C.c_int32)]
+                ("solves", C.c_int32), ("fallbacks", C.c_int32)]
 
 
 _lib = None

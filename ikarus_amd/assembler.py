@@ -344,7 +344,8 @@ class LoadControl:
         info, iters, res = dev.load_control(self.cfg, self.nr.settings)
         req.u, req.lam = dev.get_requirement()
         return dict(success=bool(info.success), totalIterations=info.total_iterations,
-                    linearIterations=info.linear_iterations, iterations=iters[:info.solves], residuals=res[:info.solves])
+                    linearIterations=info.linear_iterations, iterations=iters[:info.solves], residuals=res[:info.solves],
+                    fallbacks=info.fallbacks)
 
 
 class PathFollowing:

@@ -986,17 +986,21 @@ void launchZeroF(float* p, int64_t n, hipStream_t s) {
 }
 
 static void checkErrFlag(ik_handle* h) {
-  if (!h->herr) IK_HIP(hipHostMalloc((void**)&h->herr, sizeof(unsigned long long), hipHostMallocDefault));
-  IK_HIP(hipMemcpyAsync(h->herr, h->errFlag.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+  if (!h->herr) IK_HIP(hipHostMalloc((void**)&h->herr, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  IK_HIP(hipMemcpyAsync(h->herr, h->errFlag.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
   IK_HIP(hipStreamSynchronize(h->stream));
-  const unsigned long long v = *h->herr;
-  if (v != ~0ull) {
-    h->bad_elem = (int64_t)v;
-    unsigned long long reset = ~0ull;
-    IK_HIP(hipMemcpy(h->errFlag.p, &reset, sizeof(reset), hipMemcpyHostToDevice));
+  const unsigned long long det = h->herr[0], sing = h->herr[1];
+  if (det == ~0ull && sing == ~0ull) return;
+  const unsigned long long reset[2] = {~0ull, ~0ull};
+  IK_HIP(hipMemcpy(h->errFlag.p, reset, sizeof(reset), hipMemcpyHostToDevice));
+  if (det != ~0ull) {
+    h->bad_elem = (int64_t)det;
     throw Error{IK_ENEGDET, "Determinant of right Cauchy Green tensor C must be greater than zero (element " +
-                                std::to_string(v) + ")"};
+                                std::to_string(det) + ")"};
   }
+  h->bad_elem = (int64_t)sing;
+  throw Error{IK_ESINGULAR, "the enhanced-strain matrix D of element " + std::to_string(sing) +
+                                " is singular (a pivot of its elimination is zero or not finite)"};
 }
 
 void stressAt(ik_handle* h, int kind, const double xi[3], double* out) {

@@ -152,6 +152,11 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
   assembleR(h, IK_DBC_FULL);
   if (idbcF) ik::axpy(h, stepSize, h->idbcF.p, h->R.p, n);
   double rNorm        = normR(h);
+  if (!std::isfinite(rNorm)) {
+    info->residual_norm = rNorm;
+    info->success       = 0;
+    return;
+  }
   double rPrev        = rNorm;
   double etaPrev      = 0.0;
   info->residual_norm = rNorm;
@@ -191,8 +196,11 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     rNorm               = normR(h);
     info->residual_norm = rNorm;
     ++iter;
+    if (!std::isfinite(rNorm)) break;
   }
-  if (iter == s->max_iter) info->success = 0;
+  // success = iter != maxIter (newtonraphson.hh:251-252); a residual norm that is not finite is a failure here
+  // (the reference's loop ends on it, since NaN > tol is false, and reports success)
+  if (iter == s->max_iter || !std::isfinite(rNorm)) info->success = 0;
   info->iterations = iter;
 }
 
@@ -222,6 +230,10 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
   arcSubsidiary(h, a, f, invRoot, dfdDl);
   double rNorm        = normR(h);  // the first norm leaves f out (:244)
   info->residual_norm = rNorm;
+  if (!std::isfinite(rNorm)) {
+    info->success = 0;
+    return;
+  }
   int iter            = 0;
   while (rNorm > s->tol && iter < s->max_iter) {
     assembleK(h, IK_DBC_FULL, s->precision);
@@ -249,8 +261,9 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
     rNorm               = std::sqrt(ik::dot(h, h->R.p, h->R.p, n) + f * f);
     info->residual_norm = rNorm;
     ++iter;
+    if (!std::isfinite(rNorm)) break;
   }
-  if (iter == s->max_iter) info->success = 0;
+  if (iter == s->max_iter || !std::isfinite(rNorm)) info->success = 0;
   info->iterations = iter;
 }
 
@@ -325,9 +338,10 @@ int ik_create(ik_handle** out, const ik_problem_desc* d) {
     h->u.ensure(h->ndof);
     h->R.ensure(h->ndof);
     ik::launchZero(h->u.p, h->ndof, h->stream);
-    h->errFlag.ensure(1);
-    const unsigned long long none = ~0ull;
-    IK_HIP(hipMemcpyAsync(h->errFlag.p, &none, sizeof(none), hipMemcpyHostToDevice, h->stream));
+    // error flags: [0] lowest element with det C <= 0, [1] lowest EAS element with a singular D
+    h->errFlag.ensure(2);
+    const unsigned long long none[2] = {~0ull, ~0ull};
+    IK_HIP(hipMemcpyAsync(h->errFlag.p, none, sizeof(none), hipMemcpyHostToDevice, h->stream));
     if (h->neas) {
       h->alpha.ensure((size_t)h->ne * h->neas);
       ik::launchZero(h->alpha.p, h->ne * h->neas, h->stream);
@@ -648,16 +662,51 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
       ik::launchZero(duN.p, h->ndof, h->stream);
       ik::launchZero(duN1.p, h->ndof, h->stream);
     }
+    // the learned predictor only chooses where Newton starts: a step whose Newton solve fails from the MLP
+    // prediction (det C <= 0, a singular EAS matrix, a residual that is not finite, or maxIter) is restored to
+    // u_n, alpha_n, lambda_n and solved again from the secant prediction
+    ik::DevBuf<double> alphaSave;
+    if (c->predictor == IK_PRED_MLP && h->neas) alphaSave.ensure((size_t)h->ne * h->neas);
     for (int ls = 0; ls < c->load_steps; ++ls) {
+      const double lamN = h->lambda;
+      bool mlpStep      = false;
       if (c->predictor == IK_PRED_LINEAR && ls > 0) {
         ik::axpy(h, 1.0, duN.p, h->u.p, h->ndof);  // u += u_n - u_{n-1} (ArcLength secant semantics)
       } else if (c->predictor == IK_PRED_MLP) {
+        if (h->neas)
+          IK_HIP(hipMemcpyAsync(alphaSave.p, h->alpha.p, alphaSave.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                h->stream));
         ik::mlpPredict(h, duN.p, duN1.p, h->lambda + dl, dl, ls > 0 ? dl : 0.0, ls > 1 ? dl : 0.0);
+        mlpStep = true;
       }
       h->lambda += dl;  // predictor: lambda += dlambda (loadcontrol.inl:55-57)
       h->kValid        = false;
       h->easCacheValid = false;
-      newtonSolve(h, s, dl, &ni);
+      bool failed      = false;
+      if (mlpStep) {
+        try {
+          newtonSolve(h, s, dl, &ni);
+          failed = !ni.success;
+        } catch (const Error& e) {
+          if (e.code != IK_ENEGDET && e.code != IK_ESINGULAR) throw;
+          failed = true;
+          ni     = ik_nr_info{};
+        }
+        if (failed) {
+          info->fallbacks += 1;
+          info->total_iterations += ni.iterations;
+          info->linear_iterations += ni.linear_iterations;
+          IK_HIP(hipMemcpyAsync(h->u.p, uPrev.p, h->ndof * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+          if (h->neas)
+            IK_HIP(hipMemcpyAsync(h->alpha.p, alphaSave.p, alphaSave.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                  h->stream));
+          if (ls > 0) ik::axpy(h, 1.0, duN.p, h->u.p, h->ndof);
+          h->lambda        = lamN + dl;
+          h->kValid        = false;
+          h->easCacheValid = false;
+        }
+      }
+      if (!mlpStep || failed) newtonSolve(h, s, dl, &ni);
       if (ni.iterations == 0) syncIdbc(h);  // a step that needed no correction still takes the new values
       if (iterations) iterations[ls + 1] = ni.iterations;
       if (residuals) residuals[ls + 1] = ni.residual_norm;

@@ -112,6 +112,13 @@ __device__ inline double readLane(double x, int lane) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// maximum over the 64 lanes of the wavefront (every lane active), as a uniform value
+__device__ inline double waveMax(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return readLane(x, 0);
+}
+
 // Voigt index of the symmetric pair (i, j): 00 11 22 12 02 01
 __host__ __device__ constexpr int v6(int i, int j) {
   return i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5));
@@ -510,8 +517,8 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   }
   __syncthreads();
 
-  // (3) Y = D^-1 [L | Rt] by Gauss-Jordan elimination on the augmented 21 x 46 matrix (D is SPD: no pivoting,
-  // like the LDL^T it replaces; enhancedassumedstrains.hh inverts D, :243,294,344). Lane c < 46 keeps column
+  // (3) Y = D^-1 [L | Rt] by Gauss-Jordan elimination on the augmented 21 x 46 matrix without pivoting (D is SPD
+  // while the material tangent is positive definite; enhancedassumedstrains.hh inverts D by LU, :243,294,344). Lane c < 46 keeps column
   // c in registers; each step every lane reads the pivot column from lane j's registers (readlane). The
   // LDS broadcast this replaced (lane j stores, barrier, every lane loads) made the kernel LDS-issue bound:
   // 7.98 -> 5.69 ms per R+K pass at 96x96x48, results bitwise the same.
@@ -519,17 +526,27 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
 #pragma unroll
   for (int i = 0; i < NE; ++i)
     col[i] = t < NE ? sc.A[i][t] : (t < NE + 24 ? sc.Lm[i][t - NE] : (t == NE + 24 ? sc.rt[i] : 0.0));
+  // pivot guard: the reference inverts D with partial pivoting; without pivoting a pivot that is not finite or
+  // vanishes against the largest entry of D makes the element singular here (error flag word 1, lowest element)
+  double dmax = 0.0;
+  if (t < NE)
+#pragma unroll
+    for (int i = 0; i < NE; ++i) dmax = fmax(dmax, fabs(col[i]));
+  dmax = waveMax(dmax);
+  bool singular = !(dmax < INFINITY);
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
     double piv[NE];  // lane j's column, read into scalar registers (no LDS round trip, no barrier)
 #pragma unroll
     for (int i = 0; i < NE; ++i) piv[i] = readLane(col[i], j);
+    singular |= !(fabs(piv[j]) > 1e-13 * dmax && fabs(piv[j]) < INFINITY);
     const double f = col[j] / piv[j];
 #pragma unroll
     for (int i = 0; i < NE; ++i)
       if (i != j) col[i] -= piv[i] * f;
     col[j] = f;
   }
+  if (singular && t == 0) atomicMin(err + 1, (unsigned long long)e);
   // lanes NE..NE+23 hold Y = D^-1 L (column c = t - NE), lane NE+24 holds y = D^-1 Rt; cache them for the
   // alpha update (layout per element: Y 21 x 24 row-major, then y)
   double* cy = cache + e * (NE * 24 + NE);

@@ -166,6 +166,20 @@ def bending_block(n=(96, 96, 48), L=(2.0, 2.0, 1.0), material=0, E=1000.0, nu=0.
     return p
 
 
+def cantilever2d(n=(64, 16), L=(4.0, 1.0), material=0, E=1000.0, nu=0.3, seed=0, mag_range=(0.5, 2.0)):
+    """Plane-strain Q1 cantilever: x=0 edge clamped, uniform traction (0, -m) on the x=L edge with
+    m ~ U[mag_range] from the seed's stream (64x16 elements over 4x1: the 2D Neo-Hooke benchmark problem)."""
+    X, conn = structured_mesh(n, L)
+    lam, mu = lame(E, nu)
+    p = Problem(X, conn, material, lam, mu, name=f"quad{n[0]}x{n[1]}")
+    p.fixed = clamp_flags(X, 0, 0.0)
+    p.set_faces(face_elements(n, 0, 1), 0, 1)
+    m, _ = load_params(seed, *mag_range)
+    p.traction = (0.0, -float(m), 0.0)
+    p.meta = dict(seed=seed, E=E, nu=nu)
+    return p
+
+
 def cantilever3d(material=0, E=100.0, nu=0.3, eas=21):
     """The reference's 3D cantilever: YaspGrid 10x1x1 over 10x2x2, x=0 clamped, unit point loads in y at
     (10,2,2) and (10,2,0) (tests/src/testcantileverbeam.hh:40-80,102-140)."""

@@ -26,7 +26,10 @@ enum {
   IK_ENEGDET    = -3, /* det C <= 0 in an element; replaces abort() in materialhelpers.hh:120-126 */
   IK_ENOTIMPL   = -4, /* unsupported affordance / option (DUNE_THROW(NotImplemented))           */
   IK_ENOTBOUND  = -5, /* requirement not set before matrix()/vector() (interface.hh:187-193)     */
-  IK_ENOCONV    = -6  /* linear solver did not reach its tolerance                               */
+  IK_ENOCONV    = -6, /* linear solver did not reach its tolerance                               */
+  IK_ESINGULAR  = -7  /* EAS: D of an element is singular to working precision (a pivot of the
+                         unpivoted elimination is zero or not finite); the reference inverts D by
+                         pivoted LU (enhancedassumedstrains.hh:243,294,344) and has no such error */
 };
 
 /* DBCOption (ikarus/assembler/dirichletbcenforcement.hh:8) */
@@ -197,12 +200,19 @@ typedef struct {
 
 typedef struct {
   int32_t success;
-  int32_t total_iterations;
+  int32_t total_iterations;  /* Newton iterations, including those of failed learned-predictor attempts */
   int64_t linear_iterations;
   int32_t solves;         /* number of Newton solves run (initial + load steps)              */
+  int32_t fallbacks;      /* IK_PRED_MLP: load steps whose Newton solve failed from the MLP prediction
+                             and was solved again from the secant prediction                   */
 } ik_control_info;
 
-/* LoadControl::run. iterations/residuals: optional arrays of load_steps+1 entries. */
+/* LoadControl::run. iterations/residuals: optional arrays of load_steps+1 entries (the solve that ended each
+ * step). The run stops at the first failed solve (loadcontrol.inl:46-47). With IK_PRED_MLP, a step whose solve
+ * from the MLP prediction fails (IK_ENEGDET, IK_ESINGULAR, a residual norm that is not finite, or max_iter) is
+ * restored to u_n, alpha_n, lambda_n and solved from the secant prediction instead: plain Newton from the
+ * secant start stays the fallback. A Newton solve whose residual norm is not finite counts as failed (the
+ * reference's loop, newtonraphson.hh:219-252, ends on it and reports success). */
 int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s, ik_control_info* info,
                     int32_t* iterations, double* residuals);
 

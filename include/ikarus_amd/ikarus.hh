@@ -241,6 +241,7 @@ class NewtonRaphson {
 struct ControlInformation {
   bool success = false;
   int totalIterations = 0;
+  int fallbacks       = 0;      // IK_PRED_MLP steps solved again from the secant prediction
   std::vector<int> iterations;  // per Newton solve, the initial one first
   std::vector<double> residualNorms;
 };
@@ -265,6 +266,7 @@ class LoadControl {
     ControlInformation ci;
     ci.success         = info.success != 0;
     ci.totalIterations = info.total_iterations;
+    ci.fallbacks       = info.fallbacks;
     ci.iterations.assign(it.begin(), it.begin() + info.solves);
     ci.residualNorms.assign(res.begin(), res.begin() + info.solves);
     return ci;

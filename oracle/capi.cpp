@@ -99,6 +99,8 @@ void ikor_add_point_load(void* hp, int64_t dof, double coef) {
 }
 
 void ikor_set_faithful(void* hp, int on) { static_cast<Handle*>(hp)->P.faithfulCost = on != 0; }
+// element loops of matrix()/vector() on n threads over an element coloring (timing variant; 1 = serial)
+void ikor_set_threads(void* hp, int n) { solver(static_cast<Handle*>(hp)).asmb.threads = n < 1 ? 1 : n; }
 
 void ikor_set_alpha(void* hp, const double* a) {
   auto* h = static_cast<Handle*>(hp);
@@ -235,6 +237,7 @@ int ikor_load_control(void* hp, int steps, double tbegin, double tend, double to
     S.nr.cgTol     = cgTol;
     S.nr.cgMaxIter = cgMaxIter;
     S.nr.threads   = threads;
+    S.asmb.threads = threads;
     std::vector<double> uu(u, u + h->P.ndof());
     double lam = *lambda;
     auto ci    = loadControl(S, uu, lam, steps, tbegin, tend, predictor);
@@ -261,6 +264,7 @@ int ikor_newton(void* hp, double* u, double lambda, double stepSize, double tol,
     S.nr.linSolver = linSolver;
     S.nr.cgTol     = cgTol;
     S.nr.threads   = threads;
+    S.asmb.threads = threads;
     std::vector<double> uu(u, u + h->P.ndof());
     double lam = lambda;
     auto info  = S.solve(uu, lam, stepSize);

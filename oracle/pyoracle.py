@@ -40,6 +40,7 @@ def lib():
         L.ikor_set_traction.argtypes = [P, I, P, P, P, P]
         L.ikor_add_point_load.argtypes = [P, I64, D]
         L.ikor_set_faithful.argtypes = [P, I]
+        L.ikor_set_threads.argtypes = [P, I]
         L.ikor_set_alpha.argtypes = [P, P]
         L.ikor_get_alpha.argtypes = [P, P]
         L.ikor_pattern_nnz.restype = I64
@@ -131,6 +132,10 @@ class Oracle:
 
     def set_faithful(self, on):
         lib().ikor_set_faithful(self.h, 1 if on else 0)
+
+    def set_threads(self, n):
+        """Element loops of matrix()/vector() on n OpenMP threads over an element coloring (1: serial)."""
+        lib().ikor_set_threads(self.h, int(n))
 
     def alpha(self):
         a = np.zeros(self.ne * self.neas)

@@ -333,3 +333,111 @@ def test_stress_at_points_closed_form(kind):
         for k, ref in ((0, S), (1, tau), (2, tau / np.linalg.det(F))):
             out = o.stress_at(u, xi, k)
             assert np.allclose(out, voigt(ref)[None, :], rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+# ---- closed forms that fix the traction magnitude and the energy's additive constant ----------------------------
+# traction.hh:106-138: R_a -= N_a t w |J_Gamma| over the loaded faces with the one-point face rule. At u = 0 the
+# internal forces vanish (S(E = 0) = 0 for Neo-Hooke and SVK), so the RAW residual is minus the consistent nodal
+# load: its resultant is -t A, and on a face tiled by rectangles of sides hx, hy each face gives t hx hy / 4 to
+# each of its corners (N_a = 1/4 at the face centre). A wrong face measure, weight or N_a fails one of these.
+
+def _rot(seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).standard_normal((3, 3)))
+    return q * np.sign(np.diag(r))
+
+
+@pytest.mark.parametrize("kind", [po.NEOHOOKE, po.SVK])
+@pytest.mark.parametrize("case", ["top", "bending", "rotated"])
+def test_traction_resultant_and_nodal_split_closed_form(kind, case):
+    t = np.array([3.0, -2.0, 5.0])
+    if case == "bending":   # x = 2 face of a [0,2]x[0,2]x[0,1] block: area 2
+        n, L, axis = (3, 4, 2), (2.0, 2.0, 1.0), 0
+    else:                   # z = Lz face of a box with unequal sides: area 2.0 * 1.5
+        n, L, axis = (4, 3, 2), (2.0, 1.5, 0.7), 2
+    X, conn = po.structured_mesh(list(n), list(L), 0.08, 11)
+    Q = _rot(5) if case == "rotated" else np.eye(3)
+    Xr = X @ Q.T
+    lam, mu = po.lame(1000.0, 0.3)
+    o = po.Oracle(Xr, conn, kind, lam, mu, 0, 2)
+    o.set_traction(pb.face_elements(list(n), axis, 1), axis, 1, t)
+    R = o.vector(np.zeros(X.size), 1.0, po.RAW).reshape(-1, 3)
+    others = [k for k in range(3) if k != axis]
+    area = L[others[0]] * L[others[1]]
+    assert np.allclose(R.sum(axis=0), -t * area, rtol=1e-13, atol=0)
+    h = [L[k] / n[k] for k in others]
+    on = np.abs(X[:, axis] - L[axis]) < 1e-9
+    # number of loaded faces around each face node: 4 inside the face, 2 on its edges, 1 at its corners
+    cnt = np.ones(len(X))
+    for k, hk in zip(others, h):
+        interior = (X[:, k] > 1e-9) & (X[:, k] < L[k] - 1e-9)
+        cnt *= np.where(interior, 2.0, 1.0)
+    expect = np.where(on[:, None], -cnt[:, None] * h[0] * h[1] / 4.0 * t[None, :], 0.0)
+    assert np.allclose(R, expect, rtol=1e-12, atol=1e-13 * np.abs(t).max())
+
+
+def test_traction_resultant_2d_closed_form():
+    """Plane Q1 mesh, traction on the right edge x = 4 (length 1): resultant -t, end nodes carry t h / 2."""
+    X, conn = po.structured_mesh([4, 5], [4.0, 1.0], 0.08, 2)
+    lam, mu = po.lame(1000.0, 0.3)
+    o = po.Oracle(X, conn, po.NEOHOOKE, lam, mu, 0, 2)
+    t = np.array([0.0, -1.7])
+    o.set_traction(pb.face_elements([4, 5], 0, 1), 0, 1, t)
+    R = o.vector(np.zeros(X.size), 1.0, po.RAW).reshape(-1, 2)
+    assert np.allclose(R.sum(axis=0), -t * 1.0, rtol=1e-13, atol=1e-15)
+    right = np.abs(X[:, 0] - 4.0) < 1e-9
+    ends = right & ((X[:, 1] < 1e-9) | (X[:, 1] > 1 - 1e-9))
+    assert np.allclose(R[ends, 1], 1.7 * 0.2 / 2, rtol=1e-12)
+    assert np.allclose(R[right & ~ends, 1], 1.7 * 0.2, rtol=1e-12)
+    assert np.all(R[~right] == 0.0)
+
+
+def _psi(kind, F, lam, mu):
+    """Stored energy per reference volume at a homogeneous F (neohooke.hh:79-88, svk.hh:78-91)."""
+    C = F.T @ F
+    if kind == po.NEOHOOKE:
+        lnJ = np.log(np.sqrt(np.linalg.det(C)))
+        return mu / 2.0 * (np.trace(C) - 3.0 - 2.0 * lnJ) + lam / 2.0 * lnJ * lnJ
+    E = 0.5 * (C - np.eye(3))
+    return lam / 2.0 * np.trace(E) ** 2 + mu * np.sum(E * E)
+
+
+@pytest.mark.parametrize("kind", [po.NEOHOOKE, po.SVK])
+def test_energy_closed_forms(kind):
+    """Pi(u = (F - I) X, lambda = 0) = psi(F) V on a jittered block (every Gauss point carries the same F and the
+    element volumes sum to the block's), Pi(0) = 0 (psi(I) = 0: a dropped constant fails), and a rigid translation
+    c under traction t on a face of area A has Pi = -t.c A (traction.hh:71-104)."""
+    n, L = [3, 2, 4], [1.2, 0.8, 1.5]
+    X, conn = po.structured_mesh(n, L, 0.1, 7)
+    lam, mu = po.lame(1000.0, 0.3)
+    o = po.Oracle(X, conn, kind, lam, mu, 0, 2)
+    t = np.array([2.0, -1.0, 4.0])
+    o.set_traction(pb.face_elements(n, 2, 1), 2, 1, t)
+    F = np.array([[1.05, 0.02, -0.03], [0.01, 0.97, 0.04], [-0.02, 0.03, 1.02]])
+    u = (X @ (F - np.eye(3)).T).ravel()
+    V = L[0] * L[1] * L[2]
+    assert abs(o.energy(u, 0.0) - _psi(kind, F, lam, mu) * V) <= 1e-12 * abs(_psi(kind, F, lam, mu) * V)
+    assert o.energy(np.zeros(X.size), 0.0) == 0.0
+    assert o.energy(np.zeros(X.size), 1.0) == 0.0
+    c = np.array([0.3, -0.2, 0.1])
+    assert abs(o.energy(np.tile(c, len(X)), 1.0) - (-(t @ c) * L[0] * L[1])) <= 1e-13
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_colored_threaded_assembly_equals_serial(dim):
+    """The optimized CPU timing variant (element coloring, OpenMP per color) assembles the same K and R as the
+    serial loop, up to summation order."""
+    p = pb.hex_block((5, 4, 3), jitter=0.08, seed=9) if dim == 3 else None
+    if dim == 3:
+        X, conn, fixed = p.X, p.conn, p.fixed
+    else:
+        X, conn = po.structured_mesh([6, 4], [3.0, 1.0], 0.08, 9)
+        fixed = pb.clamp_flags(X, 0, 0.0)
+    lam, mu = po.lame(1000.0, 0.3)
+    o = po.Oracle(X, conn, po.NEOHOOKE, lam, mu, 0, 2)
+    o.set_fixed(fixed)
+    u = 0.01 * np.random.default_rng(3).standard_normal(X.size) * (1 - fixed)
+    K1, R1 = o.matrix(u, 1.0), o.vector(u, 1.0)
+    o.set_threads(4)
+    K4, R4 = o.matrix(u, 1.0), o.vector(u, 1.0)
+    assert np.allclose(K4, K1, rtol=1e-13, atol=1e-13 * np.abs(K1).max())
+    assert np.allclose(R4, R1, rtol=1e-13, atol=1e-13 * np.abs(R1).max())
