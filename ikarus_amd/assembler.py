@@ -42,8 +42,8 @@ class Device:
     def __init__(self, problem):
         L = capi.lib()
         p = problem
-        if p.dim != 3:
-            raise IkError(capi.IK_ENOTIMPL, "the device path supports 3D hex8 meshes only")
+        if p.dim not in (2, 3):
+            raise IkError(capi.IK_ENOTIMPL, "the device path supports 3D hex8 and 2D plane-strain Q1 meshes")
         self.problem = p
         self._keep = dict(
             X=np.ascontiguousarray(p.X, dtype=np.float64),
@@ -57,7 +57,7 @@ class Device:
         )
         k = self._keep
         d = capi.ProblemDesc()
-        d.dim = 3
+        d.dim = p.dim
         d.num_nodes = p.X.shape[0]
         d.num_elements = p.conn.shape[0]
         d.X = ptr(k["X"])
@@ -72,7 +72,7 @@ class Device:
         d.face_axis = ptr(k["fa"]) if len(k["fe"]) else None
         d.face_side = ptr(k["fs"]) if len(k["fe"]) else None
         for i in range(3):
-            d.traction[i] = float(p.traction[i])
+            d.traction[i] = float(p.traction[i]) if i < len(p.traction) else 0.0
         d.num_point_loads = len(k["pd"])
         d.point_dof = ptr(k["pd"]) if len(k["pd"]) else None
         d.point_coef = ptr(k["pc"]) if len(k["pd"]) else None
@@ -123,9 +123,10 @@ class Device:
         return e.value
 
     def stress_at(self, xi, kind=0):
-        """Per element, the PK2 (kind 0), Kirchhoff (1) or Cauchy (2) stress at local point xi, Voigt 6."""
+        """Per element, the PK2 (kind 0), Kirchhoff (1) or Cauchy (2) stress at local point xi, Voigt 6
+        ([11, 22, 12] in plane strain)."""
         xi = np.ascontiguousarray(xi, dtype=np.float64)
-        out = np.zeros((self.problem.ne, 6))
+        out = np.zeros((self.problem.ne, 6 if self.problem.dim == 3 else 3))
         self.check(capi.lib().ik_stress_at(self.h, int(kind), ptr(xi), ptr(out), capi.HOST))
         return out
 
@@ -162,8 +163,9 @@ class Device:
         return outer, inner, vals
 
     def element(self, e):
-        K = np.zeros((24, 24))
-        R = np.zeros(24)
+        nd = self.problem.conn.shape[1] * self.problem.dim
+        K = np.zeros((nd, nd))
+        R = np.zeros(nd)
         self.check(capi.lib().ik_element(self.h, int(e), ptr(K), ptr(R)))
         return K, R
 

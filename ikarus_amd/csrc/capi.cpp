@@ -71,6 +71,120 @@ inline int grid(int64_t n) {
   return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 4096);
 }
 
+// ---- plane strain as a one-layer extrusion (see ik_handle::plane2d) --------------------------------------
+constexpr double kSqrt2 = 1.4142135623730950488;
+
+// b (3D, both layers) <- s a (2D, FlatInterleaved 2v + k); the z components are zero
+__global__ void expand2dKernel(const double* __restrict__ a, int64_t nn2, double s, double* __restrict__ b) {
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nn2; v += (int64_t)gridDim.x * 256) {
+    const double x = a[2 * v] * s, y = a[2 * v + 1] * s;
+    double* b0 = b + 3 * v;
+    double* b1 = b + 3 * (v + nn2);
+    b0[0] = x;
+    b0[1] = y;
+    b0[2] = 0.0;
+    b1[0] = x;
+    b1[1] = y;
+    b1[2] = 0.0;
+  }
+}
+// a (2D) <- s0 (layer 0 of b) + s1 (layer 1 of b)
+__global__ void contract2dKernel(const double* __restrict__ b, int64_t nn2, double s0, double s1,
+                                 double* __restrict__ a) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < 2 * nn2; i += (int64_t)gridDim.x * 256) {
+    const int64_t v = i >> 1, k = i & 1;
+    a[i] = s0 * b[3 * v + k] + s1 * b[3 * (v + nn2) + k];
+  }
+}
+
+// 3D device vector dst <- s * (2D vector src, host or device)
+void expandIn(ik_handle* h, const double* src, int where, double s, double* dst) {
+  const int64_t n2 = 2 * h->nn2;
+  if (where != IK_DEVICE) {
+    h->map2d.ensure(n2);
+    IK_HIP(hipMemcpyAsync(h->map2d.p, src, n2 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    src = h->map2d.p;
+  }
+  hipLaunchKernelGGL(expand2dKernel, dim3(grid(h->nn2)), dim3(256), 0, h->stream, src, h->nn2, s, dst);
+}
+// 2D vector dst (host or device) <- s0 layer 0 + s1 layer 1 of the 3D device vector src
+void contractOut(ik_handle* h, const double* src, double s0, double s1, double* dst, int where) {
+  const int64_t n2 = 2 * h->nn2;
+  h->map2d.ensure(n2);
+  hipLaunchKernelGGL(contract2dKernel, dim3(grid(n2)), dim3(256), 0, h->stream, src, h->nn2, s0, s1, h->map2d.p);
+  IK_HIP(hipMemcpyAsync(dst, h->map2d.p, n2 * sizeof(double),
+                        where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
+  IK_HIP(hipStreamSynchronize(h->stream));
+}
+
+// owning storage of the extruded description
+struct Extruded {
+  std::vector<double> X;
+  std::vector<int32_t> conn;
+  std::vector<uint8_t> fixed;
+  std::vector<int64_t> pdof;
+  std::vector<double> pcoef;
+};
+
+// The 3D description of a plane-strain Q1 problem: layer l of 2D node v is node v + l nn (z = l sqrt(2)); hex
+// corner a (Dune order) is quad corner a & 3 on layer a >> 2; u_z is fixed on every node; an edge load on the
+// quad edge normal to axis 0 or 1 is the traction on the hex face normal to the same axis; a 2D point load puts
+// coef / sqrt(2) on both layers.
+ik_problem_desc extrude2d(ik_handle* h, const ik_problem_desc& d, Extruded& ex) {
+  require(d.num_nodes > 0 && d.num_elements > 0 && d.X && d.conn, IK_EINVAL, "empty mesh");
+  require(d.eas_modes == 0, IK_ENOTIMPL, "plane strain on the device has no EAS modes");
+  require(d.num_nodes * 6 < (int64_t)INT32_MAX, IK_EINVAL, "too many DOFs for int32 indices");
+  const int64_t nn = d.num_nodes, ne = d.num_elements;
+  h->plane2d = true;
+  h->nn2     = nn;
+  ex.X.resize((size_t)nn * 6);
+  for (int64_t v = 0; v < nn; ++v)
+    for (int l = 0; l < 2; ++l) {
+      double* x = &ex.X[(size_t)(v + l * nn) * 3];
+      x[0]      = d.X[2 * v];
+      x[1]      = d.X[2 * v + 1];
+      x[2]      = l * kSqrt2;
+    }
+  h->hconn2.assign(d.conn, d.conn + ne * 4);
+  ex.conn.resize((size_t)ne * 8);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int a = 0; a < 8; ++a) {
+      const int32_t v = d.conn[e * 4 + (a & 3)];
+      require(v >= 0 && v < nn, IK_EINVAL, "connectivity index out of range");
+      ex.conn[(size_t)e * 8 + a] = v + (int32_t)((a >> 2) * nn);
+    }
+  if (d.dirichlet)
+    h->hfixed2.assign(d.dirichlet, d.dirichlet + nn * 2);
+  else
+    h->hfixed2.assign(nn * 2, 0);
+  ex.fixed.resize((size_t)nn * 6);
+  for (int64_t v = 0; v < nn; ++v)
+    for (int l = 0; l < 2; ++l)
+      for (int k = 0; k < 3; ++k) ex.fixed[(size_t)(v + l * nn) * 3 + k] = k < 2 ? h->hfixed2[2 * v + k] : 1;
+  for (int64_t f = 0; f < d.num_faces; ++f)
+    require(d.face_axis && d.face_axis[f] >= 0 && d.face_axis[f] < 2, IK_EINVAL, "bad traction edge of a 2D mesh");
+  for (int64_t i = 0; i < d.num_point_loads; ++i) {
+    require(d.point_dof && d.point_coef && d.point_dof[i] >= 0 && d.point_dof[i] < 2 * nn, IK_EINVAL,
+            "point load dof out of range");
+    const int64_t v = d.point_dof[i] / 2, k = d.point_dof[i] % 2;
+    for (int l = 0; l < 2; ++l) {
+      ex.pdof.push_back(3 * (v + l * nn) + k);
+      ex.pcoef.push_back(d.point_coef[i] / kSqrt2);
+    }
+  }
+  ik_problem_desc o = d;
+  o.dim             = 3;
+  o.num_nodes       = 2 * nn;
+  o.X               = ex.X.data();
+  o.conn            = ex.conn.data();
+  o.dirichlet       = ex.fixed.data();
+  o.traction[2]     = 0.0;
+  o.num_point_loads = (int64_t)ex.pdof.size();
+  o.point_dof       = ex.pdof.data();
+  o.point_coef      = ex.pcoef.data();
+  return o;
+}
+
 void copyIn(ik_handle* h, double* dst, const double* src, int64_t n, int where) {
   IK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         h->stream));
@@ -271,11 +385,19 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
 
 extern "C" {
 
-int ik_create(ik_handle** out, const ik_problem_desc* d) {
-  if (!out || !d) return IK_EINVAL;
+int ik_create(ik_handle** out, const ik_problem_desc* d0) {
+  if (!out || !d0) return IK_EINVAL;
   auto* h = new ik_handle;
   const int rc = guard(h, [&] {
-    require(d->dim == 3, IK_ENOTIMPL, "the device path supports 3D hex8 meshes only");
+    require(d0->dim == 3 || d0->dim == 2, IK_ENOTIMPL,
+            "dim must be 3 (trilinear hex8) or 2 (bilinear Q1 in plane strain)");
+    Extruded ex;
+    ik_problem_desc d3;
+    const ik_problem_desc* d = d0;
+    if (d0->dim == 2) {
+      d3 = extrude2d(h, *d0, ex);
+      d  = &d3;
+    }
     require(d->num_nodes > 0 && d->num_elements > 0 && d->X && d->conn, IK_EINVAL, "empty mesh");
     require(d->eas_modes == 0 || d->eas_modes == 21, IK_ENOTIMPL, "eas_modes must be 0 or 21");
     require(d->material == IK_NEOHOOKE || d->material == IK_SVK, IK_EINVAL, "unknown material");
@@ -375,12 +497,19 @@ void ik_destroy(ik_handle* h) {
 
 const char* ik_last_error(const ik_handle* h) { return h ? h->err.c_str() : "null handle"; }
 int64_t ik_bad_element(const ik_handle* h) { return h ? h->bad_elem : -1; }
-int64_t ik_size(const ik_handle* h) { return h ? h->ndof : -1; }
-int64_t ik_reduced_size(const ik_handle* h) { return h ? h->ndof - h->nfixed : -1; }
+int64_t ik_size(const ik_handle* h) { return h ? (h->plane2d ? 2 * h->nn2 : h->ndof) : -1; }
+int64_t ik_reduced_size(const ik_handle* h) {
+  if (!h) return -1;
+  if (!h->plane2d) return h->ndof - h->nfixed;
+  return 2 * h->nn2 - (int64_t)std::count_if(h->hfixed2.begin(), h->hfixed2.end(), [](uint8_t f) { return f != 0; });
+}
 
 int ik_set_requirement(ik_handle* h, const double* u, double lambda, int where) {
   return guard(h, [&] {
-    if (u) copyIn(h, h->u.p, u, h->ndof, where);
+    if (u && h->plane2d)
+      expandIn(h, u, where, 1.0, h->u.p);
+    else if (u)
+      copyIn(h, h->u.p, u, h->ndof, where);
     h->lambda        = lambda;
     h->bound         = true;
     h->kValid        = false;
@@ -395,13 +524,22 @@ int ik_set_dirichlet_values(ik_handle* h, const double* ubar, int where) {
       h->hasUbar = false;
       return;
     }
-    std::vector<double> hb(h->ndof);
+    const int64_t nIn = h->plane2d ? 2 * h->nn2 : h->ndof;
+    std::vector<double> hb(nIn);
     if (where == IK_DEVICE)
-      IK_HIP(hipMemcpy(hb.data(), ubar, h->ndof * sizeof(double), hipMemcpyDeviceToHost));
+      IK_HIP(hipMemcpy(hb.data(), ubar, nIn * sizeof(double), hipMemcpyDeviceToHost));
     else
-      std::copy(ubar, ubar + h->ndof, hb.begin());
-    for (int64_t i = 0; i < h->ndof; ++i)
-      require(hb[i] == 0.0 || h->hfixed[i], IK_EINVAL, "inhomogeneous Dirichlet value on an unconstrained DOF");
+      std::copy(ubar, ubar + nIn, hb.begin());
+    const std::vector<uint8_t>& fl = h->plane2d ? h->hfixed2 : h->hfixed;
+    for (int64_t i = 0; i < nIn; ++i)
+      require(hb[i] == 0.0 || fl[i], IK_EINVAL, "inhomogeneous Dirichlet value on an unconstrained DOF");
+    if (h->plane2d) {  // both layers carry the 2D values
+      std::vector<double> b3(h->ndof, 0.0);
+      for (int64_t v = 0; v < h->nn2; ++v)
+        for (int l = 0; l < 2; ++l)
+          for (int k = 0; k < 2; ++k) b3[3 * (v + l * h->nn2) + k] = hb[2 * v + k];
+      hb.swap(b3);
+    }
     h->ubar.upload(hb.data(), hb.size(), h->stream);
     IK_HIP(hipStreamSynchronize(h->stream));
     h->hasUbar = true;
@@ -413,6 +551,7 @@ int ik_energy(ik_handle* h, double* energy) {
     require(energy != nullptr, IK_EINVAL, "null argument");
     requireBound(h);
     *energy = ik::energy(h);
+    if (h->plane2d) *energy /= kSqrt2;  // the extrusion's thickness is sqrt(2)
   });
 }
 
@@ -420,6 +559,21 @@ int ik_stress_at(ik_handle* h, int kind, const double xi[3], double* out, int wh
   return guard(h, [&] {
     require(xi != nullptr && out != nullptr, IK_EINVAL, "null argument");
     requireBound(h);
+    if (h->plane2d) {  // xi in [0,1]^2; plane-strain Voigt [11, 22, 12] of the 3D stress at the mid-plane
+      const double x3[3] = {xi[0], xi[1], 0.5};
+      h->tmp.ensure((size_t)h->ne * 6);
+      ik::stressAt(h, kind, x3, h->tmp.p);
+      std::vector<double> s6((size_t)h->ne * 6), s3((size_t)h->ne * 3);
+      copyOut(h, s6.data(), h->tmp.p, h->ne * 6, IK_HOST);
+      for (int64_t e = 0; e < h->ne; ++e) {
+        s3[e * 3]     = s6[e * 6];
+        s3[e * 3 + 1] = s6[e * 6 + 1];
+        s3[e * 3 + 2] = s6[e * 6 + 5];
+      }
+      IK_HIP(hipMemcpy(out, s3.data(), s3.size() * sizeof(double),
+                       where == IK_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+      return;
+    }
     if (where == IK_DEVICE) {
       ik::stressAt(h, kind, xi, out);
     } else {
@@ -432,7 +586,10 @@ int ik_stress_at(ik_handle* h, int kind, const double xi[3], double* out, int wh
 
 int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where) {
   return guard(h, [&] {
-    if (u) copyOut(h, u, h->u.p, h->ndof, where);
+    if (u && h->plane2d)
+      contractOut(h, h->u.p, 1.0, 0.0, u, where);
+    else if (u)
+      copyOut(h, u, h->u.p, h->ndof, where);
     if (lambda) *lambda = h->lambda;
   });
 }
@@ -440,6 +597,21 @@ int ik_get_requirement(ik_handle* h, double* u, double* lambda, int where) {
 // copies the handle's residual out: all DOFs (RAW/FULL) or the free DOFs only (REDUCED)
 static void residualOut(ik_handle* h, int dbc, double* R, int where) {
   if (!R) return;  // assembled into the handle's residual only
+  if (h->plane2d) {  // each layer holds R_2D / sqrt(2)
+    const double s = 1.0 / kSqrt2;
+    if (dbc != IK_DBC_REDUCED) {
+      contractOut(h, h->R.p, s, s, R, where);
+      return;
+    }
+    std::vector<double> r2(2 * h->nn2), rr;
+    contractOut(h, h->R.p, s, s, r2.data(), IK_HOST);
+    for (int64_t i = 0; i < 2 * h->nn2; ++i)
+      if (!h->hfixed2[i]) rr.push_back(r2[i]);
+    if (rr.empty()) return;
+    IK_HIP(hipMemcpy(R, rr.data(), rr.size() * sizeof(double),
+                     where == IK_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+    return;
+  }
   if (dbc != IK_DBC_REDUCED) {
     copyOut(h, R, h->R.p, h->ndof, where);
     return;
@@ -493,10 +665,90 @@ int ik_assemble(ik_handle* h, int dbc, int precision, double* R, int where) {
   });
 }
 
+// Eigen CSC of the plane-strain matrix from the extrusion's blocks: column 2i+k holds rows 2j+l for the 2D
+// neighbours j of node i (the layer-0 columns of 3D block row i, sorted), K2 = sqrt(2) (K(j, i) + K(j, i + nn2))
+// restricted to the in-plane components; FULL then sets the constrained rows and columns of the identity
+// (simpleassemblers.inl:152-169), REDUCED drops them.
+static void csc2d(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* outer, int32_t* inner, double* values) {
+  const bool reduced = dbc == IK_DBC_REDUCED;
+  const int64_t nn2 = h->nn2, nd = 2 * nn2;
+  const auto& fx    = h->hfixed2;
+  std::vector<int64_t> cb(nd + 1, 0);
+  for (int64_t i = 0; i < nd; ++i) cb[i + 1] = cb[i] + (fx[i] ? 1 : 0);
+  const int64_t N = reduced ? nd - cb[nd] : nd;
+  std::vector<int64_t> colStart(N + 1, 0);
+  for (int64_t i = 0; i < nn2; ++i)
+    for (int k = 0; k < 2; ++k) {
+      const int64_t c = 2 * i + k;
+      if (reduced && fx[c]) continue;
+      int64_t cnt = 0;
+      for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1] && h->hcolidx[b] < nn2; ++b)
+        for (int l = 0; l < 2; ++l) cnt += !(reduced && fx[2 * (int64_t)h->hcolidx[b] + l]);
+      colStart[(reduced ? c - cb[c] : c) + 1] = cnt;
+    }
+  for (int64_t c = 0; c < N; ++c) colStart[c + 1] += colStart[c];
+  *n   = N;
+  *nnz = colStart[N];
+  if (!outer && !inner && !values) return;
+  std::vector<double> K;
+  if (values) {
+    require(h->kValid, IK_EINVAL, "no matrix assembled");
+    require((dbc == IK_DBC_FULL) == (h->kDbc == IK_DBC_FULL), IK_EINVAL,
+            "requested DBC layout differs from the assembled matrix");
+    K.resize((size_t)h->nnzb * 9);
+    if (h->kPrecision == IK_FP64) {
+      IK_HIP(hipMemcpyAsync(K.data(), h->K64.p, K.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      IK_HIP(hipStreamSynchronize(h->stream));
+    } else {
+      std::vector<float> Kf(K.size());
+      IK_HIP(hipMemcpyAsync(Kf.data(), h->K32.p, Kf.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+      IK_HIP(hipStreamSynchronize(h->stream));
+      for (size_t i = 0; i < K.size(); ++i) K[i] = Kf[i];
+    }
+  }
+  auto blockAt = [&](int64_t r, int64_t c) {  // index of block (r, c) of the 3D pattern
+    const auto first = h->hcolidx.begin() + h->hrowptr[r], last = h->hcolidx.begin() + h->hrowptr[r + 1];
+    const auto it    = std::lower_bound(first, last, (int32_t)c);
+    require(it != last && *it == c, IK_EINVAL, "extruded pattern misses a block");
+    return (int64_t)(it - h->hcolidx.begin());
+  };
+  if (outer)
+    for (int64_t c = 0; c <= N; ++c) outer[c] = (int32_t)colStart[c];
+  for (int64_t i = 0; i < nn2; ++i)
+    for (int k = 0; k < 2; ++k) {
+      const int64_t c = 2 * i + k;
+      if (reduced && fx[c]) continue;
+      int64_t p = colStart[reduced ? c - cb[c] : c];
+      for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1] && h->hcolidx[b] < nn2; ++b) {
+        const int64_t j = h->hcolidx[b];
+        int64_t b0 = -1, b1 = -1;
+        if (values) {
+          b0 = blockAt(j, i);
+          b1 = blockAt(j, i + nn2);
+        }
+        for (int l = 0; l < 2; ++l) {
+          const int64_t r = 2 * j + l;
+          if (reduced && fx[r]) continue;
+          if (inner) inner[p] = (int32_t)(reduced ? r - cb[r] : r);
+          if (values) {
+            double v = kSqrt2 * (K[(size_t)b0 * 9 + l * 3 + k] + K[(size_t)b1 * 9 + l * 3 + k]);
+            if (dbc == IK_DBC_FULL && (fx[r] || fx[c])) v = (r == c) ? 1.0 : 0.0;
+            values[p] = v;
+          }
+          ++p;
+        }
+      }
+    }
+}
+
 int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* outer, int32_t* inner, double* values) {
   return guard(h, [&] {
     ensurePattern(h);
     require(dbc >= 0 && dbc <= 2, IK_EINVAL, "bad DBC option");
+    if (h->plane2d) {
+      csc2d(h, dbc, n, nnz, outer, inner, values);
+      return;
+    }
     const bool reduced = dbc == IK_DBC_REDUCED;
     // block transpose map and per-node block counts
     const int64_t nn = h->nn;
@@ -568,6 +820,7 @@ int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* oute
 int ik_matrix_bsr(ik_handle* h, int64_t* nbr, int64_t* nnzb, const int32_t** rowptr, const int32_t** colidx,
                   const void** values, int* precision) {
   return guard(h, [&] {
+    require(!h->plane2d, IK_ENOTIMPL, "the block view is the 3D storage; a plane-strain handle exports CSC only");
     ensurePattern(h);
     if (nbr) *nbr = h->nn;
     if (nnzb) *nnzb = h->nnzb;
@@ -582,6 +835,20 @@ int ik_element(ik_handle* h, int64_t e, double* Ke, double* Re) {
   return guard(h, [&] {
     requireBound(h);
     ensurePattern(h);
+    if (h->plane2d) {  // the Q1 element from its hex: K2 = sqrt(2) (K(a, b) + K(a, b + 4)), R2 = (R(a) + R(a + 4)) / sqrt(2)
+      std::vector<double> K3(24 * 24), R3(24);
+      ik::elementHost(h, e, Ke ? K3.data() : nullptr, Re ? R3.data() : nullptr);
+      for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 2; ++k) {
+          if (Re) Re[2 * a + k] = (R3[3 * a + k] + R3[3 * (a + 4) + k]) / kSqrt2;
+          if (Ke)
+            for (int b = 0; b < 4; ++b)
+              for (int l = 0; l < 2; ++l)
+                Ke[(2 * a + k) * 8 + 2 * b + l] =
+                    kSqrt2 * (K3[(3 * a + k) * 24 + 3 * b + l] + K3[(3 * a + k) * 24 + 3 * (b + 4) + l]);
+        }
+      return;
+    }
     ik::elementHost(h, e, Ke, Re);
   });
 }
@@ -621,9 +888,15 @@ int ik_solve(ik_handle* h, const double* rhs, double* x, int where, const ik_sol
     const int64_t n = h->ndof;
     h->rhs.ensure(n);
     h->px.ensure(n);
-    copyIn(h, h->rhs.p, rhs, n, where);
+    if (h->plane2d)  // K3 [x; x] = [b; b] / sqrt(2) is K2 x = b on the symmetric layer pair
+      expandIn(h, rhs, where, 1.0 / kSqrt2, h->rhs.p);
+    else
+      copyIn(h, h->rhs.p, rhs, n, where);
     linearSolve(h, h->rhs.p, h->px.p, o, o->rtol, info);
-    copyOut(h, x, h->px.p, n, where);
+    if (h->plane2d)
+      contractOut(h, h->px.p, 1.0, 0.0, x, where);
+    else
+      copyOut(h, x, h->px.p, n, where);
   });
 }
 
@@ -631,6 +904,7 @@ int ik_newton_solve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_n
   return guard(h, [&] {
     require(s && info, IK_EINVAL, "null argument");
     newtonSolve(h, s, stepSize, info);
+    if (h->plane2d) info->correction_norm /= kSqrt2;  // both layers carry the 2D correction
   });
 }
 
@@ -639,6 +913,8 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
   return guard(h, [&] {
     require(c && s && info, IK_EINVAL, "null argument");
     require(c->load_steps > 0, IK_EINVAL, "Number of load steps should be greater than zero.");
+    require(!(h->plane2d && c->predictor == IK_PRED_MLP), IK_ENOTIMPL,
+            "the learned predictor's features are those of a 3D mesh");
     requireBound(h);
     *info           = ik_control_info{};
     const double dl = (c->tend - c->tbegin) / c->load_steps;
@@ -729,6 +1005,7 @@ int ik_path_following(ik_handle* h, const ik_pf_config* c, const ik_nr_settings*
     require(c->steps > 0, IK_EINVAL, "Number of load steps should be greater than zero.");
     require(c->step_size > 0.0, IK_EINVAL, "the arc length must be positive");
     require(h->neas == 0, IK_ENOTIMPL, "path following on the device supports plain elements only");
+    require(!h->plane2d, IK_ENOTIMPL, "path following on the device supports 3D meshes only");
     require(!h->hasUbar, IK_ENOTIMPL, "path following on the device has no inhomogeneous Dirichlet values");
     requireBound(h);
     ensurePattern(h);

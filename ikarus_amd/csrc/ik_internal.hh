@@ -96,6 +96,15 @@ struct ik_handle {
   hipEvent_t joinEvent = nullptr;
 
   // problem
+  // Plane strain (dim 2): the Q1 mesh runs as its one-layer hex8 extrusion of thickness sqrt(2) with u_z = 0 on
+  // every node; node v of the 2D mesh is 3D node v (z = 0) and v + nn2 (z = sqrt(2)). A displacement independent
+  // of z has E33 = E13 = E23 = 0 and the 2-point Gauss rule in z is exact for it, so each layer carries
+  // sqrt(2)/2 = 1/sqrt(2) of the 2D residual and the 3D residual norm equals the 2D one.
+  bool plane2d = false;
+  int64_t nn2 = 0;                  // 2D nodes
+  std::vector<uint8_t> hfixed2;     // 2D Dirichlet flags [nn2 * 2]
+  std::vector<int32_t> hconn2;      // 2D connectivity [ne * 4]
+  ik::DevBuf<double> map2d;         // 2D-sized staging for device-pointer arguments
   int64_t nn = 0, ne = 0, ndof = 0;
   int material = IK_NEOHOOKE;
   double lam = 0, mu = 0;

@@ -50,8 +50,15 @@ enum { IK_PK2 = 0, IK_KIRCHHOFF = 1, IK_CAUCHY = 2 };
 
 /* ---- problem description (replaces makeBasis + makeFE(skills(nonLinearElastic, eas, traction))
  *      + DirichletValues + makeSparseFlatAssembler: tests/src/testcantileverbeam.hh:102-151) ---- */
+/* dim 2 is the plane-strain Q1 element (planeStrain(material), vanishingstrain.hh:93-120,148-198, 2x2 Gauss):
+ * X [num_nodes*2], conn [num_elements*4], dirichlet [num_nodes*2], face_axis 0|1 names the loaded edge,
+ * traction[2] is ignored, point_dof / every vector argument uses the 2D DOFs (2*node + component). The device
+ * evaluates it as the mesh's one-layer hex8 extrusion of thickness sqrt(2) with u_z = 0, which has the same
+ * residual norm and Newton iterates; ik_matrix_csc, ik_element, ik_energy and ik_stress_at (Voigt [11,22,12])
+ * return the 2D quantities. Not for dim 2: EAS modes, ik_matrix_bsr, IK_PRED_MLP, ik_path_following
+ * (IK_ENOTIMPL). */
 typedef struct {
-  int32_t dim;                 /* 3: trilinear hex8 with 2x2x2 Gauss (nonlinearelastic.hh:112-125)   */
+  int32_t dim;                 /* 3: trilinear hex8 with 2x2x2 Gauss (nonlinearelastic.hh:112-125); 2: see above */
   int64_t num_nodes;
   int64_t num_elements;
   const double* X;             /* [num_nodes*3] reference coordinates, host                          */

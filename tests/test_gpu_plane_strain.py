@@ -1,0 +1,176 @@
+"""Plane-strain Q1 (dim 2) on the device, through the C ABI, against the 2D FP64 CPU oracle
+(vanishingstrain.hh:93-120,148-198 restated in oracle/ikoracle.hh).
+
+The device runs a 2D mesh as its one-layer hex8 extrusion (thickness sqrt(2), u_z = 0); the C ABI maps vectors,
+the Eigen CSC export and element matrices back to the 2D DOFs. Bars as for 3D: pattern bit-exact, K and R within
+1e-10, converged u within 1e-8, Newton iterations identical; plus the reference's own elastic-strip golden values
+(tests/src/testinhomogeneousdbc.cpp:24-31) reproduced on the GPU.
+"""
+import numpy as np
+import pytest
+
+import ikarus_amd as ia
+import pyoracle as po
+from ikarus_amd import _capi as capi
+from ikarus_amd import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def _oracle_for(p):
+    o = po.Oracle(p.X, p.conn, p.material, p.lam, p.mu, 0, 2)
+    o.set_fixed(p.fixed)
+    if len(p.face_elem):
+        o.set_traction(p.face_elem, p.face_axis, p.face_side, p.traction[:2])
+    for d, c in zip(p.point_dof, p.point_coef):
+        o.add_point_load(int(d), float(c))
+    return o
+
+
+def _jittered_cantilever(material, n=(8, 5), seed=4):
+    p = pb.cantilever2d(n=n, L=(4.0, 1.0), material=material, seed=seed)
+    X, _ = pb.structured_mesh(n, (4.0, 1.0), jitter=0.08, seed=seed)
+    p.X = X
+    return p
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_plane_strain_assembly_matches_oracle(material):
+    p = _jittered_cantilever(material)
+    print(f"case: plane strain assembly material={material}", flush=True)
+    o = _oracle_for(p)
+    dev = ia.Device(p)
+    assert dev.ndof == p.ndof
+    assert int(capi.lib().ik_reduced_size(dev.h)) == int((p.fixed == 0).sum())
+    u = 0.02 * np.random.default_rng(3).standard_normal(p.ndof) * (1 - p.fixed)
+    lam = 0.7
+    dev.set_requirement(u, lam)
+    ug, lg = dev.get_requirement()
+    assert np.array_equal(ug, u) and lg == lam
+    for dbc in (capi.DBC_RAW, capi.DBC_FULL, capi.DBC_REDUCED):
+        R = dev.vector(dbc)
+        Ro = o.vector(u, lam, dbc)
+        assert _rel(R, Ro) < 1e-10, dbc
+        dev.matrix(dbc)
+        outer, inner, vals = dev.csc(dbc)
+        oo, io = o.pattern(dbc == capi.DBC_REDUCED)
+        assert np.array_equal(outer, oo) and np.array_equal(inner, io), dbc
+        assert _rel(vals, o.matrix(u, lam, dbc)) < 1e-10, dbc
+    for e in (0, p.ne // 2, p.ne - 1):
+        K, R = dev.element(e)
+        Ko, Rl = o.element(e, u, lam)
+        assert _rel(K, Ko) < 1e-10 and _rel(R, Rl) < 1e-10, e
+    assert abs(dev.energy() - o.energy(u, lam)) <= 1e-10 * abs(o.energy(u, lam))
+    dev.close()
+
+
+@pytest.mark.parametrize("material", [capi.NEOHOOKE, capi.SVK])
+def test_plane_strain_stress_closed_form(material):
+    """Homogeneous in-plane F: every element carries the plane-strain PK2 stress of C = diag(F2^T F2, 1)."""
+    p = _jittered_cantilever(material)
+    print(f"case: plane strain stress material={material}", flush=True)
+    F = np.array([[1.04, 0.03], [-0.02, 0.97]])
+    u = (p.X @ (F - np.eye(2)).T).ravel()
+    C3 = np.eye(3)
+    C3[:2, :2] = F.T @ F
+    if material == capi.NEOHOOKE:
+        Ci = np.linalg.inv(C3)
+        S = p.mu * (np.eye(3) - Ci) + p.lam * np.log(np.sqrt(np.linalg.det(C3))) * Ci
+    else:
+        E = 0.5 * (C3 - np.eye(3))
+        S = p.lam * np.trace(E) * np.eye(3) + 2 * p.mu * E
+    dev = ia.Device(p)
+    dev.set_requirement(u, 0.0)
+    out = dev.stress_at([0.3, 0.6], 0)
+    assert np.allclose(out, np.array([S[0, 0], S[1, 1], S[0, 1]])[None, :], rtol=1e-11, atol=1e-11 * np.abs(S).max())
+    dev.close()
+
+
+def test_plane_strain_linear_solve_matches_direct():
+    p = _jittered_cantilever(capi.NEOHOOKE)
+    print("case: plane strain linear solve", flush=True)
+    o = _oracle_for(p)
+    dev = ia.Device(p)
+    u = 0.02 * np.random.default_rng(5).standard_normal(p.ndof) * (1 - p.fixed)
+    dev.set_requirement(u, 0.5)
+    dev.matrix(capi.DBC_FULL)
+    b = np.random.default_rng(6).standard_normal(p.ndof) * (1 - p.fixed)
+    x, info = dev.solve(b, ia.solver_opts(rtol=1e-13))
+    assert info.converged == 1
+    assert _rel(x, o.direct_solve(u, 0.5, b)) < 1e-9
+    dev.close()
+
+
+def test_c1_cantilever_load_path_matches_oracle():
+    """Config c1 in full: 64x16 plane-strain Neo-Hooke cantilever, 10 load steps to ||R|| <= 1e-8; the device's PCG
+    (tight tolerance) against the oracle's direct solve: the same Newton iterations per step and u within 1e-8."""
+    p = pb.cantilever2d(seed=0)
+    print("case: c1 load path", flush=True)
+    r = _oracle_for(p).load_control(10, 0.0, 1.0, tol=1e-8)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.LCConfig()
+    cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 10, 0.0, 1.0, capi.PRED_CONSTANT
+    info, iters, res = dev.load_control(cfg, ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-13)))
+    ug, lam = dev.get_requirement()
+    dev.close()
+    assert r["success"] and info.success and abs(lam - 1.0) < 1e-14
+    assert list(iters) == list(r["iterations"])
+    assert np.all(res <= 1e-8)
+    assert _rel(ug, r["u"]) < 1e-8
+
+
+def _strip(material, lam, mu):
+    X, conn = pb.structured_mesh((10, 10), (10.0, 10.0))
+    p = pb.Problem(X, conn, material, lam, mu, name="strip")
+    fixed = pb.clamp_flags(X, 0, 0.0)
+    right = np.nonzero(np.abs(X[:, 0] - 10.0) < 1e-8)[0]
+    fixed[right * 2] = 1
+    fixed[right * 2 + 1] = 1
+    p.fixed = fixed
+    ubar = np.zeros(p.ndof)
+    ubar[right * 2] = 10.0
+    return p, ubar
+
+
+# tests/src/testinhomogeneousdbc.cpp:24-31 (Q1 rows) on the strip problem of tests/src/testelasticstrip.hh:43-223
+@pytest.mark.parametrize("material,lame,iters,uy", [
+    (capi.NEOHOOKE, (24.0, 6.0), 7, 2.207111977584091),
+    (capi.SVK, pb.lame(100.0, 0.3), 6, 1.814746879163122),
+])
+def test_elastic_strip_golden_on_device(material, lame, iters, uy):
+    p, ubar = _strip(material, *lame)
+    print(f"case: elastic strip golden material={material}", flush=True)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    dev.set_dirichlet_values(ubar)
+    cfg = capi.LCConfig()
+    cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 1, 0.0, 1.0, capi.PRED_CONSTANT
+    info, its, res = dev.load_control(cfg, ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-13)))
+    u, _ = dev.get_requirement()
+    dev.close()
+    assert info.success and info.total_iterations == iters
+    v = pb.vertex_at(p.X, (5.0, 0.0))
+    assert abs(u[2 * v + 1] - uy) <= 1e-9 * abs(uy)
+    right = np.nonzero(np.abs(p.X[:, 0] - 10.0) < 1e-8)[0]
+    assert np.allclose(u[2 * right], 10.0, rtol=1e-12)
+
+
+def test_plane_strain_rejects_3d_only_features():
+    p = pb.cantilever2d(n=(4, 2))
+    print("case: plane strain rejections", flush=True)
+    dev = ia.Device(p)
+    with pytest.raises(capi.IkError) as ei:
+        dev.check(capi.lib().ik_matrix_bsr(dev.h, None, None, None, None, None, None))
+    assert ei.value.code == capi.IK_ENOTIMPL
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    cfg = capi.PFConfig()
+    cfg.steps, cfg.step_size = 2, 0.1
+    with pytest.raises(capi.IkError) as ei:
+        dev.path_following(cfg, ia.nr_settings())
+    assert ei.value.code == capi.IK_ENOTIMPL
+    dev.close()

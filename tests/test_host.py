@@ -62,16 +62,34 @@ def test_library_exports_header_symbols():
 
 
 def test_capi_rejects_bad_input_without_gpu_compute():
-    """Validation happens before any device work: a 2D mesh and out-of-range connectivity are refused."""
+    """Validation happens before any device work: an unsupported dimension, EAS modes on a plane-strain mesh,
+    out-of-range connectivity and a 3D traction face on a 2D mesh are refused."""
     lib = capi.lib()
     X = np.zeros((4, 2))
     conn = np.array([[0, 1, 2, 3]], np.int32)
-    d = capi.ProblemDesc()
-    d.dim = 2
-    d.num_nodes, d.num_elements = 4, 1
-    d.X, d.conn = capi.ptr(X), capi.ptr(conn)
-    h = C.c_void_p()
-    rc = lib.ik_create(C.byref(h), C.byref(d))
-    assert rc == capi.IK_ENOTIMPL
-    assert b"3D" in lib.ik_last_error(h)
-    lib.ik_destroy(h)
+
+    def create(dim=2, eas=0, c=conn, faces=None):
+        d = capi.ProblemDesc()
+        d.dim = dim
+        d.num_nodes, d.num_elements = 4, 1
+        d.X, d.conn = capi.ptr(X), capi.ptr(c)
+        d.eas_modes = eas
+        keep = []
+        if faces is not None:
+            fe, fa, fs = (np.array([0], np.int32), np.array([faces], np.int8), np.array([1], np.int8))
+            keep += [fe, fa, fs]
+            d.num_faces, d.face_elem, d.face_axis, d.face_side = 1, capi.ptr(fe), capi.ptr(fa), capi.ptr(fs)
+        h = C.c_void_p()
+        rc = lib.ik_create(C.byref(h), C.byref(d))
+        msg = lib.ik_last_error(h)
+        lib.ik_destroy(h)
+        return rc, msg
+
+    rc, msg = create(dim=4)
+    assert rc == capi.IK_ENOTIMPL and b"dim must be 3" in msg
+    rc, msg = create(eas=4)
+    assert rc == capi.IK_ENOTIMPL and b"EAS" in msg
+    rc, msg = create(c=np.array([[0, 1, 2, 7]], np.int32))
+    assert rc == capi.IK_EINVAL and b"out of range" in msg
+    rc, msg = create(faces=2)
+    assert rc == capi.IK_EINVAL and b"2D mesh" in msg
