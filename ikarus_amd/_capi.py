@@ -29,7 +29,7 @@ EXPORTED = [
     "ik_mlp_set_load",
     "ik_timing_enable", "ik_timing_read", "ik_timing_reset", "ik_synchronize", "ik_assembly_layout",
     "ik_mlp_create", "ik_mlp_destroy", "ik_mlp_last_error", "ik_mlp_params", "ik_mlp_load", "ik_mlp_forward",
-    "ik_mlp_loss_grad", "ik_mlp_train_step",
+    "ik_mlp_loss_grad", "ik_mlp_train_step", "ik_mlp_forward_noalloc",
 ]
 
 
@@ -135,6 +135,7 @@ def lib():
         L.ik_mlp_params.argtypes = [P, P, I]
         L.ik_mlp_load.argtypes = [P, P, I]
         L.ik_mlp_forward.argtypes = [P, P, I64, P, I]
+        L.ik_mlp_forward_noalloc.argtypes = [P, P, I64, P]
         L.ik_mlp_loss_grad.argtypes = [P, P, P, I64, P, P, I]
         L.ik_mlp_train_step.argtypes = [P, P, P, I64, C.c_float, P, I]
         _lib = L

@@ -12,6 +12,7 @@
 //   [ X_v / L (3), t / 40 (3), lam_{n+1}, 40 dlam, v_n / 0.1 (3), v_{n-1} / 0.1 (3) ]
 // Output y (3): v_{n+1} = v_n + 0.1 y, u_pred = u_n + dlam v_{n+1} (clamped DOFs keep u_n).
 #include <cmath>
+#include <mutex>
 
 #include "ik_internal.hh"
 
@@ -37,7 +38,8 @@ __device__ inline uint16_t toBf16(float f) {
   return (uint16_t)(u >> 16);
 }
 
-enum { EPI_STORE = 0, EPI_ATOMIC = 1, EPI_BIAS_SILU = 2, EPI_BIAS = 3 };
+// EPI_SILU_ONLY: C = silu(A B + bias), the pre-activation is not kept (inference)
+enum { EPI_STORE = 0, EPI_ATOMIC = 1, EPI_BIAS_SILU = 2, EPI_BIAS = 3, EPI_SILU_ONLY = 4 };
 
 // C[i][j] = sum_k A(i,k) B(k,j) with A(i,k) = A[i*sai + k*sak], B(k,j) = B[k*sbk + j*sbj].
 // 256 threads, 64x64 block tile, 4 waves of 32x32 (2x2 MFMA tiles), K in steps of 16.
@@ -100,8 +102,13 @@ __global__ __launch_bounds__(256) void gemmBf16(int M, int N, int K, const float
         if constexpr (EPI == EPI_ATOMIC) {
           unsafeAtomicAdd(C + (int64_t)i * ldc + j, v);
         } else {
-          if constexpr (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS) v += bias[j];
-          C[(int64_t)i * ldc + j] = v;
+          if constexpr (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS || EPI == EPI_SILU_ONLY) v += bias[j];
+          if constexpr (EPI == EPI_SILU_ONLY) {
+            const float sg = 1.f / (1.f + __expf(-v));
+            C[(int64_t)i * ldc + j] = v * sg;
+          } else {
+            C[(int64_t)i * ldc + j] = v;
+          }
           if constexpr (EPI == EPI_BIAS_SILU) {
             const float sg = 1.f / (1.f + __expf(-v));
             act[(int64_t)i * ldc + j] = v * sg;
@@ -242,6 +249,16 @@ struct Mlp {
     gemm<EPI_BIAS>((int)n, kOut, kHid, Aa[2].p, kHid, 1, P0 + oW4, 1, kHid, Y.p, kOut, P0 + oB4, nullptr);
   }
 
+  // the same forward through caller-provided buffers: hidden activations ping-pong between a and b (n x 128
+  // each), the output goes to y (n x 3); no pre-activations are kept and nothing is allocated
+  void forwardInto(const float* x, int64_t n, float* a, float* b, float* y) {
+    const float* P0 = P.p;
+    gemm<EPI_SILU_ONLY>((int)n, kHid, kIn, x, kIn, 1, P0 + oW1, 1, kIn, a, kHid, P0 + oB1, nullptr);
+    gemm<EPI_SILU_ONLY>((int)n, kHid, kHid, a, kHid, 1, P0 + oW2, 1, kHid, b, kHid, P0 + oB2, nullptr);
+    gemm<EPI_SILU_ONLY>((int)n, kHid, kHid, b, kHid, 1, P0 + oW3, 1, kHid, a, kHid, P0 + oB3, nullptr);
+    gemm<EPI_BIAS>((int)n, kOut, kHid, a, kHid, 1, P0 + oW4, 1, kHid, y, kOut, P0 + oB4, nullptr);
+  }
+
   // mean-squared-error loss and its gradient w.r.t. all parameters (into G)
   double lossGrad(const float* x, const float* t, int64_t n) {
     forward(x, n);
@@ -338,8 +355,32 @@ void mlpPredict(ik_handle* h, const double* duN, const double* duN1, double lamN
 struct ik_mlp {
   ik::Mlp* m = nullptr;
   hipStream_t s = nullptr;
+  int device = 0;            // the device current at ik_mlp_create: the context's stream and memory live there
   std::string err;
 };
+
+// Scratch of the allocation-free forward (ik_mlp_forward_noalloc): two ping-pong buffers of hidden activations,
+// IK_MLP_NOALLOC_MAX_ROWS x 128 FP32 each (8 MiB), one copy per device held until the process ends. Host code
+// uses only the address hipGetSymbolAddress returns for the device (the array's name on the host is not a device
+// address), resolved once per device at ik_mlp_create with that device current.
+static __device__ __align__(256) float scratch[2 * IK_MLP_NOALLOC_MAX_ROWS * ik::kHid];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+float* g_scratchPtr[kMaxDevices] = {};
+
+float* resolveScratch(int device) {
+  ik::require(device >= 0 && device < kMaxDevices, IK_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[device]) {
+    void* ptr = nullptr;
+    IK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratchPtr[device] = static_cast<float*>(ptr);
+  }
+  return g_scratchPtr[device];
+}
+}  // namespace
 
 namespace {
 template <class F>
@@ -386,6 +427,8 @@ int ik_mlp_create(ik_mlp** out, uint64_t seed) {
   auto* m = new ik_mlp;
   *out    = m;
   return mguard(m, [&] {
+    IK_HIP(hipGetDevice(&m->device));
+    (void)resolveScratch(m->device);  // the device's scratch address for ik_mlp_forward_noalloc
     IK_HIP(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking));
     m->m = ik::mlpCreate(m->s);
     std::vector<float> p;
@@ -433,6 +476,20 @@ int ik_mlp_forward(ik_mlp* m, const float* x, int64_t n, float* y, int where) {
     m->m->forward(xd, n);
     IK_HIP(hipMemcpyAsync(y, m->m->Y.p, n * ik::kOut * sizeof(float),
                           where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, m->s));
+    IK_HIP(hipStreamSynchronize(m->s));
+  });
+}
+
+int ik_mlp_forward_noalloc(ik_mlp* m, const float* x, int64_t n, float* y) {
+  return mguard(m, [&] {
+    ik::require(x != nullptr && y != nullptr, IK_EINVAL, "null argument");
+    ik::require(n > 0 && n <= IK_MLP_NOALLOC_MAX_ROWS, IK_EINVAL,
+                "ik_mlp_forward_noalloc: the batch must have 1 to IK_MLP_NOALLOC_MAX_ROWS (8192) rows");
+    float* s0 = g_scratchPtr[m->device];  // resolved at ik_mlp_create; never the array's host-side name
+    ik::require(s0 != nullptr, IK_EINVAL, "ik_mlp_forward_noalloc: scratch not resolved for this device");
+    float* s1 = s0 + (int64_t)IK_MLP_NOALLOC_MAX_ROWS * ik::kHid;
+    m->m->s = m->s;
+    m->m->forwardInto(x, n, s0, s1, y);
     IK_HIP(hipStreamSynchronize(m->s));
   });
 }

@@ -261,6 +261,13 @@ int ik_mlp_params(ik_mlp* m, float* params, int where);
 int ik_mlp_load(ik_mlp* m, const float* params, int where);
 /* x: [n x 14] features, y: [n x 3] outputs (row-major FP32) */
 int ik_mlp_forward(ik_mlp* m, const float* x, int64_t n, float* y, int where);
+/* Allocation-free forward, for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc); it has no counterpart in the reference. x [n x 14] and
+ * y [n x 3] are device pointers, 1 <= n <= IK_MLP_NOALLOC_MAX_ROWS (else IK_EINVAL). The hidden activations live in
+ * a fixed 8 MiB scratch array per device (2 x IK_MLP_NOALLOC_MAX_ROWS x 128 FP32) that every context on that
+ * device shares: calls that overlap in time on one device race. Results equal ik_mlp_forward's bit for bit. */
+#define IK_MLP_NOALLOC_MAX_ROWS 8192
+int ik_mlp_forward_noalloc(ik_mlp* m, const float* x, int64_t n, float* y);
 /* loss = mean((y - t)^2) over n x 3 entries; grad (nullable): d loss / d params */
 int ik_mlp_loss_grad(ik_mlp* m, const float* x, const float* t, int64_t n, double* loss, float* grad, int where);
 /* loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr */
