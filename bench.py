@@ -511,6 +511,14 @@ def main():
             r, _ = newton_path(pb.hex_block((12, 12, 12), seed=0), "fp64", args.c2_steps)
             out["c2_load_path"]["device_on_cpu_sample_12cubed"] = {k: r[k] for k in ("seconds", "newton_iterations",
                                                                                      "linear_iterations")}
+        # config c1 on the device: the 2D plane-strain cantilever (run as its one-layer hex8 extrusion)
+        r, _ = newton_path(pb.cantilever2d(seed=0), "fp64", 10)
+        r["seconds_per_step"] = round(r["seconds"] / 10, 4)
+        r["workload"] = ("2D plane-strain NH cantilever 64x16 Q1 over 4x1, 10 load steps to ||R|| <= 1e-8, FP64 "
+                         "tangent, Jacobi-PCG to 1e-10 per Newton iteration (the reference's c1 uses a sparse direct "
+                         "solve; see cpu_baseline.c1)")
+        out["c1_load_path"] = r
+        log(f"[bench] c1 on the device: {r['seconds']} s, {r['newton_iterations']} Newton")
         mlp = np.fromfile(args.mlp, dtype=np.float32)
         out["newton"] = evaluate_newton(n, args.newton_steps, mlp, EVAL_SEEDS[:max(1, args.eval_seeds)])
     if not args.no_cpu:

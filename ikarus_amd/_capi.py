@@ -18,6 +18,7 @@ HOST, DEVICE = 0, 1
 PCG_JACOBI_FP64, PCG_BJACOBI_FP32_IR = 0, 1
 PRED_CONSTANT, PRED_LINEAR, PRED_MLP = 0, 1, 2
 PK2, KIRCHHOFF, CAUCHY = 0, 1, 2
+VTK_POINT_DATA, VTK_CELL_DATA = 0, 1
 KERNEL_ASSEMBLE_KR, KERNEL_ASSEMBLE_R, KERNEL_SPMV, KERNEL_MLP, KERNEL_GATHER_K = 0, 1, 2, 3, 4
 
 EXPORTED = [
@@ -29,7 +30,7 @@ EXPORTED = [
     "ik_mlp_set_load",
     "ik_timing_enable", "ik_timing_read", "ik_timing_reset", "ik_synchronize", "ik_assembly_layout",
     "ik_mlp_create", "ik_mlp_destroy", "ik_mlp_last_error", "ik_mlp_params", "ik_mlp_load", "ik_mlp_forward",
-    "ik_mlp_loss_grad", "ik_mlp_train_step", "ik_mlp_forward_noalloc",
+    "ik_mlp_loss_grad", "ik_mlp_train_step", "ik_mlp_forward_noalloc", "ik_write_vtu",
 ]
 
 
@@ -42,6 +43,10 @@ class ProblemDesc(C.Structure):
         ("traction", C.c_double * 3), ("num_point_loads", C.c_int64), ("point_dof", C.c_void_p),
         ("point_coef", C.c_void_p),
     ]
+
+
+class VtkField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("values", C.c_void_p), ("components", C.c_int32), ("cell", C.c_int32)]
 
 
 class SolverOpts(C.Structure):
@@ -69,7 +74,7 @@ class LCConfig(C.Structure):
 
 
 class PFConfig(C.Structure):
-    _fields_ = [("steps", C.c_int32), ("step_size", C.c_double)]
+    _fields_ = [("steps", C.c_int32), ("step_size", C.c_double), ("target_iterations", C.c_int32)]
 
 
 class ControlInfo(C.Structure):
@@ -136,6 +141,7 @@ def lib():
         L.ik_mlp_load.argtypes = [P, P, I]
         L.ik_mlp_forward.argtypes = [P, P, I64, P, I]
         L.ik_mlp_forward_noalloc.argtypes = [P, P, I64, P]
+        L.ik_write_vtu.argtypes = [P, C.c_char_p, P, C.c_int32, C.c_int32, C.c_int32]
         L.ik_mlp_loss_grad.argtypes = [P, P, P, I64, P, P, I]
         L.ik_mlp_train_step.argtypes = [P, P, P, I64, C.c_float, P, I]
         _lib = L

@@ -352,14 +352,15 @@ class LoadControl:
 
 class PathFollowing:
     """PathFollowing mirror with the ArcLength technique (pathfollowing.inl:28-77): `steps` steps of arc
-    length `step_size` from an equilibrium start at lambda = 0, plain elements."""
+    length `step_size` from an equilibrium start at lambda = 0; target_iterations > 0 selects
+    AdaptiveStepSizing::IterationBased with that target (adaptivestepsizing.hh:48-90), 0 NoOp."""
 
-    def __init__(self, nonlinear_solver, steps, step_size):
+    def __init__(self, nonlinear_solver, steps, step_size, target_iterations=0):
         if steps <= 0:
             raise IkError(capi.IK_EINVAL, "Number of load steps should be greater than zero.")
         self.nr = nonlinear_solver
         self.cfg = capi.PFConfig()
-        self.cfg.steps, self.cfg.step_size = steps, step_size
+        self.cfg.steps, self.cfg.step_size, self.cfg.target_iterations = steps, step_size, target_iterations
 
     def run(self, req):
         dev = self.nr.assembler.dev

@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libikarus_amd.so")
-SOURCES = ["pattern.cpp", "assemble.hip", "sparse.hip", "eas.hip", "mlp.hip", "capi.cpp"]
+SOURCES = ["pattern.cpp", "assemble.hip", "sparse.hip", "eas.hip", "mlp.hip", "capi.cpp", "vtk.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

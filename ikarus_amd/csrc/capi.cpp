@@ -321,24 +321,54 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
 // ---- arc-length path following -------------------------------------------------------------------
 struct ArcState {
   double Dl = 0.0, psi = 0.0, stepSize = 0.0;
+  double ub2 = 0.0;        // |ubar|^2 of the inhomogeneous Dirichlet values (0 without them)
   bool predicted = false;  // the initial prediction has run: the subsidiary derivatives are live
 };
 
-// ArcLength::operator() (pathfollowingfunctions.hh:103-115): f = sqrt(DD.DD + psi^2 Dl^2) - s; returns f,
-// 1/root (the factor of df/dDD = DD/root) and df/dDl, both zero before the initial prediction
+// ArcLength::operator() (pathfollowingfunctions.hh:103-115) with the inhomogeneous Dirichlet increment of the step,
+// idbcDelta = Dl ubar (idbcIncrement, controlroutines/common.hh:63-76): f = sqrt(DD.DD + Dl^2 |ubar|^2 + psi^2 Dl^2)
+// - s; returns f, 1/root (the factor of df/dDD = DD/root) and df/dDl = (|ubar|^2 + psi^2) Dl / root, both zero
+// before the initial prediction
 static void arcSubsidiary(ik_handle* h, const ArcState& a, double& f, double& invRoot, double& dfdDl) {
-  const double root = std::sqrt(ik::dot(h, h->arcDD.p, h->arcDD.p, h->ndof) + a.psi * a.psi * a.Dl * a.Dl);
+  const double dl2  = a.Dl * a.Dl;
+  const double root = std::sqrt(ik::dot(h, h->arcDD.p, h->arcDD.p, h->ndof) + dl2 * a.ub2 + a.psi * a.psi * dl2);
   f                 = root - a.stepSize;
   invRoot           = a.predicted ? 1.0 / root : 0.0;
-  dfdDl             = a.predicted ? a.psi * a.psi * a.Dl / root : 0.0;
+  dfdDl             = a.predicted ? (a.ub2 + a.psi * a.psi) * a.Dl / root : 0.0;
+}
+
+// force of the inhomogeneous Dirichlet values at the bound state into h->idbcF: K_raw(u) ubar with the
+// constrained entries zeroed (utils::obtainForcesDueToIDBC, functionhelper.hh:170-184); FP64 whatever the
+// tangent precision
+static void idbcForce(ik_handle* h) {
+  assembleK(h, IK_DBC_RAW, IK_FP64);
+  h->idbcF.ensure(h->ndof);
+  ik::spmv64(h, h->ubar.p, h->idbcF.p);
+  hipLaunchKernelGGL(zeroListKernel, dim3(grid(h->nfixed)), dim3(256), 0, h->stream, h->fixedList.p, h->nfixed,
+                     h->idbcF.p);
+  h->kValid = false;
 }
 
 // NewtonRaphsonWithSubsidiaryFunction::solve (newtonraphsonwithscalarsubsidiaryfunction.hh:190-307) at the
-// bound requirement; h->arcF holds Fext0
+// bound requirement
 static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_info* info) {
-  const int64_t n = h->ndof;
-  *info           = ik_nr_info{};
-  info->success   = 1;
+  const int64_t n  = h->ndof;
+  const bool idbc  = h->hasUbar && h->nfixed > 0;
+  *info            = ik_nr_info{};
+  info->success    = 1;
+  // Fext0 = -R(0, 1) (:212-221): u = 0 and lambda = 1 with the rest of the state as it is (EAS alpha kept)
+  {
+    h->tmp.ensure(n);
+    IK_HIP(hipMemcpyAsync(h->tmp.p, h->u.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    const double lam0 = h->lambda;
+    ik::launchZero(h->u.p, n, h->stream);
+    h->lambda = 1.0;
+    assembleR(h, IK_DBC_FULL);
+    IK_HIP(hipMemcpyAsync(h->arcF.p, h->R.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcF.p, n);
+    IK_HIP(hipMemcpyAsync(h->u.p, h->tmp.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    h->lambda = lam0;
+  }
   assembleR(h, IK_DBC_FULL);
   double f, invRoot, dfdDl;
   arcSubsidiary(h, a, f, invRoot, dfdDl);
@@ -348,14 +378,23 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
     info->success = 0;
     return;
   }
-  int iter            = 0;
+  int iter = 0;
   while (rNorm > s->tol && iter < s->max_iter) {
+    // the load direction Fext0 - F_idbc(u) (:251-253)
+    const double* rhsL = h->arcF.p;
+    if (idbc) {
+      idbcForce(h);
+      h->arcRhs.ensure(n);
+      IK_HIP(hipMemcpyAsync(h->arcRhs.p, h->arcF.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      ik::axpy(h, -1.0, h->idbcF.p, h->arcRhs.p, n);
+      rhsL = h->arcRhs.p;
+    }
     assembleK(h, IK_DBC_FULL, s->precision);
     ik_solver_info li{};
     linearSolve(h, h->R.p, h->arcDR.p, &s->linear, s->linear.rtol, &li);  // K x = R, dR = -x
     info->linear_iterations += li.iterations;
     hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcDR.p, n);
-    linearSolve(h, h->arcF.p, h->arcDL.p, &s->linear, s->linear.rtol, &li);  // K dL = Fext0
+    linearSolve(h, rhsL, h->arcDL.p, &s->linear, s->linear.rtol, &li);  // K dL = Fext0 - F_idbc
     info->linear_iterations += li.iterations;
     const double ddR = a.predicted ? ik::dot(h, h->arcDD.p, h->arcDR.p, n) : 0.0;
     const double ddL = a.predicted ? ik::dot(h, h->arcDD.p, h->arcDL.p, n) : 0.0;
@@ -365,11 +404,14 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
     const double dl = (-f - invRoot * ddR) / den;
     ik::axpy(h, dl, h->arcDL.p, h->arcDR.p, n);  // the correction dR + dl dL
     info->correction_norm = std::sqrt(ik::dot(h, h->arcDR.p, h->arcDR.p, n) + dl * dl);
+    if (h->neas) ik::easUpdate(h, h->arcDR.p);  // CORRECTION_UPDATED (:268), at the pre-update state
     ik::axpy(h, 1.0, h->arcDR.p, h->u.p, n);
     ik::axpy(h, 1.0, h->arcDR.p, h->arcDD.p, n);
     h->lambda += dl;
     a.Dl += dl;
-    h->kValid = false;
+    syncIdbc(h);  // syncParameterAndGlobalSolution (:276-277)
+    h->kValid        = false;
+    h->easCacheValid = false;
     assembleR(h, IK_DBC_FULL);
     arcSubsidiary(h, a, f, invRoot, dfdDl);
     rNorm               = std::sqrt(ik::dot(h, h->R.p, h->R.p, n) + f * f);
@@ -1004,29 +1046,17 @@ int ik_path_following(ik_handle* h, const ik_pf_config* c, const ik_nr_settings*
     require(c && s && info, IK_EINVAL, "null argument");
     require(c->steps > 0, IK_EINVAL, "Number of load steps should be greater than zero.");
     require(c->step_size > 0.0, IK_EINVAL, "the arc length must be positive");
-    require(h->neas == 0, IK_ENOTIMPL, "path following on the device supports plain elements only");
+    require(c->target_iterations >= 0, IK_EINVAL, "target iterations must not be negative");
     require(!h->plane2d, IK_ENOTIMPL, "path following on the device supports 3D meshes only");
-    require(!h->hasUbar, IK_ENOTIMPL, "path following on the device has no inhomogeneous Dirichlet values");
     requireBound(h);
     ensurePattern(h);
     const int64_t n = h->ndof;
     *info           = ik_control_info{};
     for (auto* b : {&h->arcDD, &h->arcDR, &h->arcDL, &h->arcF}) b->ensure(n);
-    // Fext0 = -R(0, 1): the load direction of dead loads (newtonraphsonwithscalarsubsidiaryfunction.hh:224-232)
-    {
-      h->tmp.ensure(n);
-      IK_HIP(hipMemcpyAsync(h->tmp.p, h->u.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-      const double lam0 = h->lambda;
-      ik::launchZero(h->u.p, n, h->stream);
-      h->lambda = 1.0;
-      assembleR(h, IK_DBC_FULL);
-      IK_HIP(hipMemcpyAsync(h->arcF.p, h->R.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-      hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcF.p, n);
-      IK_HIP(hipMemcpyAsync(h->u.p, h->tmp.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-      h->lambda = lam0;
-    }
+    const bool idbc = h->hasUbar && h->nfixed > 0;
     ArcState a;
     a.stepSize = c->step_size;
+    a.ub2      = idbc ? ik::dot(h, h->ubar.p, h->ubar.p, n) : 0.0;
     ik::launchZero(h->arcDD.p, n, h->stream);
     ik_nr_info ni{};
     auto record = [&](int k) {
@@ -1040,10 +1070,14 @@ int ik_path_following(ik_handle* h, const ik_pf_config* c, const ik_nr_settings*
     arcNewton(h, s, a, &ni);  // the start state (pathfollowing.inl:40-46)
     record(0);
     if (!ni.success) return;
-    // ArcLength::initialPrediction (pathfollowingfunctions.hh:131-170): K DD = -R at lambda = 1, psi = |DD|,
-    // DD and Dl = 1 scaled to the arc length, then psi = |DD| / Dl
+    // ArcLength::initialPrediction (pathfollowingfunctions.hh:131-170): K DD = -(R + F_idbc) at lambda = 1 (u as it
+    // is), psi = sqrt(|DD|^2 + |ubar|^2), DD and Dl = 1 scaled to the arc length, the inhomogeneous DOFs moved to
+    // Dl ubar, then psi = sqrt(|Dl ubar|^2 + |DD|^2) / Dl
     h->lambda = 1.0;
+    h->kValid = false;
+    if (idbc) idbcForce(h);
     assembleR(h, IK_DBC_FULL);
+    if (idbc) ik::axpy(h, 1.0, h->idbcF.p, h->R.p, n);
     assembleK(h, IK_DBC_FULL, s->precision);
     {
       ik_solver_info li{};
@@ -1052,24 +1086,32 @@ int ik_path_following(ik_handle* h, const ik_pf_config* c, const ik_nr_settings*
     }
     hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcDD.p, n);
     const double dd2 = ik::dot(h, h->arcDD.p, h->arcDD.p, n);
-    a.psi            = std::sqrt(dd2);
-    const double sc  = std::sqrt(dd2 + a.psi * a.psi);
+    a.psi            = std::sqrt(dd2 + a.ub2);
+    const double sc  = std::sqrt(dd2 + a.ub2 + a.psi * a.psi);
     require(sc > 0.0, IK_EINVAL, "path following: the load direction is zero");
     hipLaunchKernelGGL(scaleKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->arcDD.p, a.stepSize / sc, n);
     a.Dl = a.stepSize / sc;
     ik::axpy(h, 1.0, h->arcDD.p, h->u.p, n);
-    h->lambda   = a.Dl;
-    h->kValid   = false;
-    a.psi       = std::sqrt(ik::dot(h, h->arcDD.p, h->arcDD.p, n) / (a.Dl * a.Dl));
+    h->lambda = a.Dl;
+    syncIdbc(h);
+    h->kValid        = false;
+    h->easCacheValid = false;
+    a.psi       = std::sqrt((a.Dl * a.Dl * a.ub2 + ik::dot(h, h->arcDD.p, h->arcDD.p, n)) / (a.Dl * a.Dl));
     a.predicted = true;
     arcNewton(h, s, a, &ni);
     record(1);
     if (!ni.success) return;
     for (int ls = 1; ls < c->steps; ++ls) {
+      // AdaptiveStepSizing::IterationBased (adaptivestepsizing.hh:48-90): s <- sqrt(target / its) s when the last
+      // solve took more than the target iterations
+      if (c->target_iterations > 0 && ni.iterations > c->target_iterations)
+        a.stepSize = std::sqrt((double)c->target_iterations / ni.iterations) * a.stepSize;
       // ArcLength::intermediatePrediction (pathfollowingfunctions.hh:172-185): the last step's increments
       ik::axpy(h, 1.0, h->arcDD.p, h->u.p, n);
       h->lambda += a.Dl;
-      h->kValid = false;
+      syncIdbc(h);
+      h->kValid        = false;
+      h->easCacheValid = false;
       arcNewton(h, s, a, &ni);
       record(ls + 1);
       if (!ni.success) return;

@@ -121,7 +121,7 @@ struct ik_handle {
   bool hasUbar = false;
   ik::DevBuf<double> ubar, idbcF;
   // arc length: the step's accumulated increment, the two correction directions and the load direction
-  ik::DevBuf<double> arcDD, arcDR, arcDL, arcF;
+  ik::DevBuf<double> arcDD, arcDR, arcDL, arcF, arcRhs;  // arcRhs: Fext0 - F_idbc
   // loads
   int64_t nfaces = 0;
   ik::DevBuf<uint8_t> faceBits;  // per element: bit 2*axis + side set for each loaded face (empty: none)

@@ -154,6 +154,24 @@ int ik_update_state(ik_handle* h, const double* du, int where);
 int ik_get_alpha(ik_handle* h, double* alpha, int where);
 int ik_set_alpha(ik_handle* h, const double* alpha, int where);
 
+/* ---- VTK output (Ikarus::Vtk::Writer, io/vtkwriter.hh:37-146) ------------------------------------ */
+/* DataTag::asPointData / asCellData (io/vtkwriter.hh:80-100) */
+enum { IK_VTK_POINT_DATA = 0, IK_VTK_CELL_DATA = 1 };
+typedef struct {
+  const char* name;      /* array name (no '"', '<' or '&')                                              */
+  const double* values;  /* host: point data [nodes x components] (a DOF vector has dim components),
+                            cell data [elements x components]                                            */
+  int32_t components;
+  int32_t cell;          /* IK_VTK_POINT_DATA | IK_VTK_CELL_DATA                                          */
+} ik_vtk_field;
+/* Writes an ASCII XML UnstructuredGrid (.vtu) of the mesh in reference coordinates (VTK_HEXAHEDRON, or VTK_QUAD
+ * for plane strain), the given fields (Writer::addInterpolation, :125-140; vectors of dim 2 are padded to 3
+ * components) and, for every bit k of `results` (1 << IK_PK2 | 1 << IK_KIRCHHOFF | 1 << IK_CAUCHY), the stress of
+ * ResultFunction (io/resultfunction.hh:55-110) at the bound requirement: at each element centre as cell data, or
+ * at each element corner averaged over the elements sharing the node as point data (result_tag). */
+int ik_write_vtu(ik_handle* h, const char* path, const ik_vtk_field* fields, int32_t num_fields, int32_t results,
+                 int32_t result_tag);
+
 /* ---- linear solve (LinearSolverTemplate::analyzePattern/factorize/solve, linearsolver.hh:50-235) -- */
 typedef struct {
   int32_t kind;        /* IK_PCG_JACOBI_FP64 | IK_PCG_BJACOBI_FP32_IR                       */
@@ -226,15 +244,21 @@ int ik_load_control(ik_handle* h, const ik_lc_config* c, const ik_nr_settings* s
 typedef struct {
   int32_t steps;     /* PathFollowing steps (the start-state solve comes on top)        */
   double step_size;  /* arc length s of ArcLength                                       */
+  int32_t target_iterations;  /* AdaptiveStepSizing::IterationBased (adaptivestepsizing.hh:48-90): from the
+                                 second step on, s <- sqrt(target / its) s when the last solve took
+                                 its > target iterations; 0: NoOp                                       */
 } ik_pf_config;
 
 /* PathFollowing::run with ArcLength (pathfollowing.inl:28-77; pathfollowingfunctions.hh:88-190), each
  * solve NewtonRaphsonWithSubsidiaryFunction::solve (newtonraphsonwithscalarsubsidiaryfunction.hh:190-307):
- * per iteration K dR = -R and K dL = Fext0 = -R(0, 1) with the FULL tangent and s->linear, then the load
- * increment from the linearized arc-length equation. The requirement must start at lambda = 0 in
- * equilibrium (the start solve has no load direction yet, IK_EINVAL otherwise). Plain elements only (an
- * EAS element's internal forces at u = 0 do not vanish, IK_ENOTIMPL), no inhomogeneous Dirichlet values.
- * iterations/residuals/lambdas: optional arrays of steps+1 entries (per solve). s->forcing is not used. */
+ * Fext0 = -R(0, 1) at the start of each solve (u = 0, the rest of the state kept: an EAS element's alpha is not
+ * reset, as in the reference), then per iteration K dR = -R and K dL = Fext0 - F_idbc with the FULL tangent and
+ * s->linear, the load increment from the linearized arc-length equation, the EAS state update on the correction
+ * and the inhomogeneous Dirichlet DOFs set to lambda ubar. With inhomogeneous Dirichlet values the arc length
+ * includes their increment Dl ubar (idbcIncrement, controlroutines/common.hh:63-76) and F_idbc = K_raw(u) ubar
+ * (constrained entries zeroed). The requirement must start at lambda = 0 in equilibrium (the start solve has no
+ * load direction yet, IK_EINVAL otherwise). iterations/residuals/lambdas: optional arrays of steps+1 entries (per
+ * solve). s->forcing is not used. Not for plane-strain handles (IK_ENOTIMPL). */
 int ik_path_following(ik_handle* h, const ik_pf_config* c, const ik_nr_settings* s, ik_control_info* info,
                       int32_t* iterations, double* residuals, double* lambdas);
 

@@ -284,7 +284,9 @@ class LoadControl {
 // loadFactors
 class PathFollowing {
  public:
-  PathFollowing(NewtonRaphson& nr, int loadSteps, double stepSize) : nr_(nr), steps_(loadSteps), s_(stepSize) {
+  // targetIterations > 0: AdaptiveStepSizing::IterationBased with that target (adaptivestepsizing.hh:48-90)
+  PathFollowing(NewtonRaphson& nr, int loadSteps, double stepSize, int targetIterations = 0)
+      : nr_(nr), steps_(loadSteps), s_(stepSize), target_(targetIterations) {
     if (loadSteps <= 0) throw Error(IK_EINVAL, "Number of load steps should be greater than zero.");
   }
 
@@ -292,7 +294,7 @@ class PathFollowing {
     ik_handle* h = nr_.assembler().handle();
     check(h, ik_set_requirement(h, req.globalSolution().data(), req.parameter(), IK_HOST));
     const ik_nr_settings st = nr_.settings();
-    ik_pf_config c{steps_, s_};
+    ik_pf_config c{steps_, s_, target_};
     ik_control_info info{};
     std::vector<int32_t> it(steps_ + 1, 0);
     std::vector<double> res(steps_ + 1, 0.0);
@@ -314,6 +316,7 @@ class PathFollowing {
   NewtonRaphson& nr_;
   int steps_;
   double s_;
+  int target_;
 };
 
 }  // namespace Ikarus::AMD

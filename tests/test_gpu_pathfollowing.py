@@ -82,16 +82,87 @@ def test_arc_length_mixed_precision():
     assert np.linalg.norm(ug - r["u"]) / np.linalg.norm(r["u"]) < 1e-6  # the same shift along the path
 
 
-def test_arc_length_rejects_eas():
-    p = pb.hex_block((2, 2, 2), eas=21, seed=3)
-    print("case: arc length EAS", flush=True)
+def _arc_step(o, p, frac):
+    z = np.zeros(p.ndof)
+    d1 = np.linalg.solve(o.matrix_dense(z, 1.0, po.FULL), o.vector(z, 1.0, po.FULL))
+    return frac * np.sqrt(2.0) * np.linalg.norm(d1)
+
+
+def _device_path(p, steps, s, ubar=None, target=0):
     dev = ia.Device(p)
     dev.set_requirement(np.zeros(p.ndof), 0.0)
+    if ubar is not None:
+        dev.set_dirichlet_values(ubar)
     cfg = capi.PFConfig()
-    cfg.steps, cfg.step_size = 2, 0.1
-    with pytest.raises(RuntimeError):
-        dev.path_following(cfg, ia.nr_settings())
+    cfg.steps, cfg.step_size, cfg.target_iterations = steps, s, target
+    info, iters, rn, lams = dev.path_following(cfg, ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-13)))
+    ug, lam = dev.get_requirement()
     dev.close()
+    return info, iters, rn, lams, ug, lam
+
+
+def _compare(r, info, iters, rn, lams, ug, lam, steps):
+    assert r["success"] and info.success and info.solves == steps + 1
+    assert list(iters) == list(r["iterations"])
+    assert np.all(np.abs(lams - np.array(r["lambdas"])) < 1e-10)
+    assert abs(lam - r["lam"]) < 1e-10
+    assert np.linalg.norm(ug - r["u"]) / np.linalg.norm(r["u"]) < 1e-8
+    assert np.all(rn[1:] <= 1e-8)
+
+
+def test_arc_length_eas_matches_oracle():
+    """EAS-21 elements: Fext0 = -R(0, 1) at the start of every solve keeps the current alpha (the reference does
+    not reset it), and alpha is updated on every correction at the pre-update state."""
+    p = pb.hex_block((3, 2, 2), eas=21, jitter=0.05, seed=44, mag_range=(20.0, 30.0))
+    print("case: arc length EAS", flush=True)
+    o = _oracle_for(p)
+    res = lambda u, lam: o.vector(u, lam, po.FULL)
+    jac = lambda u, lam: o.matrix_dense(u, lam, po.FULL)
+    s = _arc_step(o, p, 0.3)
+    r = pf.arc_length(res, jac, np.zeros(p.ndof), steps=3, step_size=s,
+                      update_state=lambda u, lam, d: o.update_state(u, lam, d))
+    _compare(r, *_device_path(p, 3, s), 3)
+
+
+def test_arc_length_idbc_matches_oracle():
+    """Inhomogeneous Dirichlet values on the loaded face (u_z = -0.02 lambda on top) beside the traction: the arc
+    length includes their increment, the load direction subtracts K_raw(u) ubar, the DOFs follow lambda."""
+    p = pb.hex_block((3, 3, 2), jitter=0.05, seed=45, mag_range=(10.0, 20.0))
+    print("case: arc length IDBC", flush=True)
+    top = np.nonzero(p.X[:, 2] > 1 - 1e-9)[0]
+    p.fixed = p.fixed.copy()
+    p.fixed[3 * top + 2] = 1
+    ubar = np.zeros(p.ndof)
+    ubar[3 * top + 2] = -0.02
+    o = _oracle_for(p)
+    fixed = p.fixed.astype(bool)
+    res = lambda u, lam: o.vector(u, lam, po.FULL)
+    jac = lambda u, lam: o.matrix_dense(u, lam, po.FULL)
+
+    def idbc_force(u, lam):
+        f = o.matrix_dense(u, lam, po.RAW) @ ubar
+        f[fixed] = 0.0
+        return f
+    s = _arc_step(o, p, 0.25)
+    r = pf.arc_length(res, jac, np.zeros(p.ndof), steps=4, step_size=s, ubar=ubar, idbc_force=idbc_force)
+    out = _device_path(p, 4, s, ubar=ubar)
+    _compare(r, *out, 4)
+    ug, lam = out[4], out[5]
+    assert np.allclose(ug[3 * top + 2], -0.02 * lam, rtol=1e-14)
+
+
+def test_arc_length_adaptive_step_sizing_matches_oracle():
+    """IterationBased step sizing with target 1: every step after a two-iteration solve is shortened by sqrt(1/2)."""
+    p = pb.hex_block((3, 3, 2), jitter=0.05, seed=46, mag_range=(20.0, 30.0))
+    print("case: arc length adaptive", flush=True)
+    o = _oracle_for(p)
+    res = lambda u, lam: o.vector(u, lam, po.FULL)
+    jac = lambda u, lam: o.matrix_dense(u, lam, po.FULL)
+    s = _arc_step(o, p, 0.3)
+    r = pf.arc_length(res, jac, np.zeros(p.ndof), steps=4, step_size=s, target_iterations=1)
+    plain = pf.arc_length(res, jac, np.zeros(p.ndof), steps=4, step_size=s)
+    assert r["lam"] < plain["lam"]
+    _compare(r, *_device_path(p, 4, s, target=1), 4)
 
 
 def test_path_following_mirror():

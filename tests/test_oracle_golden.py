@@ -441,3 +441,67 @@ def test_colored_threaded_assembly_equals_serial(dim):
     K4, R4 = o.matrix(u, 1.0), o.vector(u, 1.0)
     assert np.allclose(K4, K1, rtol=1e-13, atol=1e-13 * np.abs(K1).max())
     assert np.allclose(R4, R1, rtol=1e-13, atol=1e-13 * np.abs(R1).max())
+
+
+def _two_dof():
+    def residual(D, lam):
+        u, w = D
+        return np.array([16.0 * u - 3 * w - w ** 3 - (2.0 / 3.0) * lam,
+                         -3 * u + 4 * w - 3 * u * w * w - (9.0 / 4.0) * lam])
+
+    def jacobian(D, lam):
+        u, w = D
+        return np.array([[16.0, -3.0 * w * w - 3], [-3 * w * w - 3, -6 * u * w + 4]])
+    return residual, jacobian
+
+
+# tests/src/testpathfollowing.cpp:73-116,386-395: u (DOF 0) is an inhomogeneous Dirichlet DOF, u = 0.1 lambda,
+# w solved by arc length, step 0.1, 5 steps. Here in the FULL form: row and column 0 of K are the identity, R[0] = 0,
+# and the IDBC force is K_raw(D) ubar with the constrained entry zeroed.
+def test_arc_length_two_dof_idbc_golden():
+    import pathfollowing as pf
+    res, jac = _two_dof()
+    ubar = np.array([0.1, 0.0])
+
+    def residual(D, lam):
+        r = res(D, lam)
+        r[0] = 0.0
+        return r
+
+    def jacobian(D, lam):
+        K = jac(D, lam)
+        K[0, :] = 0.0
+        K[:, 0] = 0.0
+        K[0, 0] = 1.0
+        return K
+
+    def idbc_force(D, lam):
+        f = jac(D, lam) @ ubar
+        f[0] = 0.0
+        return f
+
+    r = pf.arc_length(residual, jacobian, np.zeros(2), steps=5, step_size=0.1, ubar=ubar, idbc_force=idbc_force)
+    assert r["success"] and r["iterations"] == [0, 1, 2, 2, 2, 2]
+    assert np.allclose(r["u"], [0.05440020617136589, 0.351852377380214], rtol=1e-12, atol=0)
+    assert abs(r["lam"] - 0.5440020617136589) < 1e-12
+    assert abs(r["u"][0] - 0.1 * r["lam"]) < 1e-15
+
+
+def test_arc_length_iteration_based_step_sizing():
+    """AdaptiveStepSizing::IterationBased (adaptivestepsizing.hh:48-90): from the second step on, a step whose
+    previous solve took its > target iterations has its arc length scaled by sqrt(target / its); with the target at
+    or above the iterations nothing changes."""
+    import pathfollowing as pf
+    res, jac = _two_dof()
+    plain = pf.arc_length(res, jac, np.zeros(2), steps=5, step_size=0.1)
+    same = pf.arc_length(res, jac, np.zeros(2), steps=5, step_size=0.1, target_iterations=2)
+    assert same["iterations"] == plain["iterations"] and same["lam"] == plain["lam"]
+    r = pf.arc_length(res, jac, np.zeros(2), steps=5, step_size=0.1, target_iterations=1)
+    assert r["success"]
+    its, sizes = r["iterations"], r["step_sizes"]
+    expect = [0.1, 0.1]
+    for k in range(2, 6):
+        prev = its[k - 1]
+        expect.append(expect[-1] * (np.sqrt(1.0 / prev) if prev > 1 else 1.0))
+    assert np.allclose(sizes, expect, rtol=1e-15)
+    assert r["lam"] < plain["lam"]  # shorter arcs reach a smaller load
