@@ -39,11 +39,12 @@ __device__ inline uint16_t toBf16(float f) {
 }
 
 // EPI_SILU_ONLY: C = silu(A B + bias), the pre-activation is not kept (inference)
-enum { EPI_STORE = 0, EPI_ATOMIC = 1, EPI_BIAS_SILU = 2, EPI_BIAS = 3, EPI_SILU_ONLY = 4 };
+// EPI_PARTIAL: slice z of a K split stores its partial product at C + z * M * ldc (summed later in z order)
+enum { EPI_STORE = 0, EPI_PARTIAL = 1, EPI_BIAS_SILU = 2, EPI_BIAS = 3, EPI_SILU_ONLY = 4 };
 
 // C[i][j] = sum_k A(i,k) B(k,j) with A(i,k) = A[i*sai + k*sak], B(k,j) = B[k*sbk + j*sbj].
 // 256 threads, 64x64 block tile, 4 waves of 32x32 (2x2 MFMA tiles), K in steps of 16.
-// gridDim.z splits K into chunks of kChunk (EPI_ATOMIC accumulates into a zeroed C).
+// gridDim.z splits K into chunks of kChunk (EPI_PARTIAL: one partial product per chunk).
 template <int EPI>
 __global__ __launch_bounds__(256) void gemmBf16(int M, int N, int K, const float* __restrict__ A, int64_t sai,
                                                 int64_t sak, const float* __restrict__ B, int64_t sbk, int64_t sbj,
@@ -99,8 +100,8 @@ __global__ __launch_bounds__(256) void gemmBf16(int M, int N, int K, const float
         const int j = j0 + wn * 32 + nj * 16 + (lane & 15);
         if (i >= M || j >= N) continue;
         float v = acc[mi][nj][r];
-        if constexpr (EPI == EPI_ATOMIC) {
-          unsafeAtomicAdd(C + (int64_t)i * ldc + j, v);
+        if constexpr (EPI == EPI_PARTIAL) {
+          C[(int64_t)blockIdx.z * M * ldc + (int64_t)i * ldc + j] = v;
         } else {
           if constexpr (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS || EPI == EPI_SILU_ONLY) v += bias[j];
           if constexpr (EPI == EPI_SILU_ONLY) {
@@ -143,13 +144,23 @@ __global__ void mseGrad(const float* __restrict__ Y, const float* __restrict__ T
 }
 
 // db[j] += sum over rows of dZ (one block per 256-row slab, one thread per column)
-__global__ void biasGrad(const float* __restrict__ dZ, int64_t n, int cols, float* __restrict__ db) {
+// column sums of 256-row slices of dZ: part[slice * cols + j]
+__global__ void biasGrad(const float* __restrict__ dZ, int64_t n, int cols, float* __restrict__ part) {
   const int j = threadIdx.x;
   if (j >= cols) return;
   const int64_t r0 = (int64_t)blockIdx.x * 256, r1 = min<int64_t>(n, r0 + 256);
   float s = 0.f;
   for (int64_t r = r0; r < r1; ++r) s += dZ[r * cols + j];
-  unsafeAtomicAdd(db + j, s);
+  part[(int64_t)blockIdx.x * cols + j] = s;
+}
+
+// out[i] = sum over z = 0 .. nz-1 of part[z * len + i], in that order: the gradients do not depend on scheduling
+__global__ void foldSlices(const float* __restrict__ part, int nz, int64_t len, float* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < len; i += (int64_t)gridDim.x * 256) {
+    float s = 0.f;
+    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * len + i];
+    out[i] = s;
+  }
 }
 
 __global__ void adamKernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -210,6 +221,7 @@ struct Mlp {
   int64_t maxBatch = 0;
   DevBuf<float> P, G, Mo, Vo;          // params (fp32 master), grads, Adam moments
   DevBuf<float> Z[3], Aa[3], Y, dY, dA, dZ, Xin, Tin;
+  DevBuf<float> Gpart;  // per-slice partial gradients, folded in a fixed order
   DevBuf<double> partial;
   double* hred = nullptr;
   int64_t step = 0;
@@ -230,14 +242,16 @@ struct Mlp {
     Tin.ensure(n * kOut);
   }
 
+  // returns the number of K slices (gridDim.z)
   template <int EPI>
-  void gemm(int M, int N, int K, const float* A, int64_t sai, int64_t sak, const float* B, int64_t sbk, int64_t sbj,
-            float* C, int64_t ldc, const float* bias, float* act, int splitK = 1) {
+  int gemm(int M, int N, int K, const float* A, int64_t sai, int64_t sak, const float* B, int64_t sbk, int64_t sbj,
+           float* C, int64_t ldc, const float* bias, float* act, int splitK = 1) {
     const int kChunk = splitK > 1 ? ((K + splitK - 1) / splitK + 15) / 16 * 16 : K;
     const int nz     = splitK > 1 ? (K + kChunk - 1) / kChunk : 1;
     dim3 grid((N + 63) / 64, (M + 63) / 64, nz);
     hipLaunchKernelGGL((gemmBf16<EPI>), grid, dim3(256), 0, s, M, N, K, A, sai, sak, B, sbk, sbj, C, ldc, bias, act,
                        kChunk);
+    return nz;
   }
 
   // forward on x (n x 14) -> Y (n x 3), keeping Z/A for the backward pass
@@ -266,20 +280,29 @@ struct Mlp {
     partial.ensure(1024);
     hipLaunchKernelGGL(mseGrad, dim3(nb), dim3(256), 0, s, Y.p, t, n * kOut, (float)(1.0 / (double)(n * kOut)), dY.p,
                        partial.p);
-    hipLaunchKernelGGL(fillF, dim3(g1(kParams)), dim3(256), 0, s, G.p, kParams, 0.f);
     const int split = (int)std::max<int64_t>(1, std::min<int64_t>(n / 256, 256));
     const unsigned nbias = (unsigned)((n + 255) / 256);
+    Gpart.ensure(std::max<int64_t>((int64_t)split * kHid * kHid, (int64_t)nbias * kHid));
+    // dW (M x N) = dZ^T A over the batch: one partial product per batch slice, folded in slice order
+    auto weightGrad = [&](int M, int N, const float* dz, const float* ain, float* dst) {
+      const int nz = gemm<EPI_PARTIAL>(M, N, (int)n, dz, 1, M, ain, N, 1, Gpart.p, N, nullptr, nullptr, split);
+      hipLaunchKernelGGL(foldSlices, dim3(g1((int64_t)M * N)), dim3(256), 0, s, Gpart.p, nz, (int64_t)M * N, dst);
+    };
+    auto biasGradient = [&](const float* dz, int cols, float* dst) {
+      hipLaunchKernelGGL(biasGrad, dim3(nbias), dim3(kHid), 0, s, dz, n, cols, Gpart.p);
+      hipLaunchKernelGGL(foldSlices, dim3(g1(cols)), dim3(256), 0, s, Gpart.p, (int)nbias, (int64_t)cols, dst);
+    };
     // layer 4: dW4 = dY^T A3, db4, dA3 = dY W4
-    gemm<EPI_ATOMIC>(kOut, kHid, (int)n, dY.p, 1, kOut, Aa[2].p, kHid, 1, G.p + oW4, kHid, nullptr, nullptr, split);
-    hipLaunchKernelGGL(biasGrad, dim3(nbias), dim3(kHid), 0, s, dY.p, n, kOut, G.p + oB4);
+    weightGrad(kOut, kHid, dY.p, Aa[2].p, G.p + oW4);
+    biasGradient(dY.p, kOut, G.p + oB4);
     gemm<EPI_STORE>((int)n, kHid, kOut, dY.p, kOut, 1, P.p + oW4, kHid, 1, dA.p, kHid, nullptr, nullptr);
     for (int l = 2; l >= 0; --l) {
       hipLaunchKernelGGL(siluBackward, dim3(g1(n * kHid)), dim3(256), 0, s, dA.p, Z[l].p, dZ.p, n * kHid);
       const float* ain = l == 0 ? x : Aa[l - 1].p;
       const int kin    = l == 0 ? kIn : kHid;
       const int64_t oW = l == 0 ? oW1 : (l == 1 ? oW2 : oW3), oB = l == 0 ? oB1 : (l == 1 ? oB2 : oB3);
-      gemm<EPI_ATOMIC>(kHid, kin, (int)n, dZ.p, 1, kHid, ain, kin, 1, G.p + oW, kin, nullptr, nullptr, split);
-      hipLaunchKernelGGL(biasGrad, dim3(nbias), dim3(kHid), 0, s, dZ.p, n, kHid, G.p + oB);
+      weightGrad(kHid, kin, dZ.p, ain, G.p + oW);
+      biasGradient(dZ.p, kHid, G.p + oB);
       if (l > 0) gemm<EPI_STORE>((int)n, kHid, kHid, dZ.p, kHid, 1, P.p + oW, kin, 1, dA.p, kHid, nullptr, nullptr);
     }
     IK_HIP(hipMemcpyAsync(hred, partial.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));

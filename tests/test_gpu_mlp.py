@@ -130,3 +130,19 @@ def test_trained_predictor_reduces_newton_iterations():
         tot[pred] = info.total_iterations
     print("newton iterations constant/secant/mlp:", tot, flush=True)
     assert tot[capi.PRED_MLP] <= tot[capi.PRED_LINEAR] < tot[capi.PRED_CONSTANT]
+
+
+def test_gradients_are_reproducible():
+    """The batch-split weight and bias gradients are summed in a fixed slice order: two evaluations on the same
+    batch give bit-identical gradients (and so training is reproducible)."""
+    print("case: mlp gradient reproducibility", flush=True)
+    m = learned.MLP(seed=7)
+    x, t = _data(20000, 4)
+    l1, g1 = m.loss_grad(x, t)
+    l2, g2 = m.loss_grad(x, t)
+    assert l1 == l2 and np.array_equal(g1, g2)
+    m2 = learned.MLP(seed=7)
+    for _ in range(3):
+        m.train_step(x, t, 1e-3)
+        m2.train_step(x, t, 1e-3)
+    assert np.array_equal(m.params(), m2.params())

@@ -93,3 +93,18 @@ def test_capi_rejects_bad_input_without_gpu_compute():
     assert rc == capi.IK_EINVAL and b"out of range" in msg
     rc, msg = create(faces=2)
     assert rc == capi.IK_EINVAL and b"2D mesh" in msg
+
+
+def test_pattern_build_under_sanitizers():
+    """The threaded host pattern build, run on the CPU under ASan + UBSan and under TSan (tests/cpp/
+    pattern_sanitize.cpp: HIP runtime calls replaced by host test doubles; 1, 3 and 8 threads byte-identical)."""
+    import subprocess
+    import __graft_entry__ as ge
+    out = os.path.join(ROOT, "tests", "cpp", "_build")
+    exes = [os.path.join(out, n) for n in ("pattern_asan", "pattern_tsan")]
+    if not all(os.path.exists(e) for e in exes):
+        exes = ge.build_sanitizer_harness()
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1", TSAN_OPTIONS="halt_on_error=1")
+    for exe in exes:
+        r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=300)
+        assert r.returncode == 0 and r.stdout.strip().endswith("OK"), (exe, r.stdout[-2000:], r.stderr[-4000:])
