@@ -189,7 +189,10 @@ typedef struct {
   int32_t converged;
 } ik_solver_info;
 
-/* Solve K x = rhs with the last assembled (FULL) matrix. */
+/* Solve K x = rhs with the last assembled (FULL) matrix. With IK_PCG_BJACOBI_FP32_IR the FP64 refinement
+ * residual b - K x uses the FP32 matrix, so the solve converges to the solution of the FP32-rounded system and
+ * rel_residual / converged refer to it; against the FP64 tangent the residual has a floor near
+ * eps32 ||K|| ||x||. (Newton is not affected: its residual is assembled in FP64.) */
 int ik_solve(ik_handle* h, const double* rhs, double* x, int where, const ik_solver_opts* opts,
              ik_solver_info* info);
 
