@@ -349,6 +349,8 @@ def main():
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=40, help="load steps of the Newton comparison")
     ap.add_argument("--eval-seeds", type=int, default=len(EVAL_SEEDS), help="evaluation seeds of the c5 comparison")
+    ap.add_argument("--profile-newton", action="store_true",
+                    help="only one mixed + learned load path (the command profiled for the MLP and PCG kernels)")
     ap.add_argument("--tune-eta", action="store_true",
                     help="only choose eta_max per mixed predictor on the training seeds (prints the table)")
     ap.add_argument("--c2-steps", type=int, default=5, help="load steps of config c2's FP64 path on the device")
@@ -375,6 +377,13 @@ def main():
     if args.tune_eta:
         mlp = np.fromfile(args.mlp, dtype=np.float32)
         print(json.dumps({"tune_eta": tune_eta(args.n, args.newton_steps, mlp)}), flush=True)
+        return
+    if args.profile_newton:
+        # one mixed + learned load path on the first evaluation seed (the command profiled for the learned and PCG
+        # kernels)
+        mlp = np.fromfile(args.mlp, dtype=np.float32)
+        r, _ = newton_path(pb.hex_block((args.n,) * 3, seed=EVAL_SEEDS[0]), "mixed_mlp", args.newton_steps, mlp)
+        print(json.dumps({"profile_newton": r}), flush=True)
         return
 
     n = args.n

@@ -444,6 +444,22 @@ int ik_create(ik_handle** out, const ik_problem_desc* d0) {
     require(d->eas_modes == 0 || d->eas_modes == 21, IK_ENOTIMPL, "eas_modes must be 0 or 21");
     require(d->material == IK_NEOHOOKE || d->material == IK_SVK, IK_EINVAL, "unknown material");
     require(d->num_nodes * 3 < (int64_t)INT32_MAX, IK_EINVAL, "too many DOFs for int32 indices");
+    // connectivity, before any device work: indices in range, eight distinct nodes per element, and every node in
+    // some element (a grid vertex belongs to an element; an unreferenced node would have an empty block row)
+    {
+      std::vector<uint8_t> seen((size_t)d->num_nodes, 0);
+      for (int64_t e = 0; e < d->num_elements; ++e) {
+        const int32_t* c = d->conn + e * 8;
+        for (int a = 0; a < 8; ++a) {
+          require(c[a] >= 0 && c[a] < d->num_nodes, IK_EINVAL, "connectivity index out of range");
+          for (int b = 0; b < a; ++b)
+            require(c[a] != c[b], IK_EINVAL, "element " + std::to_string(e) + " repeats a node");
+          seen[c[a]] = 1;
+        }
+      }
+      for (int64_t v = 0; v < d->num_nodes; ++v)
+        require(seen[v], IK_EINVAL, "node " + std::to_string(v) + " belongs to no element");
+    }
     IK_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     IK_HIP(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
     IK_HIP(hipEventCreateWithFlags(&h->joinEvent, hipEventDisableTiming));

@@ -174,3 +174,25 @@ def test_plane_strain_rejects_3d_only_features():
         dev.path_following(cfg, ia.nr_settings())
     assert ei.value.code == capi.IK_ENOTIMPL
     dev.close()
+
+
+# tests/src/testnonlinearelasticity.hh:248-338 with the material of testnonlinearelasticitysvk.cpp:19,34: a single
+# distorted Q1 element (corners from tests/src/testcommon.hh:208-213), SVK with E = 1000 and nu = 0 (plane stress and
+# plane strain coincide for lambda = 0), 2x2 Gauss, at the displacement d: the magnitudes of K's eigenvalues
+def test_single_element_eigenvalue_golden_on_device():
+    print("case: single Q1 element golden", flush=True)
+    X = np.array([[0, 0], [1, 0], [0, 1], [1, 1]], float) + np.array(
+        [[-0.2, -0.05], [-0.15, 0.05], [0.15, 0.15], [-0.05, -0.1]])
+    lam, mu = pb.lame(1000.0, 0.0)
+    p = pb.Problem(X, np.array([[0, 1, 2, 3]], np.int32), capi.SVK, lam, mu, name="q1")
+    p.fixed = np.zeros(8, np.uint8)
+    dev = ia.Device(p)
+    d = np.array([2, 4, 3.25, -1.2, 0.003, 6, 3, 2.864])
+    dev.set_requirement(d, 0.0)
+    K, _ = dev.element(0)
+    dev.close()
+    expected = [1845.6296388251504753, 14192.4707553121224317, 19964.32719133414782, 29973.7943273325380486,
+                46641.183728849332812, 95447.6156712376251918]
+    ev = np.sort(np.abs(np.linalg.eigvalsh(0.5 * (K + K.T))))
+    assert np.all(ev[:2] < 1e-8)
+    np.testing.assert_allclose(ev[2:], expected, rtol=1e-10)

@@ -108,3 +108,27 @@ def test_pattern_build_under_sanitizers():
     for exe in exes:
         r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=300)
         assert r.returncode == 0 and r.stdout.strip().endswith("OK"), (exe, r.stdout[-2000:], r.stderr[-4000:])
+
+
+def test_capi_rejects_degenerate_connectivity_without_gpu_compute():
+    """A node in no element (no Dune grid has one; its block row would be empty) and an element that repeats a node
+    are refused before any device work."""
+    lib = capi.lib()
+    X = np.zeros((9, 3))
+    X[:8] = [[a & 1, (a >> 1) & 1, (a >> 2) & 1] for a in range(8)]
+
+    def create(conn):
+        d = capi.ProblemDesc()
+        d.dim = 3
+        d.num_nodes, d.num_elements = len(X), 1
+        d.X, d.conn = capi.ptr(X), capi.ptr(conn)
+        h = C.c_void_p()
+        rc = lib.ik_create(C.byref(h), C.byref(d))
+        msg = lib.ik_last_error(h)
+        lib.ik_destroy(h)
+        return rc, msg
+
+    rc, msg = create(np.arange(8, dtype=np.int32))
+    assert rc == capi.IK_EINVAL and b"node 8 belongs to no element" in msg
+    rc, msg = create(np.array([0, 1, 2, 3, 4, 5, 6, 6], np.int32))
+    assert rc == capi.IK_EINVAL and b"repeats a node" in msg
