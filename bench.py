@@ -394,7 +394,7 @@ def main():
     dev.set_requirement(u, 1.0)
     nbr, nnzb = np_i64(), np_i64()
     t0 = time.perf_counter()  # the first matrix query builds the pattern (createOccupationPattern, once)
-    dev.check(capi.lib().ik_matrix_bsr(dev.h, capi.ptr(nbr), capi.ptr(nnzb), None, None, None, None))
+    dev.check(capi.lib().ik_matrix_bsr(dev.h, capi.ptr(nbr), capi.ptr(nnzb), None, None, None, None, None))
     pattern_s = time.perf_counter() - t0
     nnzb = int(nnzb[0]) if int(nnzb[0]) > 0 else None
     log(f"[bench] pattern build (host, once per mesh): {pattern_s:.3f} s")

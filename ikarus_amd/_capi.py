@@ -112,7 +112,7 @@ def lib():
         L.ik_matrix.argtypes = [P, I, I]
         L.ik_assemble.argtypes = [P, I, I, P, I]
         L.ik_matrix_csc.argtypes = [P, I, P, P, P, P, P]
-        L.ik_matrix_bsr.argtypes = [P, P, P, P, P, P, P]
+        L.ik_matrix_bsr.argtypes = [P, P, P, P, P, P, P, P]
         L.ik_element.argtypes = [P, I64, P, P]
         L.ik_update_state.argtypes = [P, P, I]
         L.ik_get_alpha.argtypes = [P, P, I]

@@ -985,6 +985,19 @@ void launchZeroF(float* p, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(fillKernel<float>, dim3(gridFor(n, 256)), dim3(256), 0, s, p, n, 0.0f);
 }
 
+// K storage of the sliced-ELL layout: the padding slots must hold zeros (the SpMV reads them); every
+// assembly rewrites all real blocks, so zeroing once per allocation suffices
+static void ensureK(ik::DevBuf<double>& K, int64_t n, hipStream_t s) {
+  if (K.p && (int64_t)K.n >= n) return;
+  K.ensure((size_t)n);
+  launchZero(K.p, n, s);
+}
+static void ensureK(ik::DevBuf<float>& K, int64_t n, hipStream_t s) {
+  if (K.p && (int64_t)K.n >= n) return;
+  K.ensure((size_t)n);
+  launchZeroF(K.p, n, s);
+}
+
 static void checkErrFlag(ik_handle* h) {
   if (!h->herr) IK_HIP(hipHostMalloc((void**)&h->herr, 2 * sizeof(unsigned long long), hipHostMallocDefault));
   IK_HIP(hipMemcpyAsync(h->herr, h->errFlag.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
@@ -1141,11 +1154,11 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
   const bool easBoth = h->neas && wantK;
   const bool sideR   = !h->neas && wantK && wantR;
   if (wantK && precision == IK_FP64) {
-    h->K64.ensure((size_t)h->nnzb * 9);
+    ensureK(h->K64, h->nsell * 9, h->stream);
     h->ebeK64.ensure((size_t)h->nstage * 9);
     runChunks<double>(h, dbc, easBoth, sideR, h->ebeK64.p, h->K64.p);
   } else if (wantK) {
-    h->K32.ensure((size_t)h->nnzb * 9);
+    ensureK(h->K32, h->nsell * 9, h->stream);
     if (h->neas) {
       h->ebeK64.ensure((size_t)h->nstage * 9);
       runChunks<float>(h, dbc, easBoth, sideR, nullptr, h->K32.p);

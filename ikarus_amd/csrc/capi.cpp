@@ -723,6 +723,25 @@ int ik_assemble(ik_handle* h, int dbc, int precision, double* R, int where) {
   });
 }
 
+// The assembled blocks in CSR order (block b of rowptr / colidx at [9 b, 9 b + 9)), in FP64, from the device's
+// sliced-ELL slots
+static std::vector<double> downloadK(ik_handle* h) {
+  const size_t ns = (size_t)h->nsell * 9;
+  std::vector<double> dev(ns), K((size_t)h->nnzb * 9);
+  if (h->kPrecision == IK_FP64) {
+    IK_HIP(hipMemcpyAsync(dev.data(), h->K64.p, ns * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    IK_HIP(hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<float> f(ns);
+    IK_HIP(hipMemcpyAsync(f.data(), h->K32.p, ns * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    IK_HIP(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < ns; ++i) dev[i] = f[i];
+  }
+  for (int64_t b = 0; b < h->nnzb; ++b)
+    std::copy_n(dev.begin() + (size_t)h->hpos[b] * 9, 9, K.begin() + (size_t)b * 9);
+  return K;
+}
+
 // Eigen CSC of the plane-strain matrix from the extrusion's blocks: column 2i+k holds rows 2j+l for the 2D
 // neighbours j of node i (the layer-0 columns of 3D block row i, sorted), K2 = sqrt(2) (K(j, i) + K(j, i + nn2))
 // restricted to the in-plane components; FULL then sets the constrained rows and columns of the identity
@@ -753,16 +772,7 @@ static void csc2d(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* oute
     require(h->kValid, IK_EINVAL, "no matrix assembled");
     require((dbc == IK_DBC_FULL) == (h->kDbc == IK_DBC_FULL), IK_EINVAL,
             "requested DBC layout differs from the assembled matrix");
-    K.resize((size_t)h->nnzb * 9);
-    if (h->kPrecision == IK_FP64) {
-      IK_HIP(hipMemcpyAsync(K.data(), h->K64.p, K.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-      IK_HIP(hipStreamSynchronize(h->stream));
-    } else {
-      std::vector<float> Kf(K.size());
-      IK_HIP(hipMemcpyAsync(Kf.data(), h->K32.p, Kf.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-      IK_HIP(hipStreamSynchronize(h->stream));
-      for (size_t i = 0; i < K.size(); ++i) K[i] = Kf[i];
-    }
+    K = downloadK(h);
   }
   auto blockAt = [&](int64_t r, int64_t c) {  // index of block (r, c) of the 3D pattern
     const auto first = h->hcolidx.begin() + h->hrowptr[r], last = h->hcolidx.begin() + h->hrowptr[r + 1];
@@ -837,16 +847,7 @@ int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* oute
       require(h->kValid, IK_EINVAL, "no matrix assembled");
       require((dbc == IK_DBC_FULL) == (h->kDbc == IK_DBC_FULL), IK_EINVAL,
               "requested DBC layout differs from the assembled matrix");
-      K.resize((size_t)h->nnzb * 9);
-      if (h->kPrecision == IK_FP64) {
-        IK_HIP(hipMemcpyAsync(K.data(), h->K64.p, K.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        IK_HIP(hipStreamSynchronize(h->stream));
-      } else {
-        std::vector<float> Kf(K.size());
-        IK_HIP(hipMemcpyAsync(Kf.data(), h->K32.p, Kf.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-        IK_HIP(hipStreamSynchronize(h->stream));
-        for (size_t i = 0; i < K.size(); ++i) K[i] = Kf[i];
-      }
+      K = downloadK(h);
     }
     if (outer)
       for (int64_t c = 0; c <= N; ++c) outer[c] = (int32_t)colStart[c];
@@ -876,16 +877,19 @@ int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* oute
 }
 
 int ik_matrix_bsr(ik_handle* h, int64_t* nbr, int64_t* nnzb, const int32_t** rowptr, const int32_t** colidx,
-                  const void** values, int* precision) {
+                  const int32_t** block_pos, const void** values, int* precision) {
   return guard(h, [&] {
     require(!h->plane2d, IK_ENOTIMPL, "the block view is the 3D storage; a plane-strain handle exports CSC only");
     ensurePattern(h);
+    if (block_pos && !h->blockPos.p) h->blockPos.upload(h->hpos.data(), h->hpos.size(), h->stream);
     if (nbr) *nbr = h->nn;
     if (nnzb) *nnzb = h->nnzb;
     if (rowptr) *rowptr = h->rowptr.p;
     if (colidx) *colidx = h->colidx.p;
+    if (block_pos) *block_pos = h->blockPos.p;
     if (values) *values = h->kPrecision == IK_FP32 ? (const void*)h->K32.p : (const void*)h->K64.p;
     if (precision) *precision = h->kPrecision;
+    IK_HIP(hipStreamSynchronize(h->stream));
   });
 }
 

@@ -80,6 +80,9 @@ constexpr int kReduceBlocks = 1024;
 #ifndef IK_TILE_ELEMS
 #define IK_TILE_ELEMS 16
 #endif
+// block rows per slice of the sliced-ELL storage of K
+constexpr int kSellRows = 64;
+
 constexpr int kTileElems   = IK_TILE_ELEMS;
 constexpr int kTileThreads = kTileElems * 16;
 constexpr int kTileEntries = kTileElems * 36;  // upper element blocks per tile
@@ -136,9 +139,9 @@ struct ik_handle {
   bool patternBuilt = false;
   std::vector<int32_t> hrowptr, hcolidx, hdiag;
   int64_t nnzb = 0;
-  ik::DevBuf<int32_t> rowptr, colidx, diagBlk, blockRow;
-  // the upper blocks (column >= row) the gather assembles, in row order: their block index and the index
-  // of their transpose; huptr[r] = first of them in row r
+  ik::DevBuf<int32_t> rowptr, colidx, diagBlk;
+  // the upper blocks (column >= row) the gather assembles, in row order: the storage slot of each and of its
+  // transpose; huptr[r] = first of them in row r
   std::vector<int64_t> huptr;
   ik::DevBuf<int32_t> upBlk, upT;
   // FULL Dirichlet treatment per upper block: bit t (t = 3k+l) zeroes entry t (constrained row or column),
@@ -153,7 +156,7 @@ struct ik_handle {
   //  block touched by one tile only is then complete and written straight into K; the others get one
   //  staging slot per group (segments ordered by tile). tileElem[t*16+s] = element (-1: padding of the
   //  last tile); tgptr[t] = first group of tile t (groups ordered by upper block); gdst[g] = the slot of
-  //  group g, or -(K block + 1) for a direct group, whose mirror block is gpt[g] and FULL mask gmsk[g].
+  //  group g, or -(K slot + 1) for a direct group, whose mirror slot is gpt[g] and FULL mask gmsk[g].
   //  epos[t*576 + j] = group | rank << 10 | transposed << 15 of the tile's entry j = 36 s + sb (rank = its
   //  position in the group by increasing j; padding entries target spare accumulators past the groups),
   //  tmaxr[t] = the tile's largest group size.
@@ -170,7 +173,16 @@ struct ik_handle {
   // element-by-element staging of K_e (36 upper 3x3 blocks) and R_e (24)
   ik::DevBuf<double> ebeK64, ebeR;
   ik::DevBuf<float> ebeK32;
-  // K storage
+  // K storage: sliced ELL over the block rows. Rows come in slices of kSellRows; slice s holds (its longest row) x
+  // kSellRows slots, slot sellStart[s] + j * kSellRows + r is block j (in column order) of the slice's row r, so for
+  // a fixed j the slice's rows are adjacent. Each slot holds one 3x3 block, row-major (9 values). Block b of the
+  // CSR pattern (rowptr / colidx order) lives in slot hpos[b]; scol[slot] is its column (padding slots: the row
+  // itself, with zero values). Every K-addressing index array (diagBlk, upBlk, upT, the direct groups' gdst and
+  // gpt) holds slots.
+  int64_t nsell = 0;
+  std::vector<int32_t> hpos;
+  ik::DevBuf<int32_t> sellStart, scol;
+  ik::DevBuf<int32_t> blockPos;  // hpos on the device, uploaded when the block view asks for it
   ik::DevBuf<double> K64;
   ik::DevBuf<float> K32;
   int kPrecision = -1, kDbc = -1;

@@ -1,12 +1,12 @@
-// pattern.cpp — 3x3-block CSR sparsity pattern and element block-slot maps.
+// pattern.cpp — 3x3-block sparsity pattern, its sliced-ELL storage slots and element block-slot maps.
 //
 // Ikarus builds the scalar pattern from the union of element DOF cliques with Eigen setFromTriplets
 // and keeps per element (3*8)^2 positions into the CSC value array (simpleassemblers.inl:207-222,
 // 254-265; eigensparseaddon.hh:19-29). For FlatInterleaved DOFs (dof = 3*node + k, fehelper.hh:146-152)
 // that scalar pattern is exactly the node-adjacency pattern expanded by 3x3 blocks, so the device keeps
-// the block form: block row = node, sorted block columns, and the maps from element blocks to staging
-// slots in the gather order of their K blocks (per element for EAS, per tile group for the plain
-// element). The scalar CSC is re-derived from it on demand.
+// the block form: block row = node, sorted block columns (CSR order), each block stored at a sliced-ELL
+// slot, and the maps from element blocks to staging slots in the gather order of their K blocks (per
+// element for EAS, per tile group for the plain element). The scalar CSC is re-derived from it on demand.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -185,6 +185,56 @@ void buildPattern(ik_handle* h) {
       h->hdiag[i] = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
     }
   });
+  // Sliced-ELL slots of K: slice s (rows [64 s, 64 s + 64)) is as wide as its longest row; block j of row i
+  // sits at slot sstart[s] + 64 j + (i mod 64). The last slice is padded to 64 rows too (rows past nn own only
+  // padding slots, which the kernels never read).
+  const int64_t nslices = (nn + kSellRows - 1) / kSellRows;
+  std::vector<int32_t> sstart(nslices + 1, 0);
+  {
+    int64_t tot = 0;
+    for (int64_t s = 0; s < nslices; ++s) {
+      int32_t w = 0;
+      for (int64_t i = s * kSellRows; i < std::min(nn, (s + 1) * kSellRows); ++i)
+        w = std::max(w, h->hrowptr[i + 1] - h->hrowptr[i]);
+      tot += (int64_t)w * kSellRows;
+      require(tot < (int64_t)INT32_MAX, IK_EINVAL, "pattern too large for int32 block slots");
+      sstart[s + 1] = (int32_t)tot;
+    }
+  }
+  h->nsell = sstart[nslices];
+  h->hpos.assign(h->nnzb, 0);
+  std::vector<int32_t> scol(h->nsell);
+  parallelChunks(nslices, [&](int64_t s0, int64_t s1, int64_t) {
+    for (int64_t s = s0; s < s1; ++s) {
+      const int32_t w = (sstart[s + 1] - sstart[s]) / kSellRows;
+      for (int r = 0; r < kSellRows; ++r) {
+        const int64_t i = s * kSellRows + r;
+        const int32_t len = i < nn ? h->hrowptr[i + 1] - h->hrowptr[i] : 0;
+        for (int32_t j = 0; j < w; ++j) {
+          const int32_t slot = sstart[s] + j * kSellRows + r;
+          if (j < len) {
+            h->hpos[h->hrowptr[i] + j] = slot;
+            scol[slot]                 = h->hcolidx[h->hrowptr[i] + j];
+          } else {
+            scol[slot] = (int32_t)std::min(i, nn - 1);
+          }
+        }
+      }
+    }
+  });
+  {
+    std::vector<uint8_t> seen(h->nsell, 0);
+    for (int64_t i = 0; i < nn; ++i)
+      for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1]; ++b) {
+        const int32_t sl = h->hpos[b];
+        require(sl >= sstart[i / kSellRows] && sl < sstart[i / kSellRows + 1] && !seen[sl] &&
+                    scol[sl] == h->hcolidx[b],
+                IK_EINVAL, "internal error: bad sliced-ELL slot");
+        seen[sl] = 1;
+      }
+  }
+  h->sellStart.upload(sstart.data(), sstart.size(), h->stream);
+  h->scol.upload(scol.data(), scol.size(), h->stream);
   // Element e computes its 36 upper blocks (a <= b) sb(a,b) = 8a - a(a-1)/2 + (b-a). Block (node_a, node_b)
   // is an upper block of K when node_a <= node_b and receives the element block as it is; otherwise the
   // upper block (node_b, node_a) receives its transpose. Staging slots are grouped by upper block and
@@ -434,6 +484,12 @@ void buildPattern(ik_handle* h) {
     }
     h->ntiles = ntiles;
     nunits    = ntiles;
+    // K blocks of the direct groups as storage slots
+    for (size_t g = 0; g < gq.size(); ++g)
+      if (gdst[g] < 0) {
+        gdst[g] = -(h->hpos[-gdst[g] - 1] + 1);
+        gpt[g]  = h->hpos[gpt[g]];
+      }
     h->tileElem.upload(tileElem.data(), tileElem.size(), h->stream);
     h->tgptr.upload(tgptr.data(), tgptr.size(), h->stream);
     h->gdst.upload(gdst.data(), gdst.size(), h->stream);
@@ -451,8 +507,8 @@ void buildPattern(ik_handle* h) {
     for (int64_t i = 0; i < nn; ++i) {
       for (int64_t q = h->huptr[i]; q < h->huptr[i + 1]; ++q)
         if (gathered[q]) {
-          cup.push_back(up[q]);
-          cupT.push_back(upT[q]);
+          cup.push_back(h->hpos[up[q]]);
+          cupT.push_back(h->hpos[upT[q]]);
           cmask.push_back(mask[q]);
           cseg.push_back(seg[q + 1]);
           require(seg[q] == cseg[cseg.size() - 2], IK_EINVAL, "internal error: gather segments not contiguous");
@@ -465,11 +521,6 @@ void buildPattern(ik_handle* h) {
     h->upT.upload(cupT.data(), cupT.size(), h->stream);
     h->segptr.upload(cseg.data(), cseg.size(), h->stream);
   }
-  std::vector<int32_t> brow(h->nnzb);
-  parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
-    for (int64_t i = i0; i < i1; ++i)
-      for (int32_t p = h->hrowptr[i]; p < h->hrowptr[i + 1]; ++p) brow[p] = (int32_t)i;
-  });
   // node incidences (e*8 + a), ordered by element
   std::vector<int32_t> ncnt(nn + 1, 0), nlist((size_t)ne * 8);
   for (int64_t i = 0; i < nn; ++i) ncnt[i + 1] = (int32_t)(cnt[i + 1] - cnt[0]);
@@ -504,8 +555,11 @@ void buildPattern(ik_handle* h) {
   }
   h->rowptr.upload(h->hrowptr.data(), h->hrowptr.size(), h->stream);
   h->colidx.upload(h->hcolidx.data(), h->hcolidx.size(), h->stream);
-  h->diagBlk.upload(h->hdiag.data(), h->hdiag.size(), h->stream);
-  h->blockRow.upload(brow.data(), brow.size(), h->stream);
+  {
+    std::vector<int32_t> dslot(nn);
+    for (int64_t i = 0; i < nn; ++i) dslot[i] = h->hpos[h->hdiag[i]];
+    h->diagBlk.upload(dslot.data(), dslot.size(), h->stream);
+  }
   // residual staging slot of every element corner: its position in the node's incidence list
   std::vector<int32_t> rdest((size_t)ne * 8);
   parallelChunks((int64_t)nlist.size(), [&](int64_t j0, int64_t j1, int64_t) {

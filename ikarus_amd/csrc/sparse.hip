@@ -1,4 +1,4 @@
-// sparse.hip — 3x3-block CSR SpMV, reductions and the preconditioned CG solvers.
+// sparse.hip — sliced-ELL 3x3-block SpMV, reductions and the preconditioned CG solvers.
 //
 // FP64 Jacobi-PCG follows Eigen's ConjugateGradient with DiagonalPreconditioner, the reference's
 // si_ConjugateGradient (ikarus/solver/linearsolver/linearsolver.cpp:23-24): x0 = 0,
@@ -16,11 +16,11 @@ namespace {
 
 constexpr int SPMV_BS = 256;  // threads per block
 
-// Block rows per wavefront of the SpMV lane layouts: FP64 values 9 lanes per row (7 rows), FP32 values
-// 3 lanes per row (21 rows); lane 63 idles in both.
+// Block rows per wavefront of the SpMV lane layouts: FP64 values 9 lanes per row (7 rows, lane 63 idles),
+// FP32 values one lane per row (64 rows).
 template <typename TV>
 constexpr int spmvRowsPerWave() {
-  return sizeof(TV) == 4 ? 21 : 7;
+  return sizeof(TV) == 4 ? 64 : 7;
 }
 template <typename TV>
 constexpr int spmvRowsPerBlock() {
@@ -53,22 +53,27 @@ __device__ inline double blockSum(double v) {
   return s;
 }
 
-// y = K x over 3x3 block rows; optional fused partial dot d = sum_i x_i y_i per block. Two lane layouts:
-//  FP64 values: lane (r, m), row r = lane / 9 (7 rows per wavefront), entry m = 3k + l of every block; each
-//   load instruction reads 7 contiguous 72-byte block runs; the three partial sums of row k are combined
-//   with two lane shifts.
-//  FP32 values: lane (r, k), row r = lane / 3 (21 rows per wavefront), row k of every block (3 contiguous
-//   values) against the 3 components of x at the block's column: three times the bytes per load
-//   instruction, which one value of 4 bytes per load could not stream (64^3 pattern: 5.5 vs 3.9 TB/s; for
-//   FP64 values the two layouts measured within 3 % in isolation and 135.6 vs 153.3 us inside the PCG).
-// Both issue the loads of 9 blocks (a third of an interior 27-block row) before summing their products;
+// y = K x over the sliced-ELL block storage (slice of kSellRows = 64 block rows, block j of row r of slice s
+// at slot sellStart[s] + 64 j + r, 9 values per slot); optional fused partial dot d = sum_i x_i y_i per
+// block. Two lane layouts:
+//  FP64 values: lane (r, m), row r = lane / 9 (7 rows per wavefront), entry m = 3k + l of every block; for a
+//   fixed j the wavefront's 7 rows read 7 adjacent 72-byte blocks (one 504-byte run per load instruction);
+//   the three partial sums of row k are combined with two lane shifts. Each row runs over its own blocks in
+//   column order, so the result equals a CSR SpMV's bit for bit.
+//  FP32 values: one lane per block row, the 3 outputs of the row per lane, a wavefront = one slice: for a
+//   fixed j the 64 lanes read 64 adjacent 36-byte blocks (2304 contiguous bytes). Each lane runs to the
+//   slice's longest row; padding slots hold zeros with the row's own column.
+// 64^3 pattern, alone: FP32 48.6 us against 62.3 us for 3 lanes per row over CSR (21 rows per wavefront);
+// FP64 126.1 us against 127.9 us for the same 9-lane layout over CSR.
+// FP64 issues the loads of 9 blocks (a third of an interior 27-block row) before summing their products;
 // 9 blocks in flight beat 3, 4, 6, 12, 14 and 18.
 constexpr int SPMV_UNROLL = 9;
+static_assert(kSellRows == 64, "one slice per wavefront");
 
-// acc += sum over the row's blocks of V[j*9] * x[C[j]*3 + l] (in block order)
+// acc += sum over the row's n blocks of V[j*64*9] * x[C[j*64]*3 + l] (in block order)
 template <typename TV, typename TX, typename TA>
-__device__ inline TA rowDot(const TV* __restrict__ V, const int32_t* __restrict__ C, int n, const TX* __restrict__ x,
-                            int l) {
+__device__ inline TA rowDotSell(const TV* __restrict__ V, const int32_t* __restrict__ C, int n,
+                                const TX* __restrict__ x, int l) {
   TA acc = TA(0);
   int j  = 0;
   for (; j + SPMV_UNROLL <= n; j += SPMV_UNROLL) {
@@ -76,98 +81,94 @@ __device__ inline TA rowDot(const TV* __restrict__ V, const int32_t* __restrict_
     TV v[SPMV_UNROLL];
 #pragma unroll
     for (int u = 0; u < SPMV_UNROLL; ++u) {
-      c[u] = C[j + u];
-      v[u] = V[(int64_t)(j + u) * 9];
+      c[u] = C[(j + u) * kSellRows];
+      v[u] = V[(int64_t)(j + u) * (kSellRows * 9)];
     }
 #pragma unroll
     for (int u = 0; u < SPMV_UNROLL; ++u) acc += (TA)v[u] * (TA)x[(int64_t)c[u] * 3 + l];
   }
-  for (; j < n; ++j) acc += (TA)V[(int64_t)j * 9] * (TA)x[(int64_t)C[j] * 3 + l];
+  for (; j < n; ++j) acc += (TA)V[(int64_t)j * (kSellRows * 9)] * (TA)x[(int64_t)C[j * kSellRows] * 3 + l];
   return acc;
 }
 
-// sum over the row's blocks of V[j*9 + 0..2] . x[C[j]*3 + 0..2] (in block order)
+// a[k] = (K x)[3 row + k], k = 0..2, for one lane per row: the slice's w slots of row `row`
 template <typename TV, typename TX, typename TA>
-__device__ inline TA rowDot3(const TV* __restrict__ V, const int32_t* __restrict__ C, int n,
-                             const TX* __restrict__ x) {
-  TA acc = TA(0);
-  int j  = 0;
-  for (; j + SPMV_UNROLL <= n; j += SPMV_UNROLL) {
-    int32_t c[SPMV_UNROLL];
-    TV v[SPMV_UNROLL][3];
-    TX xv[SPMV_UNROLL][3];
-#pragma unroll
-    for (int u = 0; u < SPMV_UNROLL; ++u) {
-      c[u] = C[j + u];
-#pragma unroll
-      for (int l = 0; l < 3; ++l) v[u][l] = V[(int64_t)(j + u) * 9 + l];
-    }
-#pragma unroll
-    for (int u = 0; u < SPMV_UNROLL; ++u)
-#pragma unroll
-      for (int l = 0; l < 3; ++l) xv[u][l] = x[(int64_t)c[u] * 3 + l];
-#pragma unroll
-    for (int u = 0; u < SPMV_UNROLL; ++u)
-      acc += (TA)v[u][0] * (TA)xv[u][0] + (TA)v[u][1] * (TA)xv[u][1] + (TA)v[u][2] * (TA)xv[u][2];
+__device__ inline void rowSell3(const int32_t* __restrict__ sellStart, const int32_t* __restrict__ scol,
+                                const TV* __restrict__ val, const TX* __restrict__ x, int64_t row, TA a[3]) {
+  const int64_t s = row >> 6;
+  const int r     = (int)(row & 63);
+  const int32_t s0 = sellStart[s];
+  const int w      = (sellStart[s + 1] - s0) >> 6;
+  const TV* V      = val + ((int64_t)s0 + r) * 9;
+  const int32_t* C = scol + s0 + r;
+  TA a0 = TA(0), a1 = TA(0), a2 = TA(0);
+#pragma unroll 3
+  for (int j = 0; j < w; ++j) {
+    const int64_t c = (int64_t)C[j * kSellRows] * 3;
+    const TA x0 = (TA)x[c], x1 = (TA)x[c + 1], x2 = (TA)x[c + 2];
+    const TV* v = V + (int64_t)j * (kSellRows * 9);
+    a0 += (TA)v[0] * x0 + (TA)v[1] * x1 + (TA)v[2] * x2;
+    a1 += (TA)v[3] * x0 + (TA)v[4] * x1 + (TA)v[5] * x2;
+    a2 += (TA)v[6] * x0 + (TA)v[7] * x1 + (TA)v[8] * x2;
   }
-  for (; j < n; ++j) {
-    const int64_t cc = (int64_t)C[j] * 3;
-    acc += (TA)V[(int64_t)j * 9] * (TA)x[cc] + (TA)V[(int64_t)j * 9 + 1] * (TA)x[cc + 1] +
-           (TA)V[(int64_t)j * 9 + 2] * (TA)x[cc + 2];
-  }
-  return acc;
+  a[0] = a0;
+  a[1] = a1;
+  a[2] = a2;
 }
 
-// (K x)[3 row + k] accumulated in TA, for the lane layout of the value type; valid on lanes with `out` set
+// (K x)[3 row + k] accumulated in TA with the FP64 lane layout; valid on lanes with `out` set
 template <typename TV, typename TX, typename TA>
-__device__ inline TA spmvRow(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-                             const TV* __restrict__ val, const TX* __restrict__ x, int64_t nrows, int64_t& row,
-                             int& k, bool& out) {
+__device__ inline TA spmvRow9(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ sellStart,
+                              const int32_t* __restrict__ scol, const TV* __restrict__ val,
+                              const TX* __restrict__ x, int64_t nrows, int64_t& row, int& k, bool& out) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rl = lane / 9;
+  const int m  = lane - rl * 9;
+  k            = m / 3;
+  const int l  = m - 3 * k;
+  row          = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * spmvRowsPerWave<TV>() + rl;
+  const bool active = rl < spmvRowsPerWave<TV>() && row < nrows;
   TA acc = TA(0);
-  if constexpr (sizeof(TV) == 4) {
-    const int rl = lane / 3;
-    k            = lane - 3 * rl;
-    row          = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * spmvRowsPerWave<TV>() + rl;
-    out          = rl < spmvRowsPerWave<TV>() && row < nrows;
-    if (out) {
-      const int32_t rb = rowptr[row], re = rowptr[row + 1];
-      acc = rowDot3<TV, TX, TA>(val + (int64_t)rb * 9 + 3 * k, colidx + rb, re - rb, x);
-    }
-  } else {
-    const int rl = lane / 9;
-    const int m  = lane - rl * 9;
-    k            = m / 3;
-    const int l  = m - 3 * k;
-    row          = ((int64_t)blockIdx.x * (SPMV_BS / 64) + wave) * spmvRowsPerWave<TV>() + rl;
-    const bool active = rl < spmvRowsPerWave<TV>() && row < nrows;
-    if (active) {
-      const int32_t rb = rowptr[row], re = rowptr[row + 1];
-      acc = rowDot<TV, TX, TA>(val + (int64_t)rb * 9 + m, colidx + rb, re - rb, x, l);
-    }
-    const TA a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
-    acc += a1 + a2;
-    out = active && l == 0;
+  if (active) {
+    const int64_t base = (int64_t)sellStart[row >> 6] + (row & 63);
+    acc = rowDotSell<TV, TX, TA>(val + base * 9 + m, scol + base, rowptr[row + 1] - rowptr[row], x, l);
   }
+  const TA a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
+  acc += a1 + a2;
+  out = active && l == 0;
   return acc;
 }
 
 template <typename TV, typename TX, bool DOT>
 __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict__ rowptr,
-                                                      const int32_t* __restrict__ colidx,
+                                                      const int32_t* __restrict__ sellStart,
+                                                      const int32_t* __restrict__ scol,
                                                       const TV* __restrict__ val, const TX* __restrict__ x,
                                                       TX* __restrict__ y, int64_t nrows,
                                                       double* __restrict__ partial,
                                                       const PcgScalars* __restrict__ sc) {
   if (sc && sc->done) return;
-  int64_t row;
-  int k;
-  bool out;
-  const TX yk = spmvRow<TV, TX, TX>(rowptr, colidx, val, x, nrows, row, k, out);
-  double d    = 0.0;
-  if (out) {
-    y[row * 3 + k] = yk;
-    if (DOT) d = (double)x[row * 3 + k] * (double)yk;
+  double d = 0.0;
+  if constexpr (sizeof(TV) == 4) {
+    const int64_t row = (int64_t)blockIdx.x * SPMV_BS + threadIdx.x;
+    if (row < nrows) {
+      TX a[3];
+      rowSell3<TV, TX, TX>(sellStart, scol, val, x, row, a);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        y[row * 3 + k] = a[k];
+        if (DOT) d += (double)x[row * 3 + k] * (double)a[k];
+      }
+    }
+  } else {
+    int64_t row;
+    int k;
+    bool out;
+    const TX yk = spmvRow9<TV, TX, TX>(rowptr, sellStart, scol, val, x, nrows, row, k, out);
+    if (out) {
+      y[row * 3 + k] = yk;
+      if (DOT) d = (double)x[row * 3 + k] * (double)yk;
+    }
   }
   if (DOT) {
     const double s = blockSum<SPMV_BS>(d);
@@ -178,20 +179,35 @@ __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict_
 // r = b - K x with FP64 accumulation (K may be FP32); used by iterative refinement. Same lane layouts.
 template <typename TV>
 __global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restrict__ rowptr,
-                                                          const int32_t* __restrict__ colidx,
+                                                          const int32_t* __restrict__ sellStart,
+                                                          const int32_t* __restrict__ scol,
                                                           const TV* __restrict__ val,
                                                           const double* __restrict__ x,
                                                           const double* __restrict__ b, double* __restrict__ r,
                                                           int64_t nrows, double* __restrict__ partial) {
-  int64_t row;
-  int k;
-  bool out;
-  const double kx = spmvRow<TV, double, double>(rowptr, colidx, val, x, nrows, row, k, out);
-  double d        = 0.0;
-  if (out) {
-    const double rk = b[row * 3 + k] - kx;
-    r[row * 3 + k]  = rk;
-    d               = rk * rk;
+  double d = 0.0;
+  if constexpr (sizeof(TV) == 4) {
+    const int64_t row = (int64_t)blockIdx.x * SPMV_BS + threadIdx.x;
+    if (row < nrows) {
+      double a[3];
+      rowSell3<TV, double, double>(sellStart, scol, val, x, row, a);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double rk = b[row * 3 + k] - a[k];
+        r[row * 3 + k]  = rk;
+        d += rk * rk;
+      }
+    }
+  } else {
+    int64_t row;
+    int k;
+    bool out;
+    const double kx = spmvRow9<TV, double, double>(rowptr, sellStart, scol, val, x, nrows, row, k, out);
+    if (out) {
+      const double rk = b[row * 3 + k] - kx;
+      r[row * 3 + k]  = rk;
+      d               = rk * rk;
+    }
   }
   const double s = blockSum<SPMV_BS>(d);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -494,7 +510,7 @@ void spmv64(ik_handle* h, const double* x, double* y) {
   const unsigned grid = spmvGrid<double>(h->nn);
   ScopedKernelTimer t(h, KSPMV);
   hipLaunchKernelGGL((spmvKernel<double, double, false>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
-                     h->colidx.p, h->K64.p, x, y, h->nn, nullptr, nullptr);
+                     h->sellStart.p, h->scol.p, h->K64.p, x, y, h->nn, nullptr, nullptr);
 }
 
 void spmv32(ik_handle* h, const float* x, float* y) {
@@ -502,7 +518,7 @@ void spmv32(ik_handle* h, const float* x, float* y) {
   const unsigned grid = spmvGrid<float>(h->nn);
   ScopedKernelTimer t(h, KSPMV);
   hipLaunchKernelGGL((spmvKernel<float, float, false>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
-                     h->colidx.p, h->K32.p, x, y, h->nn, nullptr, nullptr);
+                     h->sellStart.p, h->scol.p, h->K32.p, x, y, h->nn, nullptr, nullptr);
 }
 
 double dot(ik_handle* h, const double* a, const double* b, int64_t n) {
@@ -535,8 +551,8 @@ void axpy(ik_handle* h, double a, const double* x, double* y, int64_t n) {
 void residualBlock64(ik_handle* h, const double* b, const double* x, double* r) {
   ensureWork(h);
   const unsigned grid = spmvGrid<double>(h->nn);
-  hipLaunchKernelGGL((residualKernel<double>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p, h->colidx.p,
-                     h->K64.p, x, b, r, h->nn, h->partial.p);
+  hipLaunchKernelGGL((residualKernel<double>), dim3(grid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p, h->sellStart.p,
+                     h->scol.p, h->K64.p, x, b, r, h->nn, h->partial.p);
 }
 
 // Poll interval: iterations enqueued between host checks of the done flag.
@@ -563,7 +579,7 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
       {
         ScopedKernelTimer t(h, KSPMV);
         hipLaunchKernelGGL((spmvKernel<double, double, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
-                           h->colidx.p, h->K64.p, h->pp.p, h->pq.p, h->nn, h->partial.p, h->sc.p);
+                           h->sellStart.p, h->scol.p, h->K64.p, h->pp.p, h->pq.p, h->nn, h->partial.p, h->sc.p);
       }
       hipLaunchKernelGGL(pcgPqFold, dim3(kPqBlocks), dim3(256), 0, h->stream, h->partial.p, (int)sgrid, pq2(h),
                          h->sc.p);
@@ -625,7 +641,8 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
         {
           ScopedKernelTimer t(h, KSPMV);
           hipLaunchKernelGGL((spmvKernel<float, float, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream,
-                             h->rowptr.p, h->colidx.p, h->K32.p, h->fp.p, h->fq.p, h->nn, h->partial.p, h->sc.p);
+                             h->rowptr.p, h->sellStart.p, h->scol.p, h->K32.p, h->fp.p, h->fq.p, h->nn, h->partial.p,
+                             h->sc.p);
         }
         hipLaunchKernelGGL(pcgPqFold, dim3(kPqBlocks), dim3(256), 0, h->stream, h->partial.p, (int)sgrid, pq2(h),
                            h->sc.p);
@@ -642,8 +659,8 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
     info->refinements += 1;
     // x += ||r|| d ; r = b - K x in FP64
     hipLaunchKernelGGL(accumulateF32, dim3(g), dim3(256), 0, h->stream, x, h->fx.p, rnorm, n);
-    hipLaunchKernelGGL((residualKernel<float>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p, h->colidx.p,
-                       h->K32.p, x, b, h->pr.p, h->nn, h->partial.p);
+    hipLaunchKernelGGL((residualKernel<float>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p, h->sellStart.p,
+                       h->scol.p, h->K32.p, x, b, h->pr.p, h->nn, h->partial.p);
     hipLaunchKernelGGL(foldKernel, dim3(1), dim3(1024), 0, h->stream, h->partial.p, (int)sgrid,
                        h->partial.p + h->partial.n - 1);
     IK_HIP(hipMemcpyAsync(h->hred, h->partial.p + h->partial.n - 1, sizeof(double), hipMemcpyDeviceToHost,

@@ -123,7 +123,7 @@ int ik_energy(ik_handle* h, double* energy);
 int ik_stress_at(ik_handle* h, int kind, const double xi[3], double* out, int where);
 
 /* MatrixAssembler::matrix(req, stiffness, dbc) (interface.hh:430-440; simpleassemblers.inl:121-204).
- * Assembles into library-owned 3x3-block CSR storage on the device (FP64, or FP32 for IK_FP32; the EAS
+ * Assembles into library-owned 3x3-block storage on the device (FP64, or FP32 for IK_FP32; the EAS
  * element computes in FP64 and its FP32 matrix is the FP64 one rounded entry by entry). */
 int ik_matrix(ik_handle* h, int dbc, int precision);
 
@@ -139,9 +139,11 @@ int ik_assemble(ik_handle* h, int dbc, int precision, double* R, int where);
 int ik_matrix_csc(ik_handle* h, int dbc, int64_t* n, int64_t* nnz, int32_t* outer, int32_t* inner,
                   double* values);
 
-/* Device view of the block storage: block rows = nodes, 3x3 row-major blocks (values FP64 or FP32). */
+/* Device view of the block storage: block rows = nodes, 3x3 row-major blocks (values FP64 or FP32). rowptr /
+ * colidx give the block CSR pattern; block b of it (rowptr order) holds values[9*block_pos[b] .. +9): the
+ * values are stored in 64-row slices (sliced ELL) with zero padding between the blocks. Any output may be NULL. */
 int ik_matrix_bsr(ik_handle* h, int64_t* num_block_rows, int64_t* nnzb, const int32_t** rowptr,
-                  const int32_t** colidx, const void** values, int* precision);
+                  const int32_t** colidx, const int32_t** block_pos, const void** values, int* precision);
 
 /* calculateMatrix / calculateVector for one element at the bound requirement (mixin.hh:137-152):
  * Ke row-major 24x24 (local order 3a+k, fehelper.hh:146-152), Re 24. Host outputs, either may be NULL. */

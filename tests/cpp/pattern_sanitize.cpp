@@ -123,7 +123,9 @@ std::string buildOnce(const Mesh& m, int eas, int threads) {
   put(out, h.rowptr);
   put(out, h.colidx);
   put(out, h.diagBlk);
-  put(out, h.blockRow);
+  put(out, h.hpos);
+  put(out, h.sellStart);
+  put(out, h.scol);
   put(out, h.upBlk);
   put(out, h.upT);
   put(out, h.upMask);
@@ -138,7 +140,7 @@ std::string buildOnce(const Mesh& m, int eas, int threads) {
   put(out, h.epos);
   put(out, h.gmsk);
   put(out, h.tmaxr);
-  const int64_t sizes[3] = {h.nnzb, h.ntiles, h.nstage};
+  const int64_t sizes[4] = {h.nnzb, h.ntiles, h.nstage, h.nsell};
   out.append(reinterpret_cast<const char*>(sizes), sizeof(sizes));
   return out;
 }

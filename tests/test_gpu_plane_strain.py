@@ -165,7 +165,7 @@ def test_plane_strain_rejects_3d_only_features():
     print("case: plane strain rejections", flush=True)
     dev = ia.Device(p)
     with pytest.raises(capi.IkError) as ei:
-        dev.check(capi.lib().ik_matrix_bsr(dev.h, None, None, None, None, None, None))
+        dev.check(capi.lib().ik_matrix_bsr(dev.h, None, None, None, None, None, None, None))
     assert ei.value.code == capi.IK_ENOTIMPL
     dev.set_requirement(np.zeros(p.ndof), 0.0)
     cfg = capi.PFConfig()
