@@ -296,7 +296,9 @@ def kernel_pass(dev, p, steps):
 
 
 def spmv_pass(dev, p, precision, iters=50):
-    """Average SpMV launch inside the PCG of the given precision (FULL K assembled first)."""
+    """Average SpMV launch inside the PCG of the given precision (FULL K assembled first, at u = 0 so that K is
+    positive definite and the CG runs all its iterations)."""
+    dev.set_requirement(np.zeros(p.ndof), 1.0)
     dev.check(capi.lib().ik_matrix(dev.h, capi.DBC_FULL, precision))
     x = np.random.default_rng(1).standard_normal(p.ndof)
     kind = capi.PCG_JACOBI_FP64 if precision == capi.FP64 else capi.PCG_BJACOBI_FP32_IR

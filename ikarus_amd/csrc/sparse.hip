@@ -291,8 +291,10 @@ __global__ __launch_bounds__(1024) void pcgInitFold(const double* __restrict__ p
 
 // First stage of the p.q fold: block b sums its slice of the SpMV's per-block partials, in order, into
 // pq2[b] (kPqBlocks values). A single block summing one partial per SpMV block took 17 us per iteration at
-// 1 M elements (38,631 partials); the update kernel folds the kPqBlocks values itself.
+// 1 M elements (38,631 partials); the update kernel folds the kPqBlocks values itself. Launches of at most
+// kPqDirect SpMV blocks skip this stage: every update block then folds the SpMV's partials directly.
 constexpr int kPqBlocks = 64;
+constexpr int kPqDirect = 2048;
 __global__ __launch_bounds__(256) void pcgPqFold(const double* __restrict__ partial, int nb,
                                                  double* __restrict__ pq2, const PcgScalars* __restrict__ sc) {
   if (sc->done) return;
@@ -304,20 +306,21 @@ __global__ __launch_bounds__(256) void pcgPqFold(const double* __restrict__ part
   if (threadIdx.x == 0) pq2[blockIdx.x] = s;
 }
 
-// alpha = r.z / p.q (every block folds the kPqBlocks values of pcgPqFold in the same order, so all blocks
-// agree bitwise; block 0 records the scalars), then x += alpha p, r -= alpha q with partials of r.r and
-// r.z (z = M^-1 r)
+// alpha = r.z / p.q (every block folds the npq values pq[] -- pcgPqFold's kPqBlocks sums or the SpMV's
+// partials -- in the same order, so all blocks agree bitwise; block 0 records the scalars), then
+// x += alpha p, r -= alpha q with partials of r.r and r.z (z = M^-1 r)
 template <typename T>
 __global__ __launch_bounds__(256) void pcgUpdateKernel(T* __restrict__ x, T* __restrict__ r,
                                                        const T* __restrict__ p, const T* __restrict__ q,
                                                        const T* __restrict__ pre, int blockPre, int64_t n,
-                                                       PcgScalars* __restrict__ sc, const double* __restrict__ pq2,
-                                                       double* __restrict__ partial /* 2*grid */) {
+                                                       PcgScalars* __restrict__ sc, const double* __restrict__ pq,
+                                                       int npq, double* __restrict__ partial /* 2*grid */) {
   if (sc->done) return;
   __shared__ double alphaS;
   {
-    double v = threadIdx.x < kPqBlocks ? pq2[threadIdx.x] : 0.0;
-    v        = blockSum<256>(v);
+    double v = 0.0;
+    for (int i = threadIdx.x; i < npq; i += 256) v += pq[i];
+    v = blockSum<256>(v);
     if (threadIdx.x == 0) {
       const double rz = sc->rz_new;  // r.z of the current iterate (set by the previous p update)
       alphaS          = rz / v;
@@ -490,13 +493,17 @@ inline int vecGrid(int64_t n) {
 }  // namespace
 
 static double* pq2(ik_handle* h) { return h->partial.p + h->partial.n - 8 - kPqBlocks; }
+// the PCG SpMV's per-block p.q partials, apart from the vector kernels' partials at the start (the update
+// kernel reads the former while its blocks write the latter)
+static double* pqPartials(ik_handle* h) { return h->partial.p + 2 * kReduceBlocks; }
 
-// partial[] holds one launch's per-block partials at its start, the p.q first-stage sums (kPqBlocks) in a
-// region of their own before the last 8 entries, and fold results in its last entry
+// partial[] holds one launch's per-block partials at its start, the PCG SpMV's partials from 2 kReduceBlocks
+// on, the p.q first-stage sums (kPqBlocks) in a region of their own before the last 8 entries, and fold
+// results in its last entry
 static void ensureWork(ik_handle* h) {
   const int64_t n = h->ndof;
   const int64_t spmvBlocks = spmvGrid<double>(h->nn);  // the FP64 layout launches the most blocks
-  h->partial.ensure((size_t)std::max<int64_t>(spmvBlocks, 2 * kReduceBlocks) + kPqBlocks + 8);
+  h->partial.ensure((size_t)(2 * kReduceBlocks + spmvBlocks) + kPqBlocks + 8);
   if (!h->sc.p) {
     h->sc.ensure(1);
     IK_HIP(hipHostMalloc((void**)&h->hsc, sizeof(PcgScalars), hipHostMallocDefault));
@@ -579,12 +586,18 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
       {
         ScopedKernelTimer t(h, KSPMV);
         hipLaunchKernelGGL((spmvKernel<double, double, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
-                           h->sellStart.p, h->scol.p, h->K64.p, h->pp.p, h->pq.p, h->nn, h->partial.p, h->sc.p);
+                           h->sellStart.p, h->scol.p, h->K64.p, h->pp.p, h->pq.p, h->nn, pqPartials(h), h->sc.p);
       }
-      hipLaunchKernelGGL(pcgPqFold, dim3(kPqBlocks), dim3(256), 0, h->stream, h->partial.p, (int)sgrid, pq2(h),
-                         h->sc.p);
+      const double* pqSrc = pqPartials(h);
+      int npq             = (int)sgrid;
+      if (sgrid > (unsigned)kPqDirect) {
+        hipLaunchKernelGGL(pcgPqFold, dim3(kPqBlocks), dim3(256), 0, h->stream, pqPartials(h), (int)sgrid, pq2(h),
+                           h->sc.p);
+        pqSrc = pq2(h);
+        npq   = kPqBlocks;
+      }
       hipLaunchKernelGGL((pcgUpdateKernel<double>), dim3(g), dim3(256), 0, h->stream, x, h->pr.p, h->pp.p, h->pq.p,
-                         h->pdiag.p, 0, n, h->sc.p, pq2(h), h->partial.p);
+                         h->pdiag.p, 0, n, h->sc.p, pqSrc, npq, h->partial.p);
       hipLaunchKernelGGL((pcgBetaPKernel<double>), dim3(g), dim3(256), 0, h->stream, h->pp.p, h->pr.p, h->pdiag.p, 0,
                          n, h->partial.p, g, h->sc.p, maxIter);
     }
@@ -641,13 +654,19 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
         {
           ScopedKernelTimer t(h, KSPMV);
           hipLaunchKernelGGL((spmvKernel<float, float, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream,
-                             h->rowptr.p, h->sellStart.p, h->scol.p, h->K32.p, h->fp.p, h->fq.p, h->nn, h->partial.p,
-                             h->sc.p);
+                             h->rowptr.p, h->sellStart.p, h->scol.p, h->K32.p, h->fp.p, h->fq.p, h->nn,
+                             pqPartials(h), h->sc.p);
         }
-        hipLaunchKernelGGL(pcgPqFold, dim3(kPqBlocks), dim3(256), 0, h->stream, h->partial.p, (int)sgrid, pq2(h),
-                           h->sc.p);
+        const double* pqSrc = pqPartials(h);
+        int npq             = (int)sgrid;
+        if (sgrid > (unsigned)kPqDirect) {
+          hipLaunchKernelGGL(pcgPqFold, dim3(kPqBlocks), dim3(256), 0, h->stream, pqPartials(h), (int)sgrid,
+                             pq2(h), h->sc.p);
+          pqSrc = pq2(h);
+          npq   = kPqBlocks;
+        }
         hipLaunchKernelGGL((pcgUpdateKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fx.p, h->fr.p, h->fp.p,
-                           h->fq.p, h->fdiag.p, 1, n, h->sc.p, pq2(h), h->partial.p);
+                           h->fq.p, h->fdiag.p, 1, n, h->sc.p, pqSrc, npq, h->partial.p);
         hipLaunchKernelGGL((pcgBetaPKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fp.p, h->fr.p, h->fdiag.p,
                            1, n, h->partial.p, gn, h->sc.p, maxIter);
       }
