@@ -4,7 +4,7 @@ Workload (default): 3D hex8 Neo-Hooke block, 64x64x64 elements (823,875 DOFs), u
 seeded +-5% interior jitter (seed 0), bottom face clamped, seeded top-face traction, FP64.
 
 One step = the residual R and tangent K of one Newton state (the pair NewtonRaphson evaluates per state),
-FP64, FULL Dirichlet treatment, summed into the 3x3-block CSR pattern, inputs resident in HBM, through
+FP64, FULL Dirichlet treatment, summed into the 3x3-block pattern (sliced-ELL storage), inputs resident in HBM, through
 ik_assemble (the tangent's element kernel and gather, then the residual pass; one error check and one
 host synchronisation for the pair).
 value = elements assembled per second (Melem/s) over the timed steps. The same state assembled by the two
@@ -487,7 +487,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"hex8 Neo-Hooke R+K assembly (one Newton state), {n}x{n}x{n} elements, unit "
-                               "cube, 5% jitter seed 0, FULL Dirichlet, 3x3-block CSR", "elements": p.ne,
+                               "cube, 5% jitter seed 0, FULL Dirichlet, 3x3-block sliced ELL", "elements": p.ne,
                    "dofs": p.ndof, "block_nnz": nnzb, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,

@@ -627,9 +627,10 @@ __global__ __launch_bounds__(64) void hex8Residual(const double* __restrict__ X,
   }
 }
 
-// K gather: thread per (upper block, entry) over the upper blocks [q0, q1) (in row order). The staged
-// blocks of upper block q occupy the contiguous slots [segptr[q], segptr[q+1]) (ordered by element or
-// tile), so consecutive threads read consecutive values. The entry's sum is treated with FULL (rows/columns
+// K gather: thread per (upper block, entry) over the gathered upper blocks [q0, q1) of one staging chunk (in
+// gather order: by storage slot, so consecutive upper blocks are mostly neighbouring rows' blocks j, whose
+// slots, and whose mirrors' slots, are adjacent). The staged blocks of upper block q occupy the contiguous
+// slots [segptr[q], segptr[q+1]) (ordered by element or tile), so consecutive threads read consecutive values. The entry's sum is treated with FULL (rows/columns
 // of constrained DOFs zero, unit diagonal; simpleassemblers.inl:152-169) and written to the block and,
 // transposed, to its mirror block below the diagonal (K and the Dirichlet treatment are symmetric).
 // NT: the staged values are read once and K is written once, both non-temporally. For the tiled plain
@@ -1058,9 +1059,10 @@ static void launchChunk(ik_handle* h, int64_t u0, int64_t u1, void* Kebe, void* 
                                        (double*)K, mask);
 }
 
+// the gather of staging chunk c (the upper blocks of the rows it completes)
 template <typename T>
-static void launchGatherK(ik_handle* h, int64_t r0, int64_t r1, int mask, const T* Kebe, T* K, hipStream_t s) {
-  const int64_t q0 = h->huptr[r0], q1 = h->huptr[r1];
+static void launchGatherK(ik_handle* h, size_t c, int mask, const T* Kebe, T* K, hipStream_t s) {
+  const int64_t q0 = h->chunkQ[c], q1 = h->chunkQ[c + 1];
   if (q1 <= q0) return;  // every block of these rows was written by the element kernel
   const dim3 grid((unsigned)(((q1 - q0) * 9 + 255) / 256));
   if (h->neas)  // the EAS element stages FP64 blocks (h->ebeK64) whatever the matrix precision
@@ -1107,7 +1109,7 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T*
     }
     {
       ScopedKernelTimer t(h, KGATHER, h->stream, (int64_t)nc);
-      for (size_t c = 0; c < nc; ++c) launchGatherK<T>(h, h->chunkRow[c], h->chunkRow[c + 1], mask, Kebe, K, h->stream);
+      for (size_t c = 0; c < nc; ++c) launchGatherK<T>(h, c, mask, Kebe, K, h->stream);
     }
     if (sideR) {
       ScopedKernelTimer t(h, KASM_R, h->stream);
@@ -1119,7 +1121,7 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T*
   if (nc == 1) {
     // one chunk: the K gather follows on the main stream
     launchChunk(h, h->chunkElem[0], h->chunkElem[1], Kebe, K, fp32, mask);
-    launchGatherK<T>(h, h->chunkRow[0], h->chunkRow[1], mask, Kebe, K, h->stream);
+    launchGatherK<T>(h, 0, mask, Kebe, K, h->stream);
     if (sideR) launchResidual(h, h->stream);
     if (wantR || sideR) launchGatherR(h, mask, 0, h->nn, h->stream);
     return;
@@ -1130,7 +1132,7 @@ static void runChunks(ik_handle* h, int dbc, bool wantR, bool sideR, T* Kebe, T*
     if (r1 <= r0) continue;
     IK_HIP(hipEventRecord(h->chunkEvents[c], h->stream));
     IK_HIP(hipStreamWaitEvent(h->stream2, h->chunkEvents[c], 0));
-    launchGatherK<T>(h, r0, r1, mask, Kebe, K, h->stream2);
+    launchGatherK<T>(h, c, mask, Kebe, K, h->stream2);
     if (wantR) launchGatherR(h, mask, r0, r1, h->stream2);
   }
   if (sideR) launchResidual(h, h->stream);  // behind the last element chunk

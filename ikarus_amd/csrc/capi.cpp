@@ -1191,7 +1191,7 @@ int ik_assembly_layout(ik_handle* h, int64_t* staged, int64_t* gathered, int64_t
   return guard(h, [&] {
     ensurePattern(h);
     const int64_t nup = (h->nnzb + h->nn) / 2;  // upper blocks incl. the diagonal (the pattern is symmetric)
-    const int64_t ng  = h->huptr.empty() ? 0 : h->huptr[h->nn];
+    const int64_t ng  = h->chunkQ.empty() ? 0 : h->chunkQ.back();
     if (staged) *staged = h->nstage;
     if (gathered) *gathered = ng;
     if (direct) *direct = nup - ng;

@@ -140,14 +140,15 @@ struct ik_handle {
   std::vector<int32_t> hrowptr, hcolidx, hdiag;
   int64_t nnzb = 0;
   ik::DevBuf<int32_t> rowptr, colidx, diagBlk;
-  // the upper blocks (column >= row) the gather assembles, in row order: the storage slot of each and of its
-  // transpose; huptr[r] = first of them in row r
-  std::vector<int64_t> huptr;
+  // the upper blocks (column >= row) the gather assembles, in gather order (staging chunk by staging chunk, by
+  // storage slot inside a chunk): the storage slot of each and of its transpose; chunkQ[c] = first of chunk c
+  std::vector<int64_t> chunkQ;
   ik::DevBuf<int32_t> upBlk, upT;
   // FULL Dirichlet treatment per upper block: bit t (t = 3k+l) zeroes entry t (constrained row or column),
   // bit 9+k sets diagonal entry (k,k) of a diagonal block to one
   ik::DevBuf<uint16_t> upMask;
-  // Tangent staging: slots ordered upper block by upper block (segptr[q] = first slot of upper block q),
+  // Tangent staging: slots ordered upper block by upper block in gather order (segptr[q] = first slot of the
+  // q-th gathered upper block),
   // each segment in a fixed order, so the gather reads contiguous runs and sums them deterministically.
   //  EAS element: one slot per element block; bdest[e*36+sb] = (slot << 1) | transposed, segments ordered
   //  by element.
@@ -155,7 +156,7 @@ struct ik_handle {
   //  the blocks of a tile that fall on the same upper block form a group, summed in the tile. An upper
   //  block touched by one tile only is then complete and written straight into K; the others get one
   //  staging slot per group (segments ordered by tile). tileElem[t*16+s] = element (-1: padding of the
-  //  last tile); tgptr[t] = first group of tile t (groups ordered by upper block); gdst[g] = the slot of
+  //  last tile); tgptr[t] = first group of tile t (groups in gather order); gdst[g] = the slot of
   //  group g, or -(K slot + 1) for a direct group, whose mirror slot is gpt[g] and FULL mask gmsk[g].
   //  epos[t*576 + j] = group | rank << 10 | transposed << 15 of the tile's entry j = 36 s + sb (rank = its
   //  position in the group by increasing j; padding entries target spare accumulators past the groups),

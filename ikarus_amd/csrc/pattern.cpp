@@ -254,20 +254,20 @@ void buildPattern(ik_handle* h) {
         }
   });
   // upper blocks and their transposes
-  h->huptr.assign(nn + 1, 0);
-  for (int64_t i = 0; i < nn; ++i) h->huptr[i + 1] = h->huptr[i] + (h->hrowptr[i + 1] - h->hdiag[i]);
-  std::vector<int32_t> up(h->huptr[nn]), upT(h->huptr[nn]);
+  std::vector<int64_t> uptr(nn + 1, 0);  // first upper block of each row
+  for (int64_t i = 0; i < nn; ++i) uptr[i + 1] = uptr[i] + (h->hrowptr[i + 1] - h->hdiag[i]);
+  std::vector<int32_t> up(uptr[nn]), upT(uptr[nn]);
   parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
     for (int64_t i = i0; i < i1; ++i)
       for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
         const int32_t c = h->hcolidx[p];
         const auto b = h->hcolidx.begin() + h->hrowptr[c], en = h->hcolidx.begin() + h->hrowptr[c + 1];
-        const int64_t q = h->huptr[i] + (p - h->hdiag[i]);
+        const int64_t q = uptr[i] + (p - h->hdiag[i]);
         up[q]           = p;
         upT[q]          = (int32_t)(std::lower_bound(b, en, (int32_t)i) - h->hcolidx.begin());
       }
   });
-  const int64_t nup = h->huptr[nn];
+  const int64_t nup = uptr[nn];
   std::vector<int32_t> qOf(h->nnzb, -1);
   parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
     for (int64_t q = q0; q < q1; ++q) qOf[up[q]] = (int32_t)q;
@@ -285,41 +285,19 @@ void buildPattern(ik_handle* h) {
               m |= (uint16_t)(1u << (3 * k + l));
               if (3 * i + k == 3 * c + l) m |= (uint16_t)(1u << (9 + k));
             }
-        mask[h->huptr[i] + (p - h->hdiag[i])] = m;
+        mask[uptr[i] + (p - h->hdiag[i])] = m;
       }
   });
-  std::vector<int32_t> seg(nup + 1, 0);
   std::vector<uint8_t> gathered(nup, 1);  // upper blocks the gather assembles (the rest are written directly)
   std::vector<int64_t> unitOf(ne);  // staging unit of each element: its tile (tiled) or itself
-  int64_t nunits = ne;
   h->tiled = h->neas == 0;
-  if (!h->tiled) {
-    // EAS: one slot per element block, segments in upper-block order, elements in increasing order
-    std::vector<int32_t> kcnt(h->nnzb, 0);  // element blocks per K block
-    for (const int32_t p : slots) kcnt[p]++;
-    for (int64_t q = 0; q < nup; ++q) seg[q + 1] = seg[q] + kcnt[up[q]];
-    std::vector<int32_t> dest((size_t)ne * 36);
-    std::vector<int32_t> pos(seg.begin(), seg.end() - 1);
-    for (int64_t e = 0; e < ne; ++e) {
-      unitOf[e] = e;
-      for (int a = 0; a < 8; ++a)
-        for (int c = a; c < 8; ++c) {
-          const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
-          const int32_t q = qOf[slots[e * 36 + sb]];
-          const bool tr   = conn[e * 8 + a] > conn[e * 8 + c];
-          dest[e * 36 + sb] = (pos[q]++ << 1) | (tr ? 1 : 0);
-        }
-    }
-    for (int64_t q = 0; q < nup; ++q)
-      require(pos[q] == seg[q + 1], IK_EINVAL, "internal error: staging segments do not tile the slots");
-    h->bdest.upload(dest.data(), dest.size(), h->stream);
-  } else {
-    // plain element: tiles of kTileElems in Morton order; per tile, the entries j = 36 s + sb that fall on one
-    // upper block form a group (groups in order of their first entry, entries in increasing j)
-    // Tiles are consecutive runs of the Morton order (compact boxes), then sorted by their lowest node, so
-    // tiles running at the same time write neighbouring parts of the segments.
+  int64_t ntiles = 0;
+  std::vector<int32_t> tileElem;
+  if (h->tiled) {
+    // plain element: tiles of kTileElems, consecutive runs of the Morton order (compact boxes), then sorted by
+    // their lowest node, so tiles running at the same time write neighbouring parts of the segments.
     const std::vector<int32_t> order = mortonOrder(h);
-    const int64_t ntiles = (ne + kTileElems - 1) / kTileElems;
+    ntiles = (ne + kTileElems - 1) / kTileElems;
     std::vector<std::pair<int32_t, int32_t>> tkey(ntiles);  // (lowest node, Morton tile)
     parallelChunks(ntiles, [&](int64_t m0, int64_t m1, int64_t) {
       for (int64_t m = m0; m < m1; ++m) {
@@ -330,7 +308,7 @@ void buildPattern(ik_handle* h) {
       }
     });
     std::sort(tkey.begin(), tkey.end());
-    std::vector<int32_t> tileElem((size_t)ntiles * kTileElems, -1);
+    tileElem.assign((size_t)ntiles * kTileElems, -1);
     parallelChunks(ntiles, [&](int64_t t0, int64_t t1, int64_t) {
       for (int64_t t = t0; t < t1; ++t) {
         const int64_t m = tkey[t].second;
@@ -340,8 +318,94 @@ void buildPattern(ik_handle* h) {
         }
       }
     });
-    // per tile: groups (the entries j = 36 s + sb of one upper block) ordered by upper block, i.e. by staging
-    // slot; each entry's rank = its position in its group by increasing j (the order the tile sums them).
+  } else {
+    for (int64_t e = 0; e < ne; ++e) unitOf[e] = e;
+  }
+  const int64_t nunits = h->tiled ? ntiles : ne;
+  // Staging chunks (ranges of staging units: tiles, or elements for EAS): a block row is complete once the
+  // last unit touching it is staged. With the prefix maximum of that unit index over rows (monotone by
+  // construction), rows [0, r) are complete after all units below pm[r-1], so each chunk's gather covers a
+  // contiguous row range behind the chunk. Any numbering is correct.
+  {
+    std::vector<int64_t> last(nn, -1);
+    for (int64_t i = 0; i < nn; ++i) {
+      for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) last[i] = std::max(last[i], unitOf[inc[k]]);
+      if (i > 0) last[i] = std::max(last[i], last[i - 1]);
+    }
+    const int64_t chunk = h->tiled ? (chunkElements() + kTileElems - 1) / kTileElems : chunkElements();
+    h->chunkElem.assign(1, 0);
+    h->chunkRow.assign(1, 0);
+    int64_t r = 0;
+    for (int64_t u0 = 0; u0 < nunits; u0 += chunk) {
+      const int64_t u1 = std::min(nunits, u0 + chunk);
+      if (u1 == nunits)
+        r = nn;
+      else
+        while (r < nn && last[r] < u1) ++r;
+      h->chunkElem.push_back(u1);
+      h->chunkRow.push_back(r);
+    }
+  }
+  const int64_t nchunks = (int64_t)h->chunkElem.size() - 1;
+  // Gather order of the upper blocks: chunk by chunk (the rows each chunk completes), inside a chunk by storage
+  // slot. For a fixed j the rows of a slice have adjacent slots, so the gather's consecutive lanes, and a tile's
+  // consecutive groups, write neighbouring slots. Staging segments follow this order; qrank[q] = position of q.
+  std::vector<int32_t> qord, qrank(nup, -1);
+  std::vector<int64_t> qchunk(nchunks + 1, 0);  // qord[qchunk[c] .. qchunk[c+1]) = upper blocks of chunk c
+  {
+    std::vector<int32_t> qAtSlot(h->nsell, -1);
+    parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
+      for (int64_t q = q0; q < q1; ++q) qAtSlot[h->hpos[up[q]]] = (int32_t)q;
+    });
+    qord.reserve(nup);
+    for (int64_t c = 0; c < nchunks; ++c) {
+      const int64_t r0 = h->chunkRow[c], r1 = h->chunkRow[c + 1];
+      for (int64_t sl = r0 / kSellRows; r1 > r0 && sl <= (r1 - 1) / kSellRows; ++sl)
+        for (int32_t slot = sstart[sl]; slot < sstart[sl + 1]; ++slot) {
+          const int32_t q   = qAtSlot[slot];
+          const int64_t row = sl * kSellRows + (slot - sstart[sl]) % kSellRows;
+          if (q < 0 || row < r0 || row >= r1) continue;
+          qrank[q] = (int32_t)qord.size();
+          qord.push_back(q);
+        }
+      qchunk[c + 1] = (int64_t)qord.size();
+    }
+    require((int64_t)qord.size() == nup, IK_EINVAL, "internal error: gather order misses upper blocks");
+  }
+  // staging segment of upper block q: [segBeg[q], segBeg[q] + segCnt[q]), segments laid out in gather order
+  std::vector<int32_t> segBeg(nup, 0), segCnt(nup, 0);
+  auto layoutSegments = [&]() {
+    int64_t tot = 0;
+    for (const int32_t q : qord) {
+      segBeg[q] = (int32_t)tot;
+      tot += segCnt[q];
+      require(tot < (int64_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
+    }
+    return tot;
+  };
+  if (!h->tiled) {
+    // EAS: one slot per element block, elements in increasing order inside each segment
+    std::vector<int32_t> kcnt(h->nnzb, 0);  // element blocks per K block
+    for (const int32_t p : slots) kcnt[p]++;
+    for (int64_t q = 0; q < nup; ++q) segCnt[q] = kcnt[up[q]];
+    h->nstage = layoutSegments();
+    std::vector<int32_t> dest((size_t)ne * 36);
+    std::vector<int32_t> pos(segBeg);
+    for (int64_t e = 0; e < ne; ++e)
+      for (int a = 0; a < 8; ++a)
+        for (int c = a; c < 8; ++c) {
+          const int sb    = 8 * a - (a * (a - 1)) / 2 + (c - a);
+          const int32_t q = qOf[slots[e * 36 + sb]];
+          const bool tr   = conn[e * 8 + a] > conn[e * 8 + c];
+          dest[e * 36 + sb] = (pos[q]++ << 1) | (tr ? 1 : 0);
+        }
+    for (int64_t q = 0; q < nup; ++q)
+      require(pos[q] == segBeg[q] + segCnt[q], IK_EINVAL, "internal error: staging segments do not tile the slots");
+    h->bdest.upload(dest.data(), dest.size(), h->stream);
+  } else {
+    // per tile: the entries j = 36 s + sb that fall on one upper block form a group; groups ordered by the gather
+    // order of their upper blocks, i.e. by staging slot and by K slot; each entry's rank = its position in its
+    // group by increasing j (the order the tile sums them).
     // epos[t*576 + j] = group | rank << 10 | transposed << 15; padding entries target spare accumulators
     // ng + (j - 36 nvalid), rank 0, which the tile never stores. tmaxr[t] = the largest group size.
     // Each chunk of tiles lists its groups' upper blocks in its own buffer; tgptr places them.
@@ -374,7 +438,7 @@ void buildPattern(ik_handle* h) {
               }
             }
         }
-        std::sort(touched.begin(), touched.end());
+        std::sort(touched.begin(), touched.end(), [&](int32_t a, int32_t b) { return qrank[a] < qrank[b]; });
         const int ng = (int)touched.size();
         for (int g = 0; g < ng; ++g) groupOf[touched[g]] = g;
         std::fill(fill.begin(), fill.begin() + ng, 0);
@@ -413,15 +477,16 @@ void buildPattern(ik_handle* h) {
     }
     // An upper block that only one tile touches is complete after that tile's group sum, which the tile
     // writes straight into K (with the FULL mask and the mirror block). The others are staged, in gather
-    // order: segments in upper-block order, each ordered by tile. gdst[g] = slot, or -(K block + 1) with
+    // order: segments in the order qord, each ordered by tile. gdst[g] = slot, or -(K block + 1) with
     // gpt[g] = the mirror block and gmsk[g] = the mask of a direct group.
     std::vector<int32_t> ntile(nup, 0);
     for (const int32_t q : gq) ntile[q]++;
-    for (int64_t q = 0; q < nup; ++q) gathered[q] = ntile[q] > 1;
-    for (const int32_t q : gq)
-      if (gathered[q]) seg[q + 1]++;
-    for (int64_t q = 0; q < nup; ++q) seg[q + 1] += seg[q];
-    std::vector<int32_t> pos(seg.begin(), seg.end() - 1), gdst(gq.size()), gpt(gq.size(), 0);
+    for (int64_t q = 0; q < nup; ++q) {
+      gathered[q] = ntile[q] > 1;
+      segCnt[q]   = gathered[q] ? ntile[q] : 0;
+    }
+    h->nstage = layoutSegments();
+    std::vector<int32_t> pos(segBeg), gdst(gq.size()), gpt(gq.size(), 0);
     std::vector<uint16_t> gmsk(gq.size(), 0);
     for (size_t g = 0; g < gq.size(); ++g) {
       const int32_t q = gq[g];
@@ -439,9 +504,9 @@ void buildPattern(ik_handle* h) {
     // inside its own upper block's segment, and every upper block is either staged by two or more tiles or
     // written by exactly one
     {
-      std::vector<uint8_t> slotSeen(seg[nup], 0);
+      std::vector<uint8_t> slotSeen(h->nstage, 0);
       for (int64_t q = 0; q < nup; ++q)
-        require(ntile[q] >= 1 && (gathered[q] ? seg[q + 1] - seg[q] == ntile[q] : seg[q + 1] == seg[q]), IK_EINVAL,
+        require(ntile[q] >= 1 && segCnt[q] == (gathered[q] ? ntile[q] : 0), IK_EINVAL,
                 "internal error: upper block without a tile");
       for (size_t g = 0; g < gq.size(); ++g) {
         const int32_t sl = gdst[g];
@@ -449,7 +514,7 @@ void buildPattern(ik_handle* h) {
           require(!gathered[gq[g]] && -sl - 1 == up[gq[g]], IK_EINVAL, "internal error: bad direct group");
           continue;
         }
-        require(sl >= seg[gq[g]] && sl < seg[gq[g] + 1] && !slotSeen[sl], IK_EINVAL,
+        require(sl >= segBeg[gq[g]] && sl < segBeg[gq[g]] + segCnt[gq[g]] && !slotSeen[sl], IK_EINVAL,
                 "internal error: staging slots are not a permutation");
         slotSeen[sl] = 1;
       }
@@ -483,7 +548,6 @@ void buildPattern(ik_handle* h) {
       });
     }
     h->ntiles = ntiles;
-    nunits    = ntiles;
     // K blocks of the direct groups as storage slots
     for (size_t g = 0; g < gq.size(); ++g)
       if (gdst[g] < 0) {
@@ -498,24 +562,23 @@ void buildPattern(ik_handle* h) {
     h->epos.upload(epos.data(), epos.size(), h->stream);
     h->tmaxr.upload(tmaxr.data(), tmaxr.size(), h->stream);
   }
-  h->nstage = seg[nup];
-  // the gather's upper blocks: the gathered ones in row order (huptr: first one of each row)
+  // the gather's upper blocks: the gathered ones in gather order (chunkQ[c]: first one of chunk c)
   {
     std::vector<int32_t> cup, cupT, cseg(1, 0);
     std::vector<uint16_t> cmask;
-    std::vector<int64_t> chuptr(nn + 1, 0);
-    for (int64_t i = 0; i < nn; ++i) {
-      for (int64_t q = h->huptr[i]; q < h->huptr[i + 1]; ++q)
-        if (gathered[q]) {
-          cup.push_back(h->hpos[up[q]]);
-          cupT.push_back(h->hpos[upT[q]]);
-          cmask.push_back(mask[q]);
-          cseg.push_back(seg[q + 1]);
-          require(seg[q] == cseg[cseg.size() - 2], IK_EINVAL, "internal error: gather segments not contiguous");
-        }
-      chuptr[i + 1] = (int64_t)cup.size();
+    h->chunkQ.assign(1, 0);
+    for (int64_t c = 0; c < nchunks; ++c) {
+      for (int64_t k = qchunk[c]; k < qchunk[c + 1]; ++k) {
+        const int32_t q = qord[k];
+        if (!gathered[q]) continue;
+        cup.push_back(h->hpos[up[q]]);
+        cupT.push_back(h->hpos[upT[q]]);
+        cmask.push_back(mask[q]);
+        cseg.push_back(segBeg[q] + segCnt[q]);
+        require(segBeg[q] == cseg[cseg.size() - 2], IK_EINVAL, "internal error: gather segments not contiguous");
+      }
+      h->chunkQ.push_back((int64_t)cup.size());
     }
-    h->huptr.swap(chuptr);
     h->upMask.upload(cmask.data(), cmask.size(), h->stream);
     h->upBlk.upload(cup.data(), cup.size(), h->stream);
     h->upT.upload(cupT.data(), cupT.size(), h->stream);
@@ -528,30 +591,6 @@ void buildPattern(ik_handle* h) {
     std::vector<int32_t> pos(ncnt.begin(), ncnt.end() - 1);
     for (int64_t e = 0; e < ne; ++e)
       for (int a = 0; a < 8; ++a) nlist[pos[conn[e * 8 + a]]++] = (int32_t)(e * 8 + a);
-  }
-  // Staging chunks (ranges of staging units: tiles, or elements for EAS): a block row is complete once the
-  // last unit touching it is staged. With the prefix maximum of that unit index over rows (monotone by
-  // construction), rows [0, r) are complete after all units below pm[r-1], so each chunk's gather covers a
-  // contiguous row range behind the chunk. Any numbering is correct.
-  std::vector<int64_t> last(nn, -1);
-  for (int64_t i = 0; i < nn; ++i) {
-    for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) last[i] = std::max(last[i], unitOf[inc[k]]);
-    if (i > 0) last[i] = std::max(last[i], last[i - 1]);
-  }
-  const int64_t chunk = h->tiled ? (chunkElements() + kTileElems - 1) / kTileElems : chunkElements();
-  h->chunkElem.clear();
-  h->chunkRow.clear();
-  h->chunkElem.push_back(0);
-  h->chunkRow.push_back(0);
-  int64_t r = 0;
-  for (int64_t u0 = 0; u0 < nunits; u0 += chunk) {
-    const int64_t u1 = std::min(nunits, u0 + chunk);
-    if (u1 == nunits)
-      r = nn;
-    else
-      while (r < nn && last[r] < u1) ++r;
-    h->chunkElem.push_back(u1);
-    h->chunkRow.push_back(r);
   }
   h->rowptr.upload(h->hrowptr.data(), h->hrowptr.size(), h->stream);
   h->colidx.upload(h->hcolidx.data(), h->hcolidx.size(), h->stream);

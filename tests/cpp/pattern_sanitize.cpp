@@ -117,7 +117,7 @@ std::string buildOnce(const Mesh& m, int eas, int threads) {
   put(out, h.hrowptr);
   put(out, h.hcolidx);
   put(out, h.hdiag);
-  put(out, h.huptr);
+  put(out, h.chunkQ);
   put(out, h.chunkElem);
   put(out, h.chunkRow);
   put(out, h.rowptr);

@@ -357,3 +357,48 @@ def test_tangent_is_derivative_of_residual(material):
     for m in modes:
         assert np.linalg.norm(K0 @ m) <= 1e-12 * np.abs(K0).max() * np.linalg.norm(m)
     dev.close()
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32])
+def test_block_view_matches_oracle(precision):
+    """ik_matrix_bsr's device view (block CSR pattern, each block at values + 9 block_pos[b]) rebuilds the FULL
+    matrix of the oracle, on the scrambled mesh (sliced-ELL slices of uneven width) and a structured block."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+
+    def d2h(ptr, dtype, n):
+        a = np.zeros(n, dtype=dtype)
+        if n:
+            assert hip.hipMemcpy(a.ctypes.data_as(C.c_void_p), ptr, a.nbytes, 2) == 0
+        return a
+
+    for p in (_scrambled_block((9, 8, 7), seed=23), pb.hex_block((6, 5, 4), seed=2)):
+        print(f"case: block view, {p.ne} elements, precision {precision}", flush=True)
+        o = _oracle_for(p)
+        u = 0.01 * np.random.default_rng(8).standard_normal(p.ndof) * (1 - p.fixed)
+        dev = ia.Device(p)
+        dev.set_requirement(u, 0.7)
+        dev.check(capi.lib().ik_matrix(dev.h, capi.DBC_FULL, precision))
+        nbr, nnzb = C.c_int64(), C.c_int64()
+        rp, ci, bp, vals = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        prec = C.c_int()
+        dev.check(capi.lib().ik_matrix_bsr(dev.h, C.byref(nbr), C.byref(nnzb), C.byref(rp), C.byref(ci), C.byref(bp),
+                                           C.byref(vals), C.byref(prec)))
+        assert nbr.value == p.X.shape[0] and prec.value == precision
+        rowptr = d2h(rp, np.int32, nbr.value + 1)
+        colidx = d2h(ci, np.int32, nnzb.value)
+        pos = d2h(bp, np.int32, nnzb.value)
+        assert rowptr[-1] == nnzb.value and len(np.unique(pos)) == nnzb.value and pos.min() >= 0
+        dt = np.float64 if precision == capi.FP64 else np.float32
+        V = d2h(vals, dt, 9 * (int(pos.max()) + 1)).reshape(-1, 3, 3)[pos].astype(np.float64)
+        n = 3 * nbr.value
+        K = np.zeros((n, n))
+        for i in range(nbr.value):
+            for b in range(rowptr[i], rowptr[i + 1]):
+                j = colidx[b]
+                K[3 * i:3 * i + 3, 3 * j:3 * j + 3] = V[b]
+        Ko = o.matrix_dense(u, 0.7, po.FULL)
+        tol = 1e-10 if precision == capi.FP64 else 1e-6
+        assert np.abs(K - Ko).max() <= tol * np.abs(Ko).max()
+        dev.close()
