@@ -15,7 +15,7 @@ DBC_RAW, DBC_FULL, DBC_REDUCED = 0, 1, 2
 NEOHOOKE, SVK = 0, 1
 FP64, FP32 = 0, 1
 HOST, DEVICE = 0, 1
-PCG_JACOBI_FP64, PCG_BJACOBI_FP32_IR = 0, 1
+PCG_JACOBI_FP64, PCG_BJACOBI_FP32_IR, PCG_TWOLEVEL_FP32_IR = 0, 1, 2
 PRED_CONSTANT, PRED_LINEAR, PRED_MLP = 0, 1, 2
 PK2, KIRCHHOFF, CAUCHY = 0, 1, 2
 VTK_POINT_DATA, VTK_CELL_DATA = 0, 1

@@ -8,7 +8,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libikarus_amd.so")
-SOURCES = ["pattern.cpp", "assemble.hip", "sparse.hip", "eas.hip", "mlp.hip", "capi.cpp", "vtk.cpp"]
+SOURCES = ["pattern.cpp", "assemble.hip", "sparse.hip", "coarse.hip", "eas.hip", "mlp.hip", "capi.cpp", "vtk.cpp"]
+# dense Cholesky factorisation and inverse of the two-level preconditioner's coarse operator (rocSOLVER)
+LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -33,7 +35,8 @@ def build(verbose=False):
         if verbose and msg.strip():
             print(msg)
     tmp = LIB + ".tmp"
-    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results])
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results] +
+                          LIBS)
     os.replace(tmp, LIB)
     return LIB
 

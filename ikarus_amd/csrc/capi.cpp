@@ -236,7 +236,7 @@ void linearSolve(ik_handle* h, const double* rhs, double* x, const ik_solver_opt
     info->refinements  = 0;
     info->rel_residual = rel;
     info->converged    = rel <= rtol ? 1 : 0;
-  } else if (o->kind == IK_PCG_BJACOBI_FP32_IR) {
+  } else if (o->kind == IK_PCG_BJACOBI_FP32_IR || o->kind == IK_PCG_TWOLEVEL_FP32_IR) {
     require(h->kPrecision == IK_FP32, IK_EINVAL, "mixed PCG needs an FP32 tangent");
     ik_solver_opts oo = *o;
     oo.rtol           = rtol;
@@ -545,6 +545,7 @@ void ik_destroy(ik_handle* h) {
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream2) (void)hipStreamSynchronize(h->stream2);
   ik::mlpDestroy(h->mlp);
+  ik::coarseDestroy(h->coarse);
   for (auto ev : h->chunkEvents) (void)hipEventDestroy(ev);
   if (h->joinEvent) (void)hipEventDestroy(h->joinEvent);
   hipStream_t s = h->stream, s2 = h->stream2;

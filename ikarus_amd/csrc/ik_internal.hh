@@ -12,6 +12,7 @@
 namespace ik {
 
 struct Mlp;
+struct Coarse;
 
 struct Error {
   int code;
@@ -209,6 +210,9 @@ struct ik_handle {
   double* hred = nullptr;         // pinned host scalar for reductions
   unsigned long long* herr = nullptr;  // pinned host copy of the determinant-failure flag
 
+  // two-level preconditioner of the FP32 inner PCG (coarse.hip)
+  ik::Coarse* coarse = nullptr;
+
   // learned predictor
   ik::Mlp* mlp = nullptr;
   bool mlpLoaded = false;
@@ -245,6 +249,13 @@ double energy(ik_handle* h);  // potential energy at the bound requirement (plai
 void stressAt(ik_handle* h, int kind, const double xi[3], double* out);  // out: device, ne x 6
 void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxIter, int* iters, double* relres);
 void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info);
+// coarse.hip
+void coarseDestroy(Coarse* c);
+void coarsePrepare(ik_handle* h);
+void coarseRecord(ik_handle* h, int iterations);
+void coarseApply(ik_handle* h, const float* r, float* z, float* p, const float* binv, int gn, double* partial);
+float* coarseZBuffer(ik_handle* h);
+int64_t coarseDofs(const ik_handle* h);
 void residualBlock64(ik_handle* h, const double* b, const double* x, double* r);  // r = b - K x (FP64 K)
 // mlp.hip
 Mlp* mlpCreate(hipStream_t s);

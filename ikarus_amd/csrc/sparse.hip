@@ -334,7 +334,14 @@ __global__ __launch_bounds__(256) void pcgUpdateKernel(T* __restrict__ x, T* __r
   }
   const T alpha = (T)alphaS;
   double rr = 0.0, rz = 0.0;
-  if (!blockPre) {
+  if (!pre) {  // two-level form: r.z follows from the coarse correction (coarseZ)
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+      x[i] += alpha * p[i];
+      const T ri = r[i] - alpha * q[i];
+      r[i]       = ri;
+      rr += (double)ri * (double)ri;
+    }
+  } else if (!blockPre) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
       x[i] += alpha * p[i];
       const T ri = r[i] - alpha * q[i];
@@ -362,19 +369,20 @@ __global__ __launch_bounds__(256) void pcgUpdateKernel(T* __restrict__ x, T* __r
   rr = blockSum<256>(rr);
   rz = blockSum<256>(rz);
   if (threadIdx.x == 0) {
-    partial[blockIdx.x]             = rr;
-    partial[gridDim.x + blockIdx.x] = rz;
+    partial[blockIdx.x] = rr;
+    if (pre) partial[gridDim.x + blockIdx.x] = rz;
   }
 }
 
-// Convergence test, beta and p = M^-1 r + beta p in one launch. Every block folds the update kernel's
+// Convergence test, beta and p = M^-1 r + beta p in one launch (z = M^-1 r read from zin when given). Every block folds the update kernel's
 // partials (rr, rz) itself in the same fixed order, so all blocks agree bitwise; block 0 records the
 // scalars. beta = rz_new / rz uses the r.z of the previous iterate, which only pcgAlphaFold writes.
 template <typename T>
 __global__ __launch_bounds__(256) void pcgBetaPKernel(T* __restrict__ p, const T* __restrict__ r,
                                                       const T* __restrict__ pre, int blockPre, int64_t n,
                                                       const double* __restrict__ partial, int nb,
-                                                      PcgScalars* __restrict__ sc, int maxIter) {
+                                                      PcgScalars* __restrict__ sc, int maxIter,
+                                                      const T* __restrict__ zin) {
   if (sc->done) return;
   __shared__ double bc[2];
   double rr = 0.0, rz = 0.0;
@@ -406,7 +414,10 @@ __global__ __launch_bounds__(256) void pcgBetaPKernel(T* __restrict__ p, const T
   }
   if (conv) return;
   const T beta = (T)bt;
-  if (!blockPre) {
+  if (zin) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+      p[i] = zin[i] + beta * p[i];
+  } else if (!blockPre) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
       p[i] = pre[i] * r[i] + beta * p[i];
   } else {
@@ -476,6 +487,23 @@ __global__ __launch_bounds__(256) void pcgInitF32(const double* __restrict__ b, 
     partial[blockIdx.x]             = rr;
     partial[gridDim.x + blockIdx.x] = rz;
   }
+}
+
+// two-level form: r = b / s, x = 0 and r.r partials (z and p follow from the coarse correction); clears the done
+// flag, which the coarse kernels read before the init fold sets it
+__global__ __launch_bounds__(256) void pcgInitR32(const double* __restrict__ b, double invScale,
+                                                  float* __restrict__ x, float* __restrict__ r, int64_t n,
+                                                  double* __restrict__ partial, PcgScalars* __restrict__ sc) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 0;
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const float ri = (float)(b[i] * invScale);
+    r[i]           = ri;
+    x[i]           = 0.0f;
+    rr += (double)ri * (double)ri;
+  }
+  rr = blockSum<256>(rr);
+  if (threadIdx.x == 0) partial[blockIdx.x] = rr;
 }
 
 // x64 += scale * x32
@@ -599,7 +627,7 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
       hipLaunchKernelGGL((pcgUpdateKernel<double>), dim3(g), dim3(256), 0, h->stream, x, h->pr.p, h->pp.p, h->pq.p,
                          h->pdiag.p, 0, n, h->sc.p, pqSrc, npq, h->partial.p);
       hipLaunchKernelGGL((pcgBetaPKernel<double>), dim3(g), dim3(256), 0, h->stream, h->pp.p, h->pr.p, h->pdiag.p, 0,
-                         n, h->partial.p, g, h->sc.p, maxIter);
+                         n, h->partial.p, g, h->sc.p, maxIter, (const double*)nullptr);
     }
     IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
     IK_HIP(hipStreamSynchronize(h->stream));
@@ -611,6 +639,7 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
 
 void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info) {
   ensureWork(h);
+  const bool twoLevel = o->kind == IK_PCG_TWOLEVEL_FP32_IR;
   const int64_t n = h->ndof;
   h->fr.ensure(n);
   h->fp.ensure(n);
@@ -625,6 +654,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
   const unsigned sgrid = spmvGrid<float>(h->nn);
   hipLaunchKernelGGL(blockInvKernel, dim3((unsigned)((h->nn + 255) / 256)), dim3(256), 0, h->stream, h->diagBlk.p,
                      h->K32.p, h->nn, h->fdiag.p);
+  if (twoLevel) coarsePrepare(h);
   // x = 0, r = b
   launchZero(x, n, h->stream);
   IK_HIP(hipMemcpyAsync(h->pr.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
@@ -644,8 +674,14 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
     // inner FP32 PCG on K32 d = r / ||r||, to half the reduction the outer tolerance still needs (an inexact
     // Newton step asks for little), but never below inner_rtol
     const double innerTol = std::max(o->inner_rtol, 0.5 * o->rtol * bnorm / rnorm);
-    hipLaunchKernelGGL(pcgInitF32, dim3(gn), dim3(256), 0, h->stream, h->pr.p, 1.0 / rnorm, h->fdiag.p, h->fx.p,
-                       h->fr.p, h->fp.p, h->nn, h->partial.p);
+    if (twoLevel) {
+      hipLaunchKernelGGL(pcgInitR32, dim3(gn), dim3(256), 0, h->stream, h->pr.p, 1.0 / rnorm, h->fx.p, h->fr.p, n,
+                         h->partial.p, h->sc.p);
+      coarseApply(h, h->fr.p, coarseZBuffer(h), h->fp.p, h->fdiag.p, gn, h->partial.p);
+    } else {
+      hipLaunchKernelGGL(pcgInitF32, dim3(gn), dim3(256), 0, h->stream, h->pr.p, 1.0 / rnorm, h->fdiag.p, h->fx.p,
+                         h->fr.p, h->fp.p, h->nn, h->partial.p);
+    }
     hipLaunchKernelGGL(pcgInitFold, dim3(1), dim3(1024), 0, h->stream, h->partial.p, gn, innerTol * innerTol,
                        h->sc.p);
     int enq = 0;
@@ -666,9 +702,11 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
           npq   = kPqBlocks;
         }
         hipLaunchKernelGGL((pcgUpdateKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fx.p, h->fr.p, h->fp.p,
-                           h->fq.p, h->fdiag.p, 1, n, h->sc.p, pqSrc, npq, h->partial.p);
+                           h->fq.p, twoLevel ? nullptr : h->fdiag.p, 1, n, h->sc.p, pqSrc, npq, h->partial.p);
+        if (twoLevel) coarseApply(h, h->fr.p, coarseZBuffer(h), nullptr, h->fdiag.p, gn, h->partial.p);
         hipLaunchKernelGGL((pcgBetaPKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fp.p, h->fr.p, h->fdiag.p,
-                           1, n, h->partial.p, gn, h->sc.p, maxIter);
+                           1, n, h->partial.p, gn, h->sc.p, maxIter,
+                           twoLevel ? (const float*)coarseZBuffer(h) : (const float*)nullptr);
       }
       IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
       IK_HIP(hipStreamSynchronize(h->stream));
@@ -692,6 +730,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
       break;
     }
   }
+  if (twoLevel) coarseRecord(h, info->iterations);
 }
 
 }  // namespace ik

@@ -40,8 +40,9 @@ enum { IK_NEOHOOKE = 0, IK_SVK = 1 };
 enum { IK_FP64 = 0, IK_FP32 = 1 };
 /* where a pointer lives */
 enum { IK_HOST = 0, IK_DEVICE = 1 };
-/* linear solvers (si_ConjugateGradient, linearsolver.cpp:23-24, and the mixed-precision form) */
-enum { IK_PCG_JACOBI_FP64 = 0, IK_PCG_BJACOBI_FP32_IR = 1 };
+/* linear solvers (si_ConjugateGradient, linearsolver.cpp:23-24, and the mixed-precision forms; the two-level
+ * preconditioner has no counterpart in the reference) */
+enum { IK_PCG_JACOBI_FP64 = 0, IK_PCG_BJACOBI_FP32_IR = 1, IK_PCG_TWOLEVEL_FP32_IR = 2 };
 /* load-step predictors: constant (loadcontrol.inl:55-57), secant/linear extrapolation
  * (pathfollowingfunctions.hh:188-195 semantics), learned per-node MLP (no reference counterpart) */
 enum { IK_PRED_CONSTANT = 0, IK_PRED_LINEAR = 1, IK_PRED_MLP = 2 };
@@ -176,7 +177,7 @@ int ik_write_vtu(ik_handle* h, const char* path, const ik_vtk_field* fields, int
 
 /* ---- linear solve (LinearSolverTemplate::analyzePattern/factorize/solve, linearsolver.hh:50-235) -- */
 typedef struct {
-  int32_t kind;        /* IK_PCG_JACOBI_FP64 | IK_PCG_BJACOBI_FP32_IR                       */
+  int32_t kind;        /* IK_PCG_JACOBI_FP64 | IK_PCG_BJACOBI_FP32_IR | IK_PCG_TWOLEVEL_FP32_IR  */
   double rtol;         /* ||r|| <= rtol ||b||   (Eigen ConjugateGradient semantics)         */
   int32_t max_iter;    /* <= 0: 2n (Eigen default)                                         */
   double inner_rtol;   /* IR form: an FP32 sweep stops at half the relative reduction the
@@ -191,10 +192,16 @@ typedef struct {
   int32_t converged;
 } ik_solver_info;
 
-/* Solve K x = rhs with the last assembled (FULL) matrix. With IK_PCG_BJACOBI_FP32_IR the FP64 refinement
- * residual b - K x uses the FP32 matrix, so the solve converges to the solution of the FP32-rounded system and
- * rel_residual / converged refer to it; against the FP64 tangent the residual has a floor near
- * eps32 ||K|| ||x||. (Newton is not affected: its residual is assembled in FP64.) */
+/* Solve K x = rhs with the last assembled (FULL) matrix. With the IR forms the FP64 refinement residual b - K x
+ * uses the FP32 matrix, so the solve converges to the solution of the FP32-rounded system and rel_residual /
+ * converged refer to it; against the FP64 tangent the residual has a floor near eps32 ||K|| ||x||. (Newton is
+ * not affected: its residual is assembled in FP64.)
+ * IK_PCG_TWOLEVEL_FP32_IR preconditions the FP32 PCG with 3x3 block Jacobi plus an additive coarse correction
+ * z += P (P^T K P)^-1 P^T r over box aggregates of nodes (about 512, each spanned by its six rigid-body modes).
+ * The coarse inverse is built from the FP32 tangent of the first such solve on the handle and rebuilt when a
+ * solve needs more than twice (+10) the inner iterations of the first solve after the last build, so later
+ * solves reuse it while it still preconditions well. IK_ESINGULAR if the coarse operator is not positive
+ * definite (an indefinite tangent). */
 int ik_solve(ik_handle* h, const double* rhs, double* x, int where, const ik_solver_opts* opts,
              ik_solver_info* info);
 
