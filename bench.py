@@ -16,9 +16,10 @@ as clock_settle), then --warmup steps.
 Also reported (same run): the element kernel's roofline figures (HIP events on the library stream; HBM
 traffic per launch from the committed rocprofv3 counter summary) and the whole step's FP64 fraction, the gather and
 SpMV bandwidths, config c2's 5-step FP64 load path, the Newton comparison (40 load steps to ||R|| <= 1e-8: the
-plain FP64 form on the first evaluation seed, the mixed form with the constant, secant and learned predictors on
-8 evaluation seeds never used for training or tuning) and the CPU baseline (the oracle restatement, faithful and
-optimized, on bounded samples on the host cores, plus config c1 in full).
+plain FP64 form on the first evaluation seed, the mixed form with the constant, secant and learned predictors, with
+the block-Jacobi and with the two-level preconditioner, on 8 evaluation seeds never used for training or tuning) and
+the CPU baseline (the oracle restatement, faithful and optimized, on bounded samples on the host cores, plus config
+c1 in full).
 --tune-eta chooses the forcing-term cap of each mixed predictor on training seeds.
 """
 import argparse
@@ -158,6 +159,14 @@ NEWTON_MODES = {
                          pred=capi.PRED_LINEAR, eta_max=0.1),
     "mixed_mlp": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
                       pred=capi.PRED_MLP, eta_max=0.01),
+    # the mixed form with the two-level preconditioner (block Jacobi + rigid-body-mode coarse correction over
+    # box aggregates) in the FP32 PCG
+    "tl_constant": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
+                        pred=capi.PRED_CONSTANT, eta_max=0.03),
+    "tl_secant": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
+                      pred=capi.PRED_LINEAR, eta_max=0.1),
+    "tl_mlp": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
+                   pred=capi.PRED_MLP, eta_max=0.01),
     # precision alone: the plain form's Newton and predictor with the FP32 tangent and refined FP32 PCG
     "mixed_exact": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.0,
                         pred=capi.PRED_CONSTANT),
@@ -195,7 +204,20 @@ def newton_path(p, mode, steps, mlp_params=None, eta_max=None):
                 fallbacks=int(info.fallbacks)), u
 
 
-MIXED_MODES = ("mixed_constant", "mixed_secant", "mixed_mlp")
+MIXED_MODES = ("mixed_constant", "mixed_secant", "mixed_mlp", "tl_constant", "tl_secant", "tl_mlp")
+
+
+def warm_libraries():
+    """One tiny two-level solve, untimed: rocBLAS / rocSOLVER load their kernels on the first call in a process
+    (about 0.3 s), which no later Newton path pays."""
+    p = pb.hex_block((4, 4, 4), seed=0)
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    dev.matrix(capi.DBC_FULL, capi.FP32)
+    t0 = time.perf_counter()
+    dev.solve(np.ones(p.ndof) * (1 - p.fixed), ia.solver_opts(capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-6))
+    dev.close()
+    return time.perf_counter() - t0
 
 
 def tune_eta(n, steps, mlp):
@@ -220,6 +242,8 @@ def evaluate_newton(n, steps, mlp, seeds):
     """The c5 comparison: the plain FP64 form and FP64 inexact + secant on the first evaluation seed, then the mixed
     form with the constant, secant and learned predictors on every evaluation seed; per-seed and median figures."""
     first = pb.hex_block((n, n, n), seed=seeds[0])
+    warm_s = warm_libraries()
+    log(f"[bench] rocBLAS / rocSOLVER first-call load (untimed warm-up): {warm_s:.3f} s")
     plain, us = {}, {}
     for mode in ("fp64", "fp64_inexact_secant"):
         log(f"[bench] newton {mode} seed {seeds[0]} ...")
@@ -242,8 +266,10 @@ def evaluate_newton(n, steps, mlp, seeds):
     med = {m: {"seconds": float(np.median([per_seed[str(s)][m]["seconds"] for s in seeds])),
                "newton_iterations": float(np.median([per_seed[str(s)][m]["newton_iterations"] for s in seeds]))}
            for m in MIXED_MODES}
-    ratio = {f"{a}_over_mlp": float(np.median([per_seed[str(s)][a]["seconds"] / per_seed[str(s)]["mixed_mlp"]["seconds"]
-                                              for s in seeds])) for a in ("mixed_constant", "mixed_secant")}
+    ratio = {f"{a}_over_{b}": float(np.median([per_seed[str(s)][a]["seconds"] / per_seed[str(s)][b]["seconds"]
+                                              for s in seeds]))
+             for a, b in (("mixed_constant", "mixed_mlp"), ("mixed_secant", "mixed_mlp"), ("tl_constant", "tl_mlp"),
+                          ("tl_secant", "tl_mlp"), ("mixed_mlp", "tl_mlp"), ("mixed_secant", "tl_secant"))}
     first_row = per_seed[str(seeds[0])]
     speed = {m: round(plain["fp64"]["seconds"] / first_row[m]["seconds"], 3) for m in MIXED_MODES}
     speed["fp64_inexact_secant"] = round(plain["fp64"]["seconds"] / plain["fp64_inexact_secant"]["seconds"], 3)
@@ -253,9 +279,11 @@ def evaluate_newton(n, steps, mlp, seeds):
                     "includes the one-time pattern build of the first matrix call",
         "eta_max": {m: NEWTON_MODES[m]["eta_max"] for m in MIXED_MODES},
         "plain_fp64_seed": seeds[0], "plain": plain,
-        "per_seed": per_seed, "median": med, "median_time_ratio_vs_mlp": ratio,
+        "per_seed": per_seed, "median": med, "median_time_ratios": ratio,
         "speedup_vs_fp64_on_first_seed": speed,
-        "headline": f"mixed_mlp vs fp64 on seed {seeds[0]}; medians over {len(seeds)} seeds for the mixed forms",
+        "library_warmup_s": round(warm_s, 3),
+        "fastest_median": min(med, key=lambda m: med[m]["seconds"]),
+        "headline": f"mixed forms vs fp64 on seed {seeds[0]}; medians over {len(seeds)} seeds for the mixed forms",
     }
 
 
@@ -351,6 +379,7 @@ def main():
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=40, help="load steps of the Newton comparison")
     ap.add_argument("--eval-seeds", type=int, default=len(EVAL_SEEDS), help="evaluation seeds of the c5 comparison")
+    ap.add_argument("--profile-mode", default="mixed_mlp", help="Newton mode of --profile-newton")
     ap.add_argument("--profile-newton", action="store_true",
                     help="only one mixed + learned load path (the command profiled for the MLP and PCG kernels)")
     ap.add_argument("--tune-eta", action="store_true",
@@ -384,7 +413,7 @@ def main():
         # one mixed + learned load path on the first evaluation seed (the command profiled for the learned and PCG
         # kernels)
         mlp = np.fromfile(args.mlp, dtype=np.float32)
-        r, _ = newton_path(pb.hex_block((args.n,) * 3, seed=EVAL_SEEDS[0]), "mixed_mlp", args.newton_steps, mlp)
+        r, _ = newton_path(pb.hex_block((args.n,) * 3, seed=EVAL_SEEDS[0]), args.profile_mode, args.newton_steps, mlp)
         print(json.dumps({"profile_newton": r}), flush=True)
         return
 

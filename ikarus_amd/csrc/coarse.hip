@@ -13,8 +13,8 @@
 // below 1e-10 of the largest, e.g. rotations of a collinear aggregate, are dropped), so P_i = B(d_i) T_A and
 // P^T K P is positive definite for positive definite K.
 //
-// Set-up (first two-level solve, again when a solve needs more than twice the iterations of the first solve after
-// the last set-up, plus 10): A_raw = sum over K blocks B_i^T K_ij B_j per aggregate pair on the device (FP64,
+// Set-up (first two-level solve, again when a solve needs more than twice the inner iterations per decade of
+// residual reduction of the first solve after the last set-up, plus 5): A_raw = sum over K blocks B_i^T K_ij B_j per aggregate pair on the device (FP64,
 // fixed summation order), A = T^T A_raw T, Cholesky factorisation and inverse (rocSOLVER dpotrf / dpotri), and
 // G = T A^-1 T^T stored dense in FP32. Per PCG iteration: s = B^T r per aggregate (one workgroup per aggregate),
 // w = G s (one wavefront per row), z_i = D_i^-1 r_i + B(d_i) w_A (fused with the r.z partials).
@@ -32,15 +32,16 @@ struct Coarse {
   bool ready = false;       // aggregates and T built
   bool valid = false;       // G holds the inverse of a current-enough coarse operator
   int64_t na = 0, nc = 0;   // aggregates, coarse DOFs (6 per aggregate)
-  int itersAtBuild = -1;    // inner iterations of the first solve after the last set-up
+  int64_t ld = 0;           // row stride of G and length of s: nc rounded up to 64 (padding zero)
+  double rateAtBuild = -1;  // inner iterations per decade of the first solve after the last set-up
   int builds = 0;
   DevBuf<int32_t> agg, aggPtr, aggNodes, nbrPtr, nbr;
   DevBuf<float> d;          // [nn*3] node offsets from the aggregate centroid
   DevBuf<double> T;         // [na*36] row-major 6x6 per aggregate (columns of dropped modes zero)
   DevBuf<uint8_t> drop;     // [nc] dropped modes
   DevBuf<double> A;         // [nc*nc] coarse operator, then its inverse (column-major lower triangle)
-  DevBuf<float> G;          // [nc*nc] T A^-1 T^T, row-major
-  DevBuf<float> s, w;       // [nc]
+  DevBuf<float> G;          // [nc*ld] T A^-1 T^T, row-major
+  DevBuf<float> s, w;       // [ld], [nc]
   DevBuf<float> z;          // [ndof]
   DevBuf<int32_t> info;
   rocblas_handle blas = nullptr;
@@ -171,9 +172,10 @@ __global__ void coarseCongruence(const double* __restrict__ Araw, const double* 
   A[idx] = v;  // symmetric: the row-major and column-major views agree
 }
 
-// G = T A^-1 T^T in FP32 from the inverse's lower triangle (column-major, entry (i, j) for i >= j at j nc + i)
+// G = T A^-1 T^T in FP32 (row stride ld) from the inverse's lower triangle (column-major, entry (i, j) for i >= j
+// at j nc + i)
 __global__ void coarseInverseOut(const double* __restrict__ Ainv, const double* __restrict__ T, int64_t na,
-                                 float* __restrict__ G) {
+                                 int64_t ld, float* __restrict__ G) {
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t nc  = 6 * na;
   if (idx >= nc * nc) return;
@@ -190,7 +192,7 @@ __global__ void coarseInverseOut(const double* __restrict__ Ainv, const double* 
     }
     v += T[a * 36 + m * 6 + p] * t;
   }
-  G[idx] = (float)v;
+  G[r * ld + c] = (float)v;
 }
 
 // s_a = sum over the nodes of aggregate a of B(d_i)^T r_i (constrained components zeroed): one workgroup per
@@ -226,17 +228,20 @@ __global__ __launch_bounds__(256) void coarseRestrict(const float* __restrict__ 
   }
 }
 
-// w = G s: one wavefront per row, 4 rows per workgroup
-__global__ __launch_bounds__(256) void coarseGemv(const float* __restrict__ G, const float* __restrict__ s,
-                                                  float* __restrict__ w, int64_t nc,
+// w = G s: one wavefront per row, 4 rows per workgroup, 16-byte loads (rows padded to ld = a multiple of 64 floats)
+__global__ __launch_bounds__(256) void coarseGemv(const float4* __restrict__ G, const float4* __restrict__ s,
+                                                  float* __restrict__ w, int64_t nc, int64_t ld4,
                                                   const PcgScalars* __restrict__ sc) {
   if (sc->done) return;
   const int lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= nc) return;
-  const float* g = G + row * nc;
+  const float4* g = G + row * ld4;
   float acc = 0.0f;
-  for (int64_t c = lane; c < nc; c += 64) acc += g[c] * s[c];
+  for (int64_t c = lane; c < ld4; c += 64) {
+    const float4 a = g[c], b = s[c];
+    acc += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+  }
   acc = waveSumF(acc);
   if (lane == 0) w[row] = acc;
 }
@@ -437,6 +442,7 @@ void buildAggregates(ik_handle* h, Coarse& cs) {
   }
   cs.na = na;
   cs.nc = 6 * (int64_t)na;
+  cs.ld = (cs.nc + 63) / 64 * 64;
   cs.agg.upload(agg.data(), agg.size(), h->stream);
   cs.aggPtr.upload(ptr.data(), ptr.size(), h->stream);
   cs.aggNodes.upload(nodes.data(), nodes.size(), h->stream);
@@ -445,7 +451,8 @@ void buildAggregates(ik_handle* h, Coarse& cs) {
   cs.d.upload(dd.data(), dd.size(), h->stream);
   cs.T.upload(T.data(), T.size(), h->stream);
   cs.drop.upload(drop.data(), drop.size(), h->stream);
-  cs.s.ensure(cs.nc);
+  cs.s.ensure(cs.ld);
+  launchZeroF(cs.s.p, cs.ld, h->stream);
   cs.w.ensure(cs.nc);
   cs.z.ensure(h->ndof);
   cs.info.ensure(2);
@@ -458,8 +465,9 @@ void coarseDestroy(Coarse* c) { delete c; }
 
 int64_t coarseDofs(const ik_handle* h) { return h->coarse ? h->coarse->nc : 0; }
 
-// (Re)builds the coarse inverse from the current FP32 tangent when there is none yet or when a solve needed
-// more than twice the iterations of the first solve after the last build, plus 10.
+// (Re)builds the coarse inverse from the current FP32 tangent when there is none yet or when the last solve
+// needed more than twice the inner iterations per decade of residual reduction of the first solve after the last
+// build, plus 5 (the tangent has drifted from the one the coarse operator came from).
 void coarsePrepare(ik_handle* h) {
   if (!h->coarse) h->coarse = new Coarse();
   Coarse& cs = *h->coarse;
@@ -469,7 +477,10 @@ void coarsePrepare(ik_handle* h) {
   DevBuf<double> raw;
   raw.ensure((size_t)(nc * nc));
   cs.A.ensure((size_t)(nc * nc));
-  cs.G.ensure((size_t)(nc * nc));
+  if (!cs.G.p) {
+    cs.G.ensure((size_t)(nc * cs.ld));
+    launchZeroF(cs.G.p, nc * cs.ld, h->stream);  // the padding columns stay zero
+  }
   launchZero(raw.p, nc * nc, h->stream);
   hipLaunchKernelGGL(coarseAssemble<float>, dim3((unsigned)cs.na), dim3(256), 0, h->stream, h->K32.p, h->rowptr.p,
                      h->sellStart.p, h->scol.p, cs.agg.p, cs.aggPtr.p, cs.aggNodes.p, cs.nbrPtr.p, cs.nbr.p, cs.d.p,
@@ -491,27 +502,31 @@ void coarsePrepare(ik_handle* h) {
   IK_HIP(hipStreamSynchronize(h->stream));
   require(info[0] == 0 && info[1] == 0, IK_ESINGULAR,
           "coarse operator of the two-level preconditioner is not positive definite (indefinite tangent)");
-  hipLaunchKernelGGL(coarseInverseOut, dim3(g2), dim3(256), 0, h->stream, cs.A.p, cs.T.p, cs.na, cs.G.p);
+  hipLaunchKernelGGL(coarseInverseOut, dim3(g2), dim3(256), 0, h->stream, cs.A.p, cs.T.p, cs.na, cs.ld, cs.G.p);
   IK_HIP(hipStreamSynchronize(h->stream));  // raw is released on return
-  cs.valid        = true;
-  cs.itersAtBuild = -1;
+  cs.valid       = true;
+  cs.rateAtBuild = -1;
   cs.builds++;
 }
 
-void coarseRecord(ik_handle* h, int iterations) {
+void coarseRecord(ik_handle* h, int iterations, double relres) {
   Coarse& cs = *h->coarse;
-  if (cs.itersAtBuild < 0)
-    cs.itersAtBuild = iterations;
-  else if (iterations > 2 * cs.itersAtBuild + 10)
+  const double decades = std::max(0.5, -std::log10(std::max(relres, 1e-12)));
+  const double rate    = iterations / decades;
+  if (cs.rateAtBuild < 0)
+    cs.rateAtBuild = rate;
+  else if (rate > 2.0 * cs.rateAtBuild + 5.0)
     cs.valid = false;
 }
+
+int coarseBuilds(const ik_handle* h) { return h->coarse ? h->coarse->builds : 0; }
 
 void coarseApply(ik_handle* h, const float* r, float* z, float* p, const float* binv, int gn, double* partial) {
   Coarse& cs = *h->coarse;
   hipLaunchKernelGGL(coarseRestrict, dim3((unsigned)cs.na), dim3(256), 0, h->stream, r, h->fixed.p, cs.d.p,
                      cs.aggPtr.p, cs.aggNodes.p, cs.s.p, h->sc.p);
-  hipLaunchKernelGGL(coarseGemv, dim3((unsigned)((cs.nc + 3) / 4)), dim3(256), 0, h->stream, cs.G.p, cs.s.p, cs.w.p,
-                     cs.nc, h->sc.p);
+  hipLaunchKernelGGL(coarseGemv, dim3((unsigned)((cs.nc + 3) / 4)), dim3(256), 0, h->stream, (const float4*)cs.G.p,
+                     (const float4*)cs.s.p, cs.w.p, cs.nc, cs.ld / 4, h->sc.p);
   hipLaunchKernelGGL(coarseZ, dim3(gn), dim3(256), 0, h->stream, r, binv, h->fixed.p, cs.d.p, cs.agg.p, cs.w.p, z, p,
                      h->nn, partial, h->sc.p);
 }

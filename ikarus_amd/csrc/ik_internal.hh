@@ -252,7 +252,8 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
 // coarse.hip
 void coarseDestroy(Coarse* c);
 void coarsePrepare(ik_handle* h);
-void coarseRecord(ik_handle* h, int iterations);
+void coarseRecord(ik_handle* h, int iterations, double relres);
+int coarseBuilds(const ik_handle* h);
 void coarseApply(ik_handle* h, const float* r, float* z, float* p, const float* binv, int gn, double* partial);
 float* coarseZBuffer(ik_handle* h);
 int64_t coarseDofs(const ik_handle* h);

@@ -438,7 +438,10 @@ void buildPattern(ik_handle* h) {
               }
             }
         }
-        std::sort(touched.begin(), touched.end(), [&](int32_t a, int32_t b) { return qrank[a] < qrank[b]; });
+        // gather order: sort the ranks, then map back to upper blocks
+        for (auto& q : touched) q = qrank[q];
+        std::sort(touched.begin(), touched.end());
+        for (auto& q : touched) q = qord[q];
         const int ng = (int)touched.size();
         for (int g = 0; g < ng; ++g) groupOf[touched[g]] = g;
         std::fill(fill.begin(), fill.begin() + ng, 0);

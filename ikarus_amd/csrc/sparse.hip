@@ -730,7 +730,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
       break;
     }
   }
-  if (twoLevel) coarseRecord(h, info->iterations);
+  if (twoLevel) coarseRecord(h, info->iterations, info->rel_residual);
 }
 
 }  // namespace ik
