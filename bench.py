@@ -160,13 +160,14 @@ NEWTON_MODES = {
     "mixed_mlp": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
                       pred=capi.PRED_MLP, eta_max=0.01),
     # the mixed form with the two-level preconditioner (block Jacobi + rigid-body-mode coarse correction over
-    # box aggregates) in the FP32 PCG
+    # box aggregates) in the FP32 PCG; eta_max tuned the same way (64^3, 40 load steps, seeds 1000-1003:
+    # constant 0.03, secant 0.01, learned 0.001)
     "tl_constant": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
                         pred=capi.PRED_CONSTANT, eta_max=0.03),
     "tl_secant": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
-                      pred=capi.PRED_LINEAR, eta_max=0.1),
+                      pred=capi.PRED_LINEAR, eta_max=0.01),
     "tl_mlp": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
-                   pred=capi.PRED_MLP, eta_max=0.01),
+                   pred=capi.PRED_MLP, eta_max=0.001),
     # precision alone: the plain form's Newton and predictor with the FP32 tangent and refined FP32 PCG
     "mixed_exact": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.0,
                         pred=capi.PRED_CONSTANT),
@@ -220,10 +221,11 @@ def warm_libraries():
     return time.perf_counter() - t0
 
 
-def tune_eta(n, steps, mlp):
+def tune_eta(n, steps, mlp, modes=MIXED_MODES):
     """eta_max of each mixed predictor chosen on the training seeds: total seconds over TUNE_SEEDS per candidate."""
+    warm_libraries()
     table = {}
-    for mode in MIXED_MODES:
+    for mode in modes:
         table[mode] = {}
         for eta in ETA_CANDIDATES:
             tot, its = 0.0, 0
@@ -382,6 +384,7 @@ def main():
     ap.add_argument("--profile-mode", default="mixed_mlp", help="Newton mode of --profile-newton")
     ap.add_argument("--profile-newton", action="store_true",
                     help="only one mixed + learned load path (the command profiled for the MLP and PCG kernels)")
+    ap.add_argument("--tune-modes", default="", help="comma-separated modes for --tune-eta (default: all mixed)")
     ap.add_argument("--tune-eta", action="store_true",
                     help="only choose eta_max per mixed predictor on the training seeds (prints the table)")
     ap.add_argument("--c2-steps", type=int, default=5, help="load steps of config c2's FP64 path on the device")
@@ -407,7 +410,8 @@ def main():
         return
     if args.tune_eta:
         mlp = np.fromfile(args.mlp, dtype=np.float32)
-        print(json.dumps({"tune_eta": tune_eta(args.n, args.newton_steps, mlp)}), flush=True)
+        modes = tuple(args.tune_modes.split(",")) if args.tune_modes else MIXED_MODES
+        print(json.dumps({"tune_eta": tune_eta(args.n, args.newton_steps, mlp, modes)}), flush=True)
         return
     if args.profile_newton:
         # one mixed + learned load path on the first evaluation seed (the command profiled for the learned and PCG

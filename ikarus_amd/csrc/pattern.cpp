@@ -30,12 +30,15 @@ static int patternThreads() {
   const unsigned hc = std::thread::hardware_concurrency();
   return (int)std::max(1u, std::min(hc, 16u));
 }
-static int64_t chunkCount(int64_t n) { return std::min<int64_t>(patternThreads(), std::max<int64_t>(1, n / 2048)); }
+// chunks of at least `grain` items (2048 nodes, elements or tiles; 32 slices of 64 rows)
+static int64_t chunkCount(int64_t n, int64_t grain = 2048) {
+  return std::min<int64_t>(patternThreads(), std::max<int64_t>(1, n / grain));
+}
 
 // f(lo, hi, chunk) over chunk = 0 .. chunkCount(n) - 1; the first exception thrown is rethrown here
 template <class F>
-static void parallelChunks(int64_t n, F&& f) {
-  const int64_t T = chunkCount(n);
+static void parallelChunks(int64_t n, F&& f, int64_t grain = 2048) {
+  const int64_t T = chunkCount(n, grain);
   if (T <= 1) {
     f(int64_t(0), n, int64_t(0));
     return;
@@ -221,17 +224,21 @@ void buildPattern(ik_handle* h) {
         }
       }
     }
-  });
+  }, 32);
   {
+    // every block in its row's slice, at a slot of its own holding its column (slices are disjoint slot ranges,
+    // so each thread marks only its own slices' slots)
     std::vector<uint8_t> seen(h->nsell, 0);
-    for (int64_t i = 0; i < nn; ++i)
-      for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1]; ++b) {
-        const int32_t sl = h->hpos[b];
-        require(sl >= sstart[i / kSellRows] && sl < sstart[i / kSellRows + 1] && !seen[sl] &&
-                    scol[sl] == h->hcolidx[b],
-                IK_EINVAL, "internal error: bad sliced-ELL slot");
-        seen[sl] = 1;
-      }
+    parallelChunks(nslices, [&](int64_t s0, int64_t s1, int64_t) {
+      for (int64_t i = s0 * kSellRows; i < std::min(nn, s1 * kSellRows); ++i)
+        for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1]; ++b) {
+          const int32_t sl = h->hpos[b];
+          require(sl >= sstart[i / kSellRows] && sl < sstart[i / kSellRows + 1] && !seen[sl] &&
+                      scol[sl] == h->hcolidx[b],
+                  IK_EINVAL, "internal error: bad sliced-ELL slot");
+          seen[sl] = 1;
+        }
+    }, 32);
   }
   h->sellStart.upload(sstart.data(), sstart.size(), h->stream);
   h->scol.upload(scol.data(), scol.size(), h->stream);
@@ -360,16 +367,25 @@ void buildPattern(ik_handle* h) {
     qord.reserve(nup);
     for (int64_t c = 0; c < nchunks; ++c) {
       const int64_t r0 = h->chunkRow[c], r1 = h->chunkRow[c + 1];
-      for (int64_t sl = r0 / kSellRows; r1 > r0 && sl <= (r1 - 1) / kSellRows; ++sl)
-        for (int32_t slot = sstart[sl]; slot < sstart[sl + 1]; ++slot) {
-          const int32_t q   = qAtSlot[slot];
-          const int64_t row = sl * kSellRows + (slot - sstart[sl]) % kSellRows;
-          if (q < 0 || row < r0 || row >= r1) continue;
-          qrank[q] = (int32_t)qord.size();
-          qord.push_back(q);
-        }
+      if (r1 > r0) {
+        // the chunk's slices in ranges per thread, each collecting its upper blocks in slot order, then joined
+        const int64_t sl0 = r0 / kSellRows, nsl = (r1 - 1) / kSellRows + 1 - sl0;
+        std::vector<std::vector<int32_t>> part(chunkCount(nsl, 32));
+        parallelChunks(nsl, [&](int64_t a0, int64_t a1, int64_t ch) {
+          for (int64_t sl = sl0 + a0; sl < sl0 + a1; ++sl)
+            for (int32_t slot = sstart[sl]; slot < sstart[sl + 1]; ++slot) {
+              const int32_t q   = qAtSlot[slot];
+              const int64_t row = sl * kSellRows + (slot - sstart[sl]) % kSellRows;
+              if (q >= 0 && row >= r0 && row < r1) part[ch].push_back(q);
+            }
+        }, 32);
+        for (const auto& v : part) qord.insert(qord.end(), v.begin(), v.end());
+      }
       qchunk[c + 1] = (int64_t)qord.size();
     }
+    parallelChunks((int64_t)qord.size(), [&](int64_t k0, int64_t k1, int64_t) {
+      for (int64_t k = k0; k < k1; ++k) qrank[qord[k]] = (int32_t)k;
+    });
     require((int64_t)qord.size() == nup, IK_EINVAL, "internal error: gather order misses upper blocks");
   }
   // staging segment of upper block q: [segBeg[q], segBeg[q] + segCnt[q]), segments laid out in gather order
