@@ -209,9 +209,9 @@ MIXED_MODES = ("mixed_constant", "mixed_secant", "mixed_mlp", "tl_constant", "tl
 
 
 def warm_libraries():
-    """One tiny two-level solve, untimed: rocBLAS / rocSOLVER load their kernels on the first call in a process
-    (about 0.3 s), which no later Newton path pays."""
-    p = pb.hex_block((4, 4, 4), seed=0)
+    """One small two-level solve, untimed: rocBLAS / rocSOLVER load their kernels on the first call of a size in a
+    process (about 0.25 s for the 3,072 coarse DOFs of both 16^3 and 64^3), which no later Newton path pays."""
+    p = pb.hex_block((16, 16, 16), seed=0)
     dev = ia.Device(p)
     dev.set_requirement(np.zeros(p.ndof), 0.0)
     dev.matrix(capi.DBC_FULL, capi.FP32)
@@ -417,6 +417,8 @@ def main():
         # one mixed + learned load path on the first evaluation seed (the command profiled for the learned and PCG
         # kernels)
         mlp = np.fromfile(args.mlp, dtype=np.float32)
+        if NEWTON_MODES[args.profile_mode]["kind"] == capi.PCG_TWOLEVEL_FP32_IR:
+            warm_libraries()
         r, _ = newton_path(pb.hex_block((args.n,) * 3, seed=EVAL_SEEDS[0]), args.profile_mode, args.newton_steps, mlp)
         print(json.dumps({"profile_newton": r}), flush=True)
         return

@@ -42,7 +42,7 @@ struct Coarse {
   DevBuf<double> A;         // [nc*nc] coarse operator, then its inverse (column-major lower triangle)
   DevBuf<float> G;          // [nc*ld] T A^-1 T^T, row-major
   DevBuf<float> s, w;       // [ld], [nc]
-  DevBuf<float> z;          // [ndof]
+  DevBuf<double> ws;        // [ld / 4] per-workgroup partials of w . s (coarse GEMV)
   DevBuf<int32_t> info;
   rocblas_handle blas = nullptr;
   ~Coarse() {
@@ -53,7 +53,7 @@ struct Coarse {
 namespace {
 
 constexpr int kTargetAggregates = 512;
-constexpr int kMaxAggregates    = 1536;
+constexpr int kMaxAggregates    = 1024;  // the update's 2 na partials fit the first 2 kReduceBlocks entries
 
 __device__ inline double waveSumD(double v) {
 #pragma unroll
@@ -280,6 +280,160 @@ __global__ __launch_bounds__(256) void coarseZ(const float* __restrict__ r, cons
   if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// N workgroup sums (4 wavefronts) at once, in a fixed order; the results are valid in every thread. One barrier
+// pair for all N (the wave reductions run side by side).
+template <int N>
+__device__ inline void groupSums(double (&v)[N], double (&red)[4][N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = waveSumD(v[i]);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) red[threadIdx.x >> 6][i] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+  __syncthreads();
+}
+
+// PCG iteration of the two-level form, part 1 (one workgroup per aggregate, over its nodes): alpha = r.z / p.q
+// (every workgroup folds the npq p.q partials in the same order; workgroup 0 records the scalars),
+// x += alpha p, r -= alpha q, and for the new r: the r.r and r.D^-1 r partials (partial[a], partial[na + a]) and
+// the restriction s_a = sum B(d_i)^T r_i (constrained components zeroed). With w = G s, r.z = r.D^-1 r + w.s,
+// since r . B(d_i) w_a summed over the nodes is w_a . s_a.
+__global__ __launch_bounds__(256) void coarseUpdate(float* __restrict__ x, float* __restrict__ r,
+                                                    const float* __restrict__ p, const float* __restrict__ q,
+                                                    const float* __restrict__ binv, const uint8_t* __restrict__ fixed,
+                                                    const float* __restrict__ dd, const int32_t* __restrict__ aggPtr,
+                                                    const int32_t* __restrict__ aggNodes, int na, float* __restrict__ s,
+                                                    const double* __restrict__ pq, int npq,
+                                                    double* __restrict__ partial, PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double red1[4][1];
+  __shared__ double red8[4][8];
+  const int a = blockIdx.x;
+  double pqv[1] = {0.0};
+  for (int i = threadIdx.x; i < npq; i += 256) pqv[0] += pq[i];
+  groupSums<1>(pqv, red1);
+  const double rz = sc->rz_new;  // r.z of the current iterate (set by the previous p update or the init fold)
+  const double alpha = rz / pqv[0];
+  if (a == 0 && threadIdx.x == 0) {
+    sc->rz    = rz;
+    sc->pq    = pqv[0];
+    sc->alpha = alpha;
+  }
+  const float al = (float)alpha;
+  double rr = 0.0, rzb = 0.0;
+  float sa[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = aggPtr[a] + threadIdx.x; t < aggPtr[a + 1]; t += 256) {
+    const int64_t i = aggNodes[t];
+    float rl[3], v[3], o[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int64_t j = 3 * i + k;
+      x[j] += al * p[j];
+      rl[k] = r[j] - al * q[j];
+      r[j]  = rl[k];
+      rr += (double)rl[k] * (double)rl[k];
+      v[k] = fixed[j] ? 0.0f : rl[k];
+    }
+    const float* B = binv + i * 9;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      rzb += (double)rl[k] * (double)(B[3 * k] * rl[0] + B[3 * k + 1] * rl[1] + B[3 * k + 2] * rl[2]);
+    const float di[3] = {dd[3 * i], dd[3 * i + 1], dd[3 * i + 2]};
+    modesT<float>(v, di, o);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) sa[m] += o[m];
+  }
+  double v[8] = {rr, rzb, sa[0], sa[1], sa[2], sa[3], sa[4], sa[5]};
+  groupSums<8>(v, red8);
+  if (threadIdx.x == 0) {
+    partial[a]      = v[0];
+    partial[na + a] = v[1];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) s[6 * a + m] = (float)v[2 + m];
+  }
+}
+
+// part 2: w = G s (as coarseGemv) and per workgroup the partial of w . s
+__global__ __launch_bounds__(256) void coarseGemvDot(const float4* __restrict__ G, const float4* __restrict__ s4,
+                                                     const float* __restrict__ s, float* __restrict__ w, int64_t nc,
+                                                     int64_t ld4, double* __restrict__ ws,
+                                                     const PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  float acc = 0.0f;
+  if (row < nc) {
+    const float4* g = G + row * ld4;
+    for (int64_t c = lane; c < ld4; c += 64) {
+      const float4 a = g[c], b = s4[c];
+      acc += a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+    }
+  }
+  acc = waveSumF(acc);
+  if (lane == 0) {
+    if (row < nc) w[row] = acc;
+    red[threadIdx.x >> 6] = row < nc ? (double)acc * (double)s[row] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) ws[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// part 3: every workgroup folds r.r, r.D^-1 r (2 na partials) and w.s (nws partials) in the same order; the
+// convergence test and beta = r.z / r.z_prev, then p = z + beta p with z_i = D_i^-1 r_i + B(d_i) w_a
+// (constrained components of the coarse part zeroed) formed on the fly
+__global__ __launch_bounds__(256) void coarseBetaP(float* __restrict__ p, const float* __restrict__ r,
+                                                   const float* __restrict__ binv, const uint8_t* __restrict__ fixed,
+                                                   const float* __restrict__ dd, const int32_t* __restrict__ agg,
+                                                   const float* __restrict__ w, int64_t nn,
+                                                   const double* __restrict__ partial, int na,
+                                                   const double* __restrict__ ws, int nws,
+                                                   PcgScalars* __restrict__ sc, int maxIter) {
+  if (sc->done) return;
+  __shared__ double red3[4][3];
+  double f[3] = {0.0, 0.0, 0.0};  // r.r, r.D^-1 r, w.s
+  for (int i = threadIdx.x; i < na; i += 256) {
+    f[0] += partial[i];
+    f[1] += partial[na + i];
+  }
+  for (int i = threadIdx.x; i < nws; i += 256) f[2] += ws[i];
+  groupSums<3>(f, red3);
+  const double rr = f[0], rz = f[1] + f[2];
+  const bool conv = rr < sc->threshold;
+  const double bt = rz / sc->rz;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rr = rr;
+    if (conv) {
+      sc->done = 1;
+    } else {
+      sc->beta   = bt;
+      sc->rz_new = rz;
+      sc->iters += 1;
+      if (sc->iters >= maxIter) sc->done = 2;
+    }
+  }
+  if (conv) return;
+  const float beta = (float)bt;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nn; i += (int64_t)gridDim.x * 256) {
+    const float rl[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+    const float di[3] = {dd[3 * i], dd[3 * i + 1], dd[3 * i + 2]};
+    const int32_t a   = agg[i];
+    float wa[6], cz[3];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) wa[m] = w[6 * a + m];
+    modes<float>(wa, di, cz);
+    const float* B = binv + i * 9;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float zk = B[3 * k] * rl[0] + B[3 * k + 1] * rl[1] + B[3 * k + 2] * rl[2];
+      if (!fixed[3 * i + k]) zk += cz[k];
+      p[3 * i + k] = zk + beta * p[3 * i + k];
+    }
+  }
+}
+
 // cyclic Jacobi eigen-decomposition of a symmetric 6x6 matrix (host): a becomes diagonal, v the eigenvectors
 // (columns)
 void jacobiEigen6(double a[6][6], double v[6][6]) {
@@ -454,7 +608,7 @@ void buildAggregates(ik_handle* h, Coarse& cs) {
   cs.s.ensure(cs.ld);
   launchZeroF(cs.s.p, cs.ld, h->stream);
   cs.w.ensure(cs.nc);
-  cs.z.ensure(h->ndof);
+  cs.ws.ensure(cs.ld / 4);
   cs.info.ensure(2);
   cs.ready = true;
 }
@@ -531,6 +685,18 @@ void coarseApply(ik_handle* h, const float* r, float* z, float* p, const float* 
                      h->nn, partial, h->sc.p);
 }
 
-float* coarseZBuffer(ik_handle* h) { return h->coarse->z.p; }
+// the p.q fold, update and restriction, coarse GEMV and p update of one two-level PCG iteration (grid gn for the p
+// update; the update's partials occupy partial[0, 2 na))
+void coarseIteration(ik_handle* h, const double* pq, int npq, float* x, float* r, float* p, const float* q,
+                     const float* binv, int gn, double* partial, int maxIter) {
+  Coarse& cs = *h->coarse;
+  hipLaunchKernelGGL(coarseUpdate, dim3((unsigned)cs.na), dim3(256), 0, h->stream, x, r, p, q, binv, h->fixed.p,
+                     cs.d.p, cs.aggPtr.p, cs.aggNodes.p, (int)cs.na, cs.s.p, pq, npq, partial, h->sc.p);
+  const unsigned gg = (unsigned)((cs.nc + 3) / 4);
+  hipLaunchKernelGGL(coarseGemvDot, dim3(gg), dim3(256), 0, h->stream, (const float4*)cs.G.p, (const float4*)cs.s.p,
+                     cs.s.p, cs.w.p, cs.nc, cs.ld / 4, cs.ws.p, h->sc.p);
+  hipLaunchKernelGGL(coarseBetaP, dim3(gn), dim3(256), 0, h->stream, p, r, binv, h->fixed.p, cs.d.p, cs.agg.p, cs.w.p,
+                     h->nn, partial, (int)cs.na, cs.ws.p, (int)gg, h->sc.p, maxIter);
+}
 
 }  // namespace ik

@@ -255,7 +255,8 @@ void coarsePrepare(ik_handle* h);
 void coarseRecord(ik_handle* h, int iterations, double relres);
 int coarseBuilds(const ik_handle* h);
 void coarseApply(ik_handle* h, const float* r, float* z, float* p, const float* binv, int gn, double* partial);
-float* coarseZBuffer(ik_handle* h);
+void coarseIteration(ik_handle* h, const double* pq, int npq, float* x, float* r, float* p, const float* q,
+                     const float* binv, int gn, double* partial, int maxIter);
 int64_t coarseDofs(const ik_handle* h);
 void residualBlock64(ik_handle* h, const double* b, const double* x, double* r);  // r = b - K x (FP64 K)
 // mlp.hip

@@ -334,14 +334,7 @@ __global__ __launch_bounds__(256) void pcgUpdateKernel(T* __restrict__ x, T* __r
   }
   const T alpha = (T)alphaS;
   double rr = 0.0, rz = 0.0;
-  if (!pre) {  // two-level form: r.z follows from the coarse correction (coarseZ)
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-      x[i] += alpha * p[i];
-      const T ri = r[i] - alpha * q[i];
-      r[i]       = ri;
-      rr += (double)ri * (double)ri;
-    }
-  } else if (!blockPre) {
+  if (!blockPre) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
       x[i] += alpha * p[i];
       const T ri = r[i] - alpha * q[i];
@@ -369,20 +362,19 @@ __global__ __launch_bounds__(256) void pcgUpdateKernel(T* __restrict__ x, T* __r
   rr = blockSum<256>(rr);
   rz = blockSum<256>(rz);
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = rr;
-    if (pre) partial[gridDim.x + blockIdx.x] = rz;
+    partial[blockIdx.x]             = rr;
+    partial[gridDim.x + blockIdx.x] = rz;
   }
 }
 
-// Convergence test, beta and p = M^-1 r + beta p in one launch (z = M^-1 r read from zin when given). Every block folds the update kernel's
+// Convergence test, beta and p = M^-1 r + beta p in one launch. Every block folds the update kernel's
 // partials (rr, rz) itself in the same fixed order, so all blocks agree bitwise; block 0 records the
 // scalars. beta = rz_new / rz uses the r.z of the previous iterate, which only pcgAlphaFold writes.
 template <typename T>
 __global__ __launch_bounds__(256) void pcgBetaPKernel(T* __restrict__ p, const T* __restrict__ r,
                                                       const T* __restrict__ pre, int blockPre, int64_t n,
                                                       const double* __restrict__ partial, int nb,
-                                                      PcgScalars* __restrict__ sc, int maxIter,
-                                                      const T* __restrict__ zin) {
+                                                      PcgScalars* __restrict__ sc, int maxIter) {
   if (sc->done) return;
   __shared__ double bc[2];
   double rr = 0.0, rz = 0.0;
@@ -414,10 +406,7 @@ __global__ __launch_bounds__(256) void pcgBetaPKernel(T* __restrict__ p, const T
   }
   if (conv) return;
   const T beta = (T)bt;
-  if (zin) {
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-      p[i] = zin[i] + beta * p[i];
-  } else if (!blockPre) {
+  if (!blockPre) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
       p[i] = pre[i] * r[i] + beta * p[i];
   } else {
@@ -627,7 +616,7 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
       hipLaunchKernelGGL((pcgUpdateKernel<double>), dim3(g), dim3(256), 0, h->stream, x, h->pr.p, h->pp.p, h->pq.p,
                          h->pdiag.p, 0, n, h->sc.p, pqSrc, npq, h->partial.p);
       hipLaunchKernelGGL((pcgBetaPKernel<double>), dim3(g), dim3(256), 0, h->stream, h->pp.p, h->pr.p, h->pdiag.p, 0,
-                         n, h->partial.p, g, h->sc.p, maxIter, (const double*)nullptr);
+                         n, h->partial.p, g, h->sc.p, maxIter);
     }
     IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
     IK_HIP(hipStreamSynchronize(h->stream));
@@ -677,7 +666,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
     if (twoLevel) {
       hipLaunchKernelGGL(pcgInitR32, dim3(gn), dim3(256), 0, h->stream, h->pr.p, 1.0 / rnorm, h->fx.p, h->fr.p, n,
                          h->partial.p, h->sc.p);
-      coarseApply(h, h->fr.p, coarseZBuffer(h), h->fp.p, h->fdiag.p, gn, h->partial.p);
+      coarseApply(h, h->fr.p, h->fp.p, nullptr, h->fdiag.p, gn, h->partial.p);  // p = z
     } else {
       hipLaunchKernelGGL(pcgInitF32, dim3(gn), dim3(256), 0, h->stream, h->pr.p, 1.0 / rnorm, h->fdiag.p, h->fx.p,
                          h->fr.p, h->fp.p, h->nn, h->partial.p);
@@ -701,12 +690,14 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
           pqSrc = pq2(h);
           npq   = kPqBlocks;
         }
+        if (twoLevel) {
+          coarseIteration(h, pqSrc, npq, h->fx.p, h->fr.p, h->fp.p, h->fq.p, h->fdiag.p, gn, h->partial.p, maxIter);
+          continue;
+        }
         hipLaunchKernelGGL((pcgUpdateKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fx.p, h->fr.p, h->fp.p,
-                           h->fq.p, twoLevel ? nullptr : h->fdiag.p, 1, n, h->sc.p, pqSrc, npq, h->partial.p);
-        if (twoLevel) coarseApply(h, h->fr.p, coarseZBuffer(h), nullptr, h->fdiag.p, gn, h->partial.p);
+                           h->fq.p, h->fdiag.p, 1, n, h->sc.p, pqSrc, npq, h->partial.p);
         hipLaunchKernelGGL((pcgBetaPKernel<float>), dim3(gn), dim3(256), 0, h->stream, h->fp.p, h->fr.p, h->fdiag.p,
-                           1, n, h->partial.p, gn, h->sc.p, maxIter,
-                           twoLevel ? (const float*)coarseZBuffer(h) : (const float*)nullptr);
+                           1, n, h->partial.p, gn, h->sc.p, maxIter);
       }
       IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
       IK_HIP(hipStreamSynchronize(h->stream));
