@@ -29,7 +29,8 @@ enum {
   IK_ENOCONV    = -6, /* linear solver did not reach its tolerance                               */
   IK_ESINGULAR  = -7  /* EAS: D of an element is singular to working precision (a pivot of the
                          unpivoted elimination is zero or not finite); the reference inverts D by
-                         pivoted LU (enhancedassumedstrains.hh:243,294,344) and has no such error */
+                         pivoted LU (enhancedassumedstrains.hh:243,294,344) and has no such error.
+                         IK_PCG_TWOLEVEL_FP32_IR: its coarse operator is not positive definite */
 };
 
 /* DBCOption (ikarus/assembler/dirichletbcenforcement.hh:8) */
