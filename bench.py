@@ -575,13 +575,15 @@ def main():
 
 def config_precision(args):
     """128x128x64 hex8 (3.24 M DOFs), Neo-Hooke and SVK: the FP32-tangent / FP32-PCG + FP64-refinement form
-    against the FP64-tangent form on the same seeded mesh and load, 5 load steps to ||R|| <= 1e-8."""
+    (block-Jacobi and two-level preconditioners) against the FP64-tangent form on the same seeded mesh and load,
+    5 load steps to ||R|| <= 1e-8."""
+    warm_libraries()
     res = {}
     for name, mat in (("neohooke", capi.NEOHOOKE), ("svk", capi.SVK)):
         p = pb.hex_block((128, 128, 64), material=mat, seed=0)
         log(f"[bench] c3 {name}: {p.ne} elements, {p.ndof} dofs")
         out, us = {}, {}
-        for mode in ("fp64", "mixed_exact", "mixed_secant"):
+        for mode in ("fp64", "mixed_exact", "mixed_secant", "tl_secant"):
             out[mode], us[mode] = newton_path(p, mode, args.c3_steps)
             out[mode]["seconds_per_load_step"] = round(out[mode]["seconds"] / args.c3_steps, 4)
             log(f"[bench] c3 {name} {mode}: {out[mode]}")
@@ -627,6 +629,14 @@ def config_eas(args):
         r["speedup_vs_fp64"] = round(out["newton_fp64"]["seconds"] / r["seconds"], 3)
         r["rel_diff_u_vs_fp64"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
         out["newton_mixed_secant"] = r
+        # the same with the two-level preconditioner
+        log("[bench] c4 newton tl_secant ...")
+        warm_libraries()
+        r, u2l = newton_path(p, "tl_secant", args.c4_load_steps)
+        r["seconds_per_load_step"] = round(r["seconds"] / args.c4_load_steps, 4)
+        r["speedup_vs_fp64"] = round(out["newton_fp64"]["seconds"] / r["seconds"], 3)
+        r["rel_diff_u_vs_fp64"] = float(np.linalg.norm(u2l - u64) / np.linalg.norm(u64))
+        out["newton_tl_secant"] = r
     return out
 
 

@@ -51,12 +51,14 @@ def test_arc_length_matches_oracle(material):
     assert np.all(rn[1:] <= 1e-8)
 
 
-def test_arc_length_mixed_precision():
-    """The bordered solves by the mixed form (FP32 tangent and PCG, FP64 refinement to 1e-10): same
-    iterations as the oracle, the final state in FP64 equilibrium (oracle residual <= 2e-8), load factors
-    and displacements within 1e-6 (the tolerance region's extent, see below)."""
+@pytest.mark.parametrize("kind", [capi.PCG_BJACOBI_FP32_IR, capi.PCG_TWOLEVEL_FP32_IR])
+def test_arc_length_mixed_precision(kind):
+    """The bordered solves by the mixed form (FP32 tangent and PCG with the block-Jacobi or the two-level
+    preconditioner, FP64 refinement to 1e-10): same iterations as the oracle, the final state in FP64 equilibrium
+    (oracle residual <= 2e-8), load factors and displacements within 1e-6 (the tolerance region's extent, see
+    below)."""
     p = pb.hex_block((4, 3, 3), jitter=0.05, seed=41, mag_range=(20.0, 30.0))
-    print("case: arc length, mixed precision", flush=True)
+    print(f"case: arc length, mixed precision, solver kind {kind}", flush=True)
     o = _oracle_for(p)
     res = lambda u, lam: o.vector(u, lam, po.FULL)
     jac = lambda u, lam: o.matrix_dense(u, lam, po.FULL)
@@ -68,7 +70,7 @@ def test_arc_length_mixed_precision():
     cfg = capi.PFConfig()
     cfg.steps, cfg.step_size = 4, s
     st = ia.nr_settings(tol=1e-8, precision=capi.FP32,
-                        linear=ia.solver_opts(kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+                        linear=ia.solver_opts(kind=kind, rtol=1e-10, inner_rtol=1e-3))
     info, iters, rn, lams = dev.path_following(cfg, st)
     ug, lam = dev.get_requirement()
     dev.close()

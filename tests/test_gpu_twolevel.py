@@ -141,3 +141,21 @@ def test_two_level_plane_strain_load_control_matches_oracle():
     assert info.success and res[info.solves - 1] <= 1e-8
     assert _rel(ug, r["u"]) < 1e-8
     dev.close()
+
+
+def test_two_level_solve_on_scrambled_mesh():
+    """Box aggregation works from coordinates, so a mesh with holes, shuffled elements and relabelled nodes gives
+    the same solution as the direct solve."""
+    from test_gpu_parity import _scrambled_block
+    p = _scrambled_block((9, 8, 7), seed=17)
+    print(f"case: two-level scrambled mesh, {p.ne} elements", flush=True)
+    o = _oracle_for(p)
+    u = 0.01 * np.random.default_rng(5).standard_normal(p.ndof) * (1 - p.fixed)
+    dev = ia.Device(p)
+    dev.set_requirement(u, 0.9)
+    dev.matrix(capi.DBC_FULL, capi.FP32)
+    b = np.random.default_rng(6).standard_normal(p.ndof) * (1 - p.fixed)
+    x, info = dev.solve(b, ia.solver_opts(kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+    assert info.converged == 1
+    assert _rel(x, o.direct_solve(u, 0.9, b)) < 1e-5
+    dev.close()
