@@ -883,6 +883,7 @@ int ik_matrix_bsr(ik_handle* h, int64_t* nbr, int64_t* nnzb, const int32_t** row
     require(!h->plane2d, IK_ENOTIMPL, "the block view is the 3D storage; a plane-strain handle exports CSC only");
     ensurePattern(h);
     if (block_pos && !h->blockPos.p) h->blockPos.upload(h->hpos.data(), h->hpos.size(), h->stream);
+    if (colidx && !h->colidx.p) h->colidx.upload(h->hcolidx.data(), h->hcolidx.size(), h->stream);
     if (nbr) *nbr = h->nn;
     if (nnzb) *nnzb = h->nnzb;
     if (rowptr) *rowptr = h->rowptr.p;

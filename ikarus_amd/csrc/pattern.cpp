@@ -137,17 +137,19 @@ static std::vector<int32_t> mortonOrder(const ik_handle* h) {
 void buildPattern(ik_handle* h) {
   const int64_t nn = h->nn, ne = h->ne;
   const int32_t* conn = h->hconn.data();
-  // node -> incident elements (CSR)
+  // node -> incidences e*8 + a (CSR, ordered by element): nlist; the incident elements are nlist / 8
   std::vector<int64_t> cnt(nn + 1, 0);
   for (int64_t e = 0; e < ne; ++e)
     for (int a = 0; a < 8; ++a) cnt[conn[e * 8 + a] + 1]++;
   for (int64_t i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
-  std::vector<int64_t> inc(cnt[nn]);
+  require(cnt[nn] < (int64_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 incidence lists");
+  std::vector<int32_t> nlist(cnt[nn]);
   {
     std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
     for (int64_t e = 0; e < ne; ++e)
-      for (int a = 0; a < 8; ++a) inc[pos[conn[e * 8 + a]]++] = e;
+      for (int a = 0; a < 8; ++a) nlist[pos[conn[e * 8 + a]]++] = (int32_t)(e * 8 + a);
   }
+  auto inc = [&](int64_t k) { return (int64_t)(nlist[k] >> 3); };
   // rows: union of the nodes of incident elements (per chunk of rows, then placed by the row counts)
   h->hrowptr.assign(nn + 1, 0);
   {
@@ -160,7 +162,7 @@ void buildPattern(ik_handle* h) {
       for (int64_t i = i0; i < i1; ++i) {
         buf.clear();
         for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) {
-          const int64_t e = inc[k];
+          const int64_t e = inc(k);
           for (int b = 0; b < 8; ++b) buf.push_back(conn[e * 8 + b]);
         }
         std::sort(buf.begin(), buf.end());
@@ -336,7 +338,7 @@ void buildPattern(ik_handle* h) {
   {
     std::vector<int64_t> last(nn, -1);
     for (int64_t i = 0; i < nn; ++i) {
-      for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) last[i] = std::max(last[i], unitOf[inc[k]]);
+      for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) last[i] = std::max(last[i], unitOf[inc(k)]);
       if (i > 0) last[i] = std::max(last[i], last[i - 1]);
     }
     const int64_t chunk = h->tiled ? (chunkElements() + kTileElems - 1) / kTileElems : chunkElements();
@@ -499,11 +501,15 @@ void buildPattern(ik_handle* h) {
     // order: segments in the order qord, each ordered by tile. gdst[g] = slot, or -(K block + 1) with
     // gpt[g] = the mirror block and gmsk[g] = the mask of a direct group.
     std::vector<int32_t> ntile(nup, 0);
-    for (const int32_t q : gq) ntile[q]++;
-    for (int64_t q = 0; q < nup; ++q) {
-      gathered[q] = ntile[q] > 1;
-      segCnt[q]   = gathered[q] ? ntile[q] : 0;
-    }
+    parallelChunks((int64_t)gq.size(), [&](int64_t g0, int64_t g1, int64_t) {
+      for (int64_t g = g0; g < g1; ++g) __atomic_fetch_add(&ntile[gq[g]], 1, __ATOMIC_RELAXED);
+    });
+    parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
+      for (int64_t q = q0; q < q1; ++q) {
+        gathered[q] = ntile[q] > 1;
+        segCnt[q]   = gathered[q] ? ntile[q] : 0;
+      }
+    });
     h->nstage = layoutSegments();
     std::vector<int32_t> pos(segBeg), gdst(gq.size()), gpt(gq.size(), 0);
     std::vector<uint16_t> gmsk(gq.size(), 0);
@@ -520,23 +526,25 @@ void buildPattern(ik_handle* h) {
     // the kernel's indexing relies on these: in every tile each valid entry has a distinct (group, rank)
     // with group < ng and rank < its group's size < tmaxr + 1, every group is non-empty, padding entries
     // target spare accumulators in [ng, 576); the slots are a permutation of [0, nstage) with each group
-    // inside its own upper block's segment, and every upper block is either staged by two or more tiles or
-    // written by exactly one
+    // inside its own upper block's segment (the segments are disjoint consecutive ranges, each slot handed out
+    // by a counter that ends exactly at its segment's end), and every upper block is either staged by two or
+    // more tiles or written by exactly one
     {
-      std::vector<uint8_t> slotSeen(h->nstage, 0);
-      for (int64_t q = 0; q < nup; ++q)
-        require(ntile[q] >= 1 && segCnt[q] == (gathered[q] ? ntile[q] : 0), IK_EINVAL,
-                "internal error: upper block without a tile");
-      for (size_t g = 0; g < gq.size(); ++g) {
-        const int32_t sl = gdst[g];
-        if (sl < 0) {
-          require(!gathered[gq[g]] && -sl - 1 == up[gq[g]], IK_EINVAL, "internal error: bad direct group");
-          continue;
+      parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
+        for (int64_t q = q0; q < q1; ++q)
+          require(ntile[q] >= 1 && segCnt[q] == (gathered[q] ? ntile[q] : 0) && pos[q] == segBeg[q] + segCnt[q],
+                  IK_EINVAL, "internal error: upper block without a tile or a segment not filled");
+      });
+      parallelChunks((int64_t)gq.size(), [&](int64_t g0, int64_t g1, int64_t) {
+        for (int64_t g = g0; g < g1; ++g) {
+          const int32_t sl = gdst[g], q = gq[g];
+          if (sl < 0)
+            require(!gathered[q] && -sl - 1 == up[q], IK_EINVAL, "internal error: bad direct group");
+          else
+            require(sl >= segBeg[q] && sl < segBeg[q] + segCnt[q], IK_EINVAL,
+                    "internal error: staging slot outside its segment");
         }
-        require(sl >= segBeg[gq[g]] && sl < segBeg[gq[g]] + segCnt[gq[g]] && !slotSeen[sl], IK_EINVAL,
-                "internal error: staging slots are not a permutation");
-        slotSeen[sl] = 1;
-      }
+      });
       parallelChunks(ntiles, [&](int64_t t0, int64_t t1, int64_t) {
         std::vector<uint8_t> used(kTileEntries * 16);
         std::vector<int32_t> fill(kTileEntries);
@@ -603,16 +611,9 @@ void buildPattern(ik_handle* h) {
     h->upT.upload(cupT.data(), cupT.size(), h->stream);
     h->segptr.upload(cseg.data(), cseg.size(), h->stream);
   }
-  // node incidences (e*8 + a), ordered by element
-  std::vector<int32_t> ncnt(nn + 1, 0), nlist((size_t)ne * 8);
-  for (int64_t i = 0; i < nn; ++i) ncnt[i + 1] = (int32_t)(cnt[i + 1] - cnt[0]);
-  {
-    std::vector<int32_t> pos(ncnt.begin(), ncnt.end() - 1);
-    for (int64_t e = 0; e < ne; ++e)
-      for (int a = 0; a < 8; ++a) nlist[pos[conn[e * 8 + a]]++] = (int32_t)(e * 8 + a);
-  }
-  h->rowptr.upload(h->hrowptr.data(), h->hrowptr.size(), h->stream);
-  h->colidx.upload(h->hcolidx.data(), h->hcolidx.size(), h->stream);
+  std::vector<int32_t> ncnt(nn + 1, 0);
+  for (int64_t i = 0; i <= nn; ++i) ncnt[i] = (int32_t)cnt[i];
+  h->rowptr.upload(h->hrowptr.data(), h->hrowptr.size(), h->stream);  // colidx: uploaded when ik_matrix_bsr asks
   {
     std::vector<int32_t> dslot(nn);
     for (int64_t i = 0; i < nn; ++i) dslot[i] = h->hpos[h->hdiag[i]];

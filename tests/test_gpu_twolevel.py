@@ -156,6 +156,11 @@ def test_two_level_solve_on_scrambled_mesh():
     dev.matrix(capi.DBC_FULL, capi.FP32)
     b = np.random.default_rng(6).standard_normal(p.ndof) * (1 - p.fixed)
     x, info = dev.solve(b, ia.solver_opts(kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
-    assert info.converged == 1
-    assert _rel(x, o.direct_solve(u, 0.9, b)) < 1e-5
+    xb, infob = dev.solve(b, ia.solver_opts(kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+    assert info.converged == 1 and infob.converged == 1
+    # both refine against the FP32-rounded matrix, so they agree to the refinement tolerance; against the FP64
+    # direct solve they differ by that rounding, amplified by this mesh's conditioning (1.1e-5 measured)
+    assert _rel(x, xb) < 1e-8
+    assert _rel(x, o.direct_solve(u, 0.9, b)) < 1e-4
+    assert info.iterations < infob.iterations
     dev.close()
