@@ -431,11 +431,20 @@ void buildPattern(ik_handle* h) {
     std::vector<uint8_t> tmaxr(ntiles);
     std::vector<int32_t> tgptr(ntiles + 1, 0), gq;
     std::vector<int64_t> tng(ntiles);
-    std::vector<std::vector<int32_t>> gpart(chunkCount(ntiles));
+    // a tile's upper blocks -> group: open-addressing table of 2048 slots (a tile touches at most 576 upper blocks),
+    // a slot valid when its stamp is the current tile
+    constexpr int kHash = 2048;
+    std::vector<std::vector<int32_t>> gpart(chunkCount(ntiles, 256));
     parallelChunks(ntiles, [&](int64_t t0, int64_t t1, int64_t ch) {
-      std::vector<int32_t> groupOf(nup, -1), touched, entQ(kTileEntries), fill(kTileEntries);
+      std::vector<int32_t> hkey(kHash), hval(kHash), touched, entQ(kTileEntries), fill(kTileEntries);
+      std::vector<int64_t> hstamp(kHash, -1);
       std::vector<uint16_t> entTr(kTileEntries);
       std::vector<int32_t>& out = gpart[ch];
+      auto slotOf = [&](int32_t q, int64_t t) {  // the table slot of q in tile t (free or holding q)
+        uint32_t k = ((uint32_t)q * 2654435761u) & (kHash - 1);
+        while (hstamp[k] == t && hkey[k] != q) k = (k + 1) & (kHash - 1);
+        return k;
+      };
       for (int64_t t = t0; t < t1; ++t) {
         touched.clear();
         int nvalid = 0;
@@ -450,8 +459,10 @@ void buildPattern(ik_handle* h) {
               const int32_t q = qOf[slots[(int64_t)e * 36 + sb]];
               entQ[j]         = q;
               entTr[j]        = conn[(int64_t)e * 8 + a] > conn[(int64_t)e * 8 + c] ? 0x8000u : 0u;
-              if (groupOf[q] < 0) {
-                groupOf[q] = 0;
+              const uint32_t k = slotOf(q, t);
+              if (hstamp[k] != t) {
+                hstamp[k] = t;
+                hkey[k]   = q;
                 touched.push_back(q);
               }
             }
@@ -461,13 +472,13 @@ void buildPattern(ik_handle* h) {
         std::sort(touched.begin(), touched.end());
         for (auto& q : touched) q = qord[q];
         const int ng = (int)touched.size();
-        for (int g = 0; g < ng; ++g) groupOf[touched[g]] = g;
+        for (int g = 0; g < ng; ++g) hval[slotOf(touched[g], t)] = g;
         std::fill(fill.begin(), fill.begin() + ng, 0);
         uint16_t* ep = epos.data() + t * kTileEntries;
         int maxr     = 0;
         for (int j = 0; j < kTileEntries; ++j) {
           if (j < 36 * nvalid) {
-            const int g  = groupOf[entQ[j]];
+            const int g  = hval[slotOf(entQ[j], t)];
             const int rk = fill[g]++;
             require(rk < 16, IK_EINVAL, "more than 16 element blocks of one tile on one K block");
             maxr  = std::max(maxr, rk + 1);
@@ -477,13 +488,10 @@ void buildPattern(ik_handle* h) {
           }
         }
         tmaxr[t] = (uint8_t)maxr;
-        for (const int32_t q : touched) {
-          out.push_back(q);
-          groupOf[q] = -1;
-        }
+        out.insert(out.end(), touched.begin(), touched.end());
         tng[t] = ng;
       }
-    });
+    }, 256);
     {
       int64_t tot = 0;
       for (int64_t t = 0; t < ntiles; ++t) {
@@ -492,9 +500,9 @@ void buildPattern(ik_handle* h) {
         tgptr[t + 1] = (int32_t)tot;
       }
       gq.resize(tot);
-      parallelChunks(ntiles, [&](int64_t t0, int64_t, int64_t ch) {
+      parallelChunks(ntiles, [&](int64_t t0, int64_t, int64_t ch) {  // the same chunks as the loop above
         std::copy(gpart[ch].begin(), gpart[ch].end(), gq.begin() + tgptr[t0]);
-      });
+      }, 256);
     }
     // An upper block that only one tile touches is complete after that tile's group sum, which the tile
     // writes straight into K (with the FULL mask and the mirror block). The others are staged, in gather
