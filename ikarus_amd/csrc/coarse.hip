@@ -54,6 +54,9 @@ namespace {
 
 constexpr int kTargetAggregates = 512;
 constexpr int kMaxAggregates    = 1024;  // the update's 2 na partials fit the first 2 kReduceBlocks entries
+// threads of the per-aggregate update workgroup (64^3 learned path: 14.3 us per launch with 256, 16.4 with 512,
+// 51.9 with 1024)
+constexpr int kUpdateThreads = 256;
 
 __device__ inline double waveSumD(double v) {
 #pragma unroll
@@ -280,10 +283,10 @@ __global__ __launch_bounds__(256) void coarseZ(const float* __restrict__ r, cons
   if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// N workgroup sums (4 wavefronts) at once, in a fixed order; the results are valid in every thread. One barrier
+// N workgroup sums (W wavefronts) at once, in a fixed order; the results are valid in every thread. One barrier
 // pair for all N (the wave reductions run side by side).
-template <int N>
-__device__ inline void groupSums(double (&v)[N], double (&red)[4][N]) {
+template <int W, int N>
+__device__ inline void groupSums(double (&v)[N], double (&red)[W][N]) {
 #pragma unroll
   for (int i = 0; i < N; ++i) v[i] = waveSumD(v[i]);
   if ((threadIdx.x & 63) == 0)
@@ -291,7 +294,12 @@ __device__ inline void groupSums(double (&v)[N], double (&red)[4][N]) {
     for (int i = 0; i < N; ++i) red[threadIdx.x >> 6][i] = v[i];
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < N; ++i) v[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+  for (int i = 0; i < N; ++i) {
+    double t = red[0][i];
+#pragma unroll
+    for (int w = 1; w < W; ++w) t += red[w][i];
+    v[i] = t;
+  }
   __syncthreads();
 }
 
@@ -300,7 +308,8 @@ __device__ inline void groupSums(double (&v)[N], double (&red)[4][N]) {
 // x += alpha p, r -= alpha q, and for the new r: the r.r and r.D^-1 r partials (partial[a], partial[na + a]) and
 // the restriction s_a = sum B(d_i)^T r_i (constrained components zeroed). With w = G s, r.z = r.D^-1 r + w.s,
 // since r . B(d_i) w_a summed over the nodes is w_a . s_a.
-__global__ __launch_bounds__(256) void coarseUpdate(float* __restrict__ x, float* __restrict__ r,
+template <int BS>
+__global__ __launch_bounds__(BS) void coarseUpdate(float* __restrict__ x, float* __restrict__ r,
                                                     const float* __restrict__ p, const float* __restrict__ q,
                                                     const float* __restrict__ binv, const uint8_t* __restrict__ fixed,
                                                     const float* __restrict__ dd, const int32_t* __restrict__ aggPtr,
@@ -308,12 +317,12 @@ __global__ __launch_bounds__(256) void coarseUpdate(float* __restrict__ x, float
                                                     const double* __restrict__ pq, int npq,
                                                     double* __restrict__ partial, PcgScalars* __restrict__ sc) {
   if (sc->done) return;
-  __shared__ double red1[4][1];
-  __shared__ double red8[4][8];
+  __shared__ double red1[BS / 64][1];
+  __shared__ double red8[BS / 64][8];
   const int a = blockIdx.x;
   double pqv[1] = {0.0};
-  for (int i = threadIdx.x; i < npq; i += 256) pqv[0] += pq[i];
-  groupSums<1>(pqv, red1);
+  for (int i = threadIdx.x; i < npq; i += BS) pqv[0] += pq[i];
+  groupSums<BS / 64, 1>(pqv, red1);
   const double rz = sc->rz_new;  // r.z of the current iterate (set by the previous p update or the init fold)
   const double alpha = rz / pqv[0];
   if (a == 0 && threadIdx.x == 0) {
@@ -324,7 +333,7 @@ __global__ __launch_bounds__(256) void coarseUpdate(float* __restrict__ x, float
   const float al = (float)alpha;
   double rr = 0.0, rzb = 0.0;
   float sa[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = aggPtr[a] + threadIdx.x; t < aggPtr[a + 1]; t += 256) {
+  for (int t = aggPtr[a] + threadIdx.x; t < aggPtr[a + 1]; t += BS) {
     const int64_t i = aggNodes[t];
     float rl[3], v[3], o[6];
 #pragma unroll
@@ -346,7 +355,7 @@ __global__ __launch_bounds__(256) void coarseUpdate(float* __restrict__ x, float
     for (int m = 0; m < 6; ++m) sa[m] += o[m];
   }
   double v[8] = {rr, rzb, sa[0], sa[1], sa[2], sa[3], sa[4], sa[5]};
-  groupSums<8>(v, red8);
+  groupSums<BS / 64, 8>(v, red8);
   if (threadIdx.x == 0) {
     partial[a]      = v[0];
     partial[na + a] = v[1];
@@ -399,7 +408,7 @@ __global__ __launch_bounds__(256) void coarseBetaP(float* __restrict__ p, const 
     f[1] += partial[na + i];
   }
   for (int i = threadIdx.x; i < nws; i += 256) f[2] += ws[i];
-  groupSums<3>(f, red3);
+  groupSums<4, 3>(f, red3);
   const double rr = f[0], rz = f[1] + f[2];
   const bool conv = rr < sc->threshold;
   const double bt = rz / sc->rz;
@@ -690,7 +699,8 @@ void coarseApply(ik_handle* h, const float* r, float* z, float* p, const float* 
 void coarseIteration(ik_handle* h, const double* pq, int npq, float* x, float* r, float* p, const float* q,
                      const float* binv, int gn, double* partial, int maxIter) {
   Coarse& cs = *h->coarse;
-  hipLaunchKernelGGL(coarseUpdate, dim3((unsigned)cs.na), dim3(256), 0, h->stream, x, r, p, q, binv, h->fixed.p,
+  hipLaunchKernelGGL(coarseUpdate<kUpdateThreads>, dim3((unsigned)cs.na), dim3(kUpdateThreads), 0, h->stream, x, r,
+                     p, q, binv, h->fixed.p,
                      cs.d.p, cs.aggPtr.p, cs.aggNodes.p, (int)cs.na, cs.s.p, pq, npq, partial, h->sc.p);
   const unsigned gg = (unsigned)((cs.nc + 3) / 4);
   hipLaunchKernelGGL(coarseGemvDot, dim3(gg), dim3(256), 0, h->stream, (const float4*)cs.G.p, (const float4*)cs.s.p,
