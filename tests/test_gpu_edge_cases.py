@@ -51,11 +51,22 @@ def test_thin_meshes_match_oracle(n, material, eas):
     cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 2, 0.0, 1.0, capi.PRED_CONSTANT
     info, iters, res = dev.load_control(cfg, ia.nr_settings(tol=1e-8, linear=ia.solver_opts(rtol=1e-13)))
     ug, _ = dev.get_requirement()
+    # the same path by the mixed form with the two-level preconditioner (aggregates of a few nodes, some modes
+    # dropped on these thin meshes)
+    dev.set_requirement(np.zeros(p.ndof), 0.0)
+    if eas:
+        dev.set_alpha(np.zeros((p.ne, eas)))
+    s2 = ia.nr_settings(tol=1e-8, precision=capi.FP32,
+                        linear=ia.solver_opts(kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+    info2, iters2, res2 = dev.load_control(cfg, s2)
+    u2, _ = dev.get_requirement()
     dev.close()
     o2 = _oracle_for(p)
     r = o2.load_control(2, 0.0, 1.0, tol=1e-8)
     assert info.success and r["success"] and list(iters) == list(r["iterations"])
     assert _rel(ug, r["u"]) < 1e-8
+    assert info2.success and res2[info2.solves - 1] <= 1e-8
+    assert _rel(u2, r["u"]) < 1e-8
 
 
 def test_fully_clamped_mesh():
@@ -82,5 +93,10 @@ def test_fully_clamped_mesh():
     cfg = capi.LCConfig()
     cfg.load_steps, cfg.tbegin, cfg.tend, cfg.predictor = 3, 0.0, 1.0, capi.PRED_LINEAR
     info, iters, res = dev.load_control(cfg, ia.nr_settings(tol=1e-8))
-    dev.close()
     assert info.success and np.all(iters == 0) and np.all(res == 0.0)
+    # two-level solve with every rigid-body mode dropped (no free DOF): the coarse part vanishes and K = I
+    dev.matrix(capi.DBC_FULL, capi.FP32)
+    b = np.linspace(1.0, 2.0, p.ndof)
+    x, sinfo = dev.solve(b, ia.solver_opts(kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, inner_rtol=1e-3))
+    dev.close()
+    assert sinfo.converged == 1 and _rel(x, b) < 1e-9
