@@ -27,7 +27,7 @@ EXPORTED = [
     "ik_matrix", "ik_assemble", "ik_matrix_csc", "ik_matrix_bsr",
     "ik_element", "ik_update_state", "ik_get_alpha", "ik_set_alpha", "ik_solve", "ik_newton_solve",
     "ik_load_control", "ik_path_following", "ik_mlp_num_params", "ik_mlp_set_params", "ik_mlp_get_params",
-    "ik_mlp_set_load",
+    "ik_mlp_set_load", "ik_mlp_predict",
     "ik_timing_enable", "ik_timing_read", "ik_timing_reset", "ik_synchronize", "ik_assembly_layout",
     "ik_mlp_create", "ik_mlp_destroy", "ik_mlp_last_error", "ik_mlp_params", "ik_mlp_load", "ik_mlp_forward",
     "ik_mlp_loss_grad", "ik_mlp_train_step", "ik_mlp_forward_noalloc", "ik_write_vtu",
@@ -127,6 +127,7 @@ def lib():
         L.ik_mlp_set_params.argtypes = [P, P, I]
         L.ik_mlp_get_params.argtypes = [P, P, I]
         L.ik_mlp_set_load.argtypes = [P, P, D]
+        L.ik_mlp_predict.argtypes = [P, P, P, D, D, D, D, I]
         L.ik_timing_enable.argtypes = [P, I]
         L.ik_timing_read.argtypes = [P, I, P, P]
         L.ik_timing_reset.argtypes = [P]

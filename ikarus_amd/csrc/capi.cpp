@@ -1168,6 +1168,31 @@ int ik_mlp_set_load(ik_handle* h, const double traction[3], double charLen) {
   });
 }
 
+int ik_mlp_predict(ik_handle* h, const double* du_n, const double* du_n1, double lam_next, double dlam,
+                   double dlam_prev, double dlam_prev2, int where) {
+  return guard(h, [&] {
+    require(h->mlpLoaded, IK_EINVAL, "no MLP parameters set");
+    requireBound(h);
+    require(!h->plane2d, IK_ENOTIMPL, "the learned predictor works on 3D handles");
+    require(du_n != nullptr && du_n1 != nullptr, IK_EINVAL, "null increment");
+    const double* dn  = du_n;
+    const double* dn1 = du_n1;
+    ik::DevBuf<double> hn, hn1;
+    if (where != IK_DEVICE) {
+      hn.ensure(h->ndof);
+      hn1.ensure(h->ndof);
+      copyIn(h, hn.p, du_n, h->ndof, IK_HOST);
+      copyIn(h, hn1.p, du_n1, h->ndof, IK_HOST);
+      dn  = hn.p;
+      dn1 = hn1.p;
+    }
+    ik::mlpPredict(h, dn, dn1, lam_next, dlam, dlam_prev, dlam_prev2);
+    h->kValid        = false;
+    h->easCacheValid = false;
+    IK_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
 int ik_timing_enable(ik_handle* h, int on) {
   return guard(h, [&] { h->timing = on != 0; });
 }

@@ -3,10 +3,11 @@
 // (loadcontrol.inl:55-57) / secant (pathfollowingfunctions.hh:188-195) displacement predictor, and
 // NewtonRaphson still has to reach the same ||R|| tolerance from the predicted state.
 //
-// All products run through one tiled GEMM on v_mfma_f32_16x16x16bf16_1k: operands are rounded to BF16
+// Training products run through one tiled GEMM on v_mfma_f32_16x16x16bf16_1k: operands are rounded to BF16
 // (round to nearest even) while staged into LDS, accumulation is FP32, parameters keep an FP32 master copy
 // updated by Adam. Strided operand access covers every product of training: Z = A W^T + b (forward),
-// dW = dZ^T A (weight gradient, split over the batch), dA = dZ W (back-propagation).
+// dW = dZ^T A (weight gradient, split over the batch), dA = dZ W (back-propagation). The prediction inside the
+// load loop runs in one fused kernel (features, four layers, update of u) on the same instruction and rounding.
 //
 // Features of node v for the step n -> n+1 (load increment dlam, velocities v_n = du_n / dlam):
 //   [ X_v / L (3), t / 40 (3), lam_{n+1}, 40 dlam, v_n / 0.1 (3), v_{n-1} / 0.1 (3) ]
@@ -30,13 +31,14 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16 bits, round to nearest even (NaN kept quiet)
-__device__ inline uint16_t toBf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// FP32 -> BF16 bits, round to nearest even (NaN kept quiet): v_cvt_pk_bf16_f32
+__device__ inline uint16_t toBf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// logistic function and SiLU; the reciprocal is v_rcp_f32 (1 ulp), 1 + e^-z >= 1 so it never sees a denormal
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid(z); }
 
 // EPI_SILU_ONLY: C = silu(A B + bias), the pre-activation is not kept (inference)
 // EPI_PARTIAL: slice z of a K split stores its partial product at C + z * M * ldc (summed later in z order)
@@ -105,15 +107,11 @@ __global__ __launch_bounds__(256) void gemmBf16(int M, int N, int K, const float
         } else {
           if constexpr (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS || EPI == EPI_SILU_ONLY) v += bias[j];
           if constexpr (EPI == EPI_SILU_ONLY) {
-            const float sg = 1.f / (1.f + __expf(-v));
-            C[(int64_t)i * ldc + j] = v * sg;
+            C[(int64_t)i * ldc + j] = silu(v);
           } else {
             C[(int64_t)i * ldc + j] = v;
           }
-          if constexpr (EPI == EPI_BIAS_SILU) {
-            const float sg = 1.f / (1.f + __expf(-v));
-            act[(int64_t)i * ldc + j] = v * sg;
-          }
+          if constexpr (EPI == EPI_BIAS_SILU) act[(int64_t)i * ldc + j] = silu(v);
         }
       }
 }
@@ -122,7 +120,7 @@ __global__ __launch_bounds__(256) void gemmBf16(int M, int N, int K, const float
 __global__ void siluBackward(const float* __restrict__ dA, const float* __restrict__ Z, float* __restrict__ dZ,
                              int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const float z = Z[i], sg = 1.f / (1.f + __expf(-z));
+    const float z = Z[i], sg = sigmoid(z);
     dZ[i]         = dA[i] * sg * (1.f + z * (1.f - sg));
   }
 }
@@ -180,33 +178,152 @@ __global__ void fillF(float* __restrict__ p, int64_t n, float v) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
 }
 
-// predictor features per node (see file header); velocities from the two previous increments
-__global__ void predictorFeatures(const double* __restrict__ X, const double* __restrict__ duN,
-                                  const double* __restrict__ duN1, int64_t nn, double invL, double t0, double t1,
-                                  double t2, double lamNext, double dlam, double dlamPrev, double dlamPrev2,
-                                  float* __restrict__ F) {
-  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nn) return;
-  float* f = F + v * kIn;
-  for (int k = 0; k < 3; ++k) f[k] = (float)(X[v * 3 + k] * invL);
-  f[3] = (float)(t0 * kTracScale);
-  f[4] = (float)(t1 * kTracScale);
-  f[5] = (float)(t2 * kTracScale);
-  f[6] = (float)lamNext;
-  f[7] = (float)(dlam * kDlamScale);
-  for (int k = 0; k < 3; ++k) {
-    f[8 + k]  = dlamPrev > 0 ? (float)(duN[v * 3 + k] / dlamPrev / kVelScale) : 0.f;
-    f[11 + k] = dlamPrev2 > 0 ? (float)(duN1[v * 3 + k] / dlamPrev2 / kVelScale) : 0.f;
+// BF16 copy of the network for the fused predictor, row-major output by input: W1 [128][16] (inputs past 14
+// zero), W2 and W3 [128][kWs], W4 [16][kWs] (rows past the 3 outputs zero). The rows of 128 inputs are padded to
+// kWs = 136 values (68 dwords), so the 16 rows one MFMA A fragment reads start on 16 different bank quadruples
+// instead of all on bank 0; the whole copy (kWbN values, 78 KB) is staged into LDS as it lies. Within each group
+// of 32 inputs 32 s .. 32 s + 31 the columns are stored in the order the 16x16x32 MFMA consumes them: lane group
+// g (lanes 16 g .. 16 g + 15) takes slots 8 g .. 8 g + 7 = inputs 32 s + 4 g + (0..3) and 32 s + 16 + 4 g + (0..3).
+constexpr int kWs = kHid + 8;
+constexpr int oWb1 = 0, oWb2 = oWb1 + kHid * 16, oWb3 = oWb2 + kHid * kWs, oWb4 = oWb3 + kHid * kWs,
+              kWbN = oWb4 + 16 * kWs;
+static_assert(kWbN % 8 == 0, "staged in 16-byte pieces");
+
+__global__ void packBf16Weights(const float* __restrict__ P, uint16_t* __restrict__ Wb) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kWbN) return;
+  float v = 0.f;
+  if (i < oWb2) {
+    const int j = i / 16, k = i % 16;
+    v = k < kIn ? P[oW1 + j * kIn + k] : 0.f;
+  } else {
+    const int l = i < oWb3 ? 0 : i < oWb4 ? 1 : 2;
+    const int o = i - (l == 0 ? oWb2 : l == 1 ? oWb3 : oWb4);
+    const int j = o / kWs, c = o % kWs;
+    if (c < kHid && (l < 2 || j < kOut)) {
+      const int g = (c % 32) / 8, q = c % 8;
+      const int k = c / 32 * 32 + (q < 4 ? 4 * g + q : 16 + 4 * g + q - 4);
+      v           = P[(l == 0 ? oW2 : l == 1 ? oW3 : oW4) + j * kHid + k];
+    }
   }
+  Wb[i] = toBf16(v);
 }
 
-// u <- u + dlam (v_n + 0.1 y) on free DOFs
-__global__ void predictorApply(double* __restrict__ u, const double* __restrict__ duN, const float* __restrict__ Y,
-                               const uint8_t* __restrict__ fixed, int64_t ndof, double dlam, double dlamPrev) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= ndof || fixed[i]) return;
-  const double vn = dlamPrev > 0 ? duN[i] / dlamPrev : 0.0;
-  u[i] += dlam * (vn + kVelScale * (double)Y[i]);
+// four (eight) consecutive BF16 values of a weight row in LDS, 8- (16-) byte aligned: an MFMA A fragment
+__device__ inline bf16x4 wfrag(const uint16_t* w) { return *reinterpret_cast<const bf16x4*>(w); }
+__device__ inline bf16x8 wfrag8(const uint16_t* w) { return *reinterpret_cast<const bf16x8*>(w); }
+// the 16x16x32 B fragment of K step s: this lane's outputs of tiles 2 s and 2 s + 1
+__device__ inline bf16x8 actPair(bf16x4 lo, bf16x4 hi) {
+  return __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+// The in-solver prediction in one launch: the per-node features (file header; velocities from the two previous
+// increments), the three SiLU layers, the output layer and u <- u + dlam (v_n + 0.1 y) on free DOFs, with the
+// activations kept in registers. A wavefront takes 16 nodes at a time and computes each layer transposed,
+// out^T (128 x 16) = W (128 x 128) act^T, with W as the MFMA A operand (from LDS) and the activations as the B
+// operand. Lane (n, kq) = (lane & 15, lane >> 4) holds node n's outputs 16 t + 4 kq + r of output tile t (C
+// fragment). Layer 1 (K = 16) runs on v_mfma_f32_16x16x16_bf16; layers 2-4 on v_mfma_f32_16x16x32_bf16, whose K
+// step s takes from this lane exactly its outputs of tiles 2 s and 2 s + 1 (the weight columns are stored in
+// that order): no data moves between lanes from layer to layer. The 8 output tiles of a layer are independent
+// accumulation chains, interleaved per K step. Same operand rounding (BF16, round to nearest even), FP32
+// accumulation, bias and SiLU as the layer-by-layer GEMMs (within a 32-input step the summation order differs).
+__global__ __launch_bounds__(256) void mlpPredictFused(const uint16_t* __restrict__ Wb, const float* __restrict__ P,
+                                                       const double* __restrict__ X, const double* __restrict__ duN,
+                                                       const double* __restrict__ duN1, int64_t nn, double invL,
+                                                       double t0, double t1, double t2, double lamNext, double dlam,
+                                                       double dlamPrev, double dlamPrev2, double* __restrict__ u,
+                                                       const uint8_t* __restrict__ fixed) {
+  __shared__ __align__(16) uint16_t sW[kWbN];  // 78 KB
+  __shared__ float sB[3 * kHid + 16];
+  for (int i = threadIdx.x; i < kWbN / 8; i += 256) reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(Wb)[i];
+  for (int i = threadIdx.x; i < 3 * kHid + 16; i += 256) {
+    float v = 0.f;
+    if (i < kHid) v = P[oB1 + i];
+    else if (i < 2 * kHid) v = P[oB2 + i - kHid];
+    else if (i < 3 * kHid) v = P[oB3 + i - 2 * kHid];
+    else if (i - 3 * kHid < kOut) v = P[oB4 + i - 3 * kHid];
+    sB[i] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, n = lane & 15, kq = lane >> 4;
+  const int64_t nblk = (nn + 15) / 16;
+  for (int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); blk < nblk; blk += (int64_t)gridDim.x * 4) {
+    const int64_t v = blk * 16 + n;
+    const bool valid = v < nn;
+    // input fragment: features 4 kq .. 4 kq + 3 of node v (as predictorFeatures)
+    float f[4] = {0.f, 0.f, 0.f, 0.f};
+    if (valid) {
+      if (kq == 0) {
+        f[0] = (float)(X[v * 3] * invL);
+        f[1] = (float)(X[v * 3 + 1] * invL);
+        f[2] = (float)(X[v * 3 + 2] * invL);
+        f[3] = (float)(t0 * kTracScale);
+      } else if (kq == 1) {
+        f[0] = (float)(t1 * kTracScale);
+        f[1] = (float)(t2 * kTracScale);
+        f[2] = (float)lamNext;
+        f[3] = (float)(dlam * kDlamScale);
+      } else if (kq == 2) {
+        for (int k = 0; k < 3; ++k) f[k] = dlamPrev > 0 ? (float)(duN[v * 3 + k] / dlamPrev / kVelScale) : 0.f;
+        f[3] = dlamPrev2 > 0 ? (float)(duN1[v * 3] / dlamPrev2 / kVelScale) : 0.f;
+      } else {
+        for (int k = 0; k < 2; ++k) f[k] = dlamPrev2 > 0 ? (float)(duN1[v * 3 + 1 + k] / dlamPrev2 / kVelScale) : 0.f;
+      }
+    }
+    bf16x4 act[8];
+    act[0] = bf16x4{(short)toBf16(f[0]), (short)toBf16(f[1]), (short)toBf16(f[2]), (short)toBf16(f[3])};
+    f32x4 acc[8];
+    // layer 1: K = 16 (one step)
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const bf16x4 a = wfrag(sW + oWb1 + (16 * t + n) * 16 + 4 * kq);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[0], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    }
+    // bias, SiLU, BF16: the next layer's B fragments
+    auto activate = [&](const float* bias) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        short q[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r] = (short)toBf16(silu(acc[t][r] + bias[16 * t + 4 * kq + r]));
+        act[t] = bf16x4{q[0], q[1], q[2], q[3]};
+      }
+    };
+    activate(sB);
+    // layers 2 and 3: 4 K steps of 32 x 8 independent output tiles
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const uint16_t* W = sW + (l == 0 ? oWb2 : oWb3);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 b = actPair(act[2 * ks], act[2 * ks + 1]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag8(W + (16 * t + n) * kWs + 32 * ks + 8 * kq), b,
+                                                           acc[t], 0, 0, 0);
+      }
+      activate(sB + (l + 1) * kHid);
+    }
+    // output layer: one tile (rows 0..2 of 16 used), 4 K steps of 32
+    f32x4 y = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag8(sW + oWb4 + n * kWs + 32 * ks + 8 * kq),
+                                                  actPair(act[2 * ks], act[2 * ks + 1]), y, 0, 0, 0);
+    // lanes kq = 0 hold outputs 0..3 of node n: u <- u + dlam (v_n + 0.1 y) on free DOFs
+    if (valid && kq == 0) {
+#pragma unroll
+      for (int k = 0; k < kOut; ++k) {
+        const int64_t i = v * 3 + k;
+        if (fixed[i]) continue;
+        const float yk   = y[k] + sB[3 * kHid + k];
+        const double vn  = dlamPrev > 0 ? duN[i] / dlamPrev : 0.0;
+        u[i] += dlam * (vn + kVelScale * (double)yk);
+      }
+    }
+  }
 }
 
 inline unsigned g1(int64_t n) { return (unsigned)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096); }
@@ -222,6 +339,7 @@ struct Mlp {
   DevBuf<float> P, G, Mo, Vo;          // params (fp32 master), grads, Adam moments
   DevBuf<float> Z[3], Aa[3], Y, dY, dA, dZ, Xin, Tin;
   DevBuf<float> Gpart;  // per-slice partial gradients, folded in a fixed order
+  DevBuf<uint16_t> Wb;  // BF16 weights of the fused predictor (packed when the parameters are set)
   DevBuf<double> partial;
   double* hred = nullptr;
   int64_t step = 0;
@@ -345,6 +463,8 @@ int64_t mlpNumParams() { return kParams; }
 void mlpSetParams(Mlp* m, const float* p, int where, hipStream_t s) {
   IK_HIP(hipMemcpyAsync(m->P.p, p, kParams * sizeof(float),
                         where == IK_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  m->Wb.ensure(kWbN);
+  hipLaunchKernelGGL(packBf16Weights, dim3((kWbN + 255) / 256), dim3(256), 0, s, m->P.p, m->Wb.p);
   IK_HIP(hipStreamSynchronize(s));
 }
 
@@ -360,14 +480,13 @@ void mlpPredict(ik_handle* h, const double* duN, const double* duN1, double lamN
   require(h->mlp != nullptr && h->mlpLoaded, IK_EINVAL, "MLP predictor selected but no parameters set");
   Mlp* m = h->mlp;
   m->s   = h->stream;
-  m->ensure(h->nn);
   ScopedKernelTimer t(h, KMLP);
-  hipLaunchKernelGGL(predictorFeatures, dim3((unsigned)((h->nn + 255) / 256)), dim3(256), 0, h->stream, h->X.p, duN,
-                     duN1, h->nn, 1.0 / h->mlpCharLen, h->mlpTraction[0], h->mlpTraction[1], h->mlpTraction[2],
-                     lamNext, dlam, dlamPrev, dlamPrev2, m->Xin.p);
-  m->forward(m->Xin.p, h->nn);
-  hipLaunchKernelGGL(predictorApply, dim3((unsigned)((h->ndof + 255) / 256)), dim3(256), 0, h->stream, h->u.p, duN,
-                     m->Y.p, h->fixed.p, h->ndof, dlam, dlamPrev);
+  // one fused launch; 2 workgroups of 80 KB LDS per CU, each wavefront looping over blocks of 16 nodes
+  const int64_t blocks = (h->nn + 63) / 64;
+  const unsigned grid  = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks, 512));
+  hipLaunchKernelGGL(mlpPredictFused, dim3(grid), dim3(256), 0, h->stream, m->Wb.p, m->P.p, h->X.p, duN, duN1, h->nn,
+                     1.0 / h->mlpCharLen, h->mlpTraction[0], h->mlpTraction[1], h->mlpTraction[2], lamNext, dlam,
+                     dlamPrev, dlamPrev2, h->u.p, h->fixed.p);
 }
 
 }  // namespace ik

@@ -288,6 +288,13 @@ int ik_mlp_get_params(ik_handle* h, float* params, int where);
 /* traction per unit load factor and length scale L used by the features (defaults: the problem's
  * traction, L = 1) */
 int ik_mlp_set_load(ik_handle* h, const double traction[3], double char_length);
+/* The prediction IK_PRED_MLP makes at the start of a load step, on its own: u <- u + dlam (v_n + 0.1 y) on the
+ * free DOFs of the bound requirement, with v_n = du_n / dlam_prev (0 if dlam_prev <= 0), v_{n-1} = du_n1 /
+ * dlam_prev2 (0 if dlam_prev2 <= 0) and y the network's output for lambda_{n+1} = lam_next. du_n, du_n1: the
+ * last two increments of u (ndof values each, host or device). One fused kernel; lambda is not changed.
+ * IK_EINVAL without parameters, IK_ENOTBOUND without requirement, IK_ENOTIMPL on plane-strain handles. */
+int ik_mlp_predict(ik_handle* h, const double* du_n, const double* du_n1, double lam_next, double dlam,
+                   double dlam_prev, double dlam_prev2, int where);
 
 /* Training context: forward, mean-squared-error gradient and Adam steps with the same kernels. */
 typedef struct ik_mlp ik_mlp;

@@ -146,3 +146,55 @@ def test_gradients_are_reproducible():
         m.train_step(x, t, 1e-3)
         m2.train_step(x, t, 1e-3)
     assert np.array_equal(m.params(), m2.params())
+
+
+@pytest.mark.parametrize("n", [(3, 4, 5), (33, 32, 32)])
+def test_fused_predictor_matches_oracle(n):
+    """ik_mlp_predict (features, four layers and the update of u in one kernel) against the oracle's BF16 forward
+    on the host-built features, for no, one and two previous increments: the network output recovered from the
+    update agrees to 2e-3 (as the layer-by-layer forward) and with ik_mlp_forward on the same features to 2e-4.
+    Both round the same operands to BF16, but the fused kernel sums each 32-input step in another FP32 order,
+    which now and then flips a BF16 rounding of an activation (3.7e-5 measured at 37026 nodes).
+    (33, 32, 32) has 37026 nodes, more than one pass of the 512-workgroup grid."""
+    p = pb.hex_block(n, jitter=0.05, seed=11)
+    print(f"case: fused predictor {n}, {p.X.shape[0]} nodes", flush=True)
+    m = learned.MLP(seed=6)
+    params = m.params()
+    rng = np.random.default_rng(12)
+    free = p.fixed == 0
+    u0 = 0.01 * rng.standard_normal(p.ndof) * free
+    du_n = 0.002 * rng.standard_normal(p.ndof) * free
+    du_n1 = 0.002 * rng.standard_normal(p.ndof) * free
+    traction, char_len, dl = np.array([0.0, -3.0, 12.0]), 2.0, 0.05
+    dev = ia.Device(p)
+    dev.check(capi.lib().ik_mlp_set_params(dev.h, capi.ptr(params), capi.HOST))
+    dev.check(capi.lib().ik_mlp_set_load(dev.h, capi.ptr(traction), char_len))
+    for dprev, dprev2 in ((0.0, 0.0), (dl, 0.0), (dl, dl)):
+        lam_next = 0.4 + dl
+        dev.set_requirement(u0, 0.4)
+        dev.check(capi.lib().ik_mlp_predict(dev.h, capi.ptr(du_n), capi.ptr(du_n1), lam_next, dl, dprev, dprev2,
+                                            capi.HOST))
+        ug, lam = dev.get_requirement()
+        assert lam == 0.4
+        assert np.array_equal(ug[~free], u0[~free])
+        vn = du_n / dprev if dprev > 0 else np.zeros(p.ndof)
+        y = ((ug - u0) / dl - vn) / 0.1
+        f = learned.features(p.X, traction, lam_next, dl, du_n, du_n1, dprev, dprev2, char_len)
+        yo, _, _ = mo.forward(params, f, bf16=True)
+        yt = m.forward(f)
+        yo, yt = yo.reshape(-1)[free], yt.reshape(-1)[free]
+        assert _rel(y[free], yo) < 2e-3
+        assert _rel(y[free], yt) < 2e-4
+    dev.close()
+
+
+def test_fused_predictor_errors():
+    p = pb.hex_block((2, 2, 2), seed=1)
+    print("case: fused predictor errors", flush=True)
+    dev = ia.Device(p)
+    z = np.zeros(p.ndof)
+    L = capi.lib()
+    assert L.ik_mlp_predict(dev.h, capi.ptr(z), capi.ptr(z), 0.1, 0.1, 0.0, 0.0, capi.HOST) == capi.IK_EINVAL
+    dev.check(L.ik_mlp_set_params(dev.h, capi.ptr(learned.MLP(seed=1).params()), capi.HOST))
+    assert L.ik_mlp_predict(dev.h, capi.ptr(z), capi.ptr(z), 0.1, 0.1, 0.0, 0.0, capi.HOST) == capi.IK_ENOTBOUND
+    dev.close()
