@@ -4,7 +4,10 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/ikarus_amd.h"
@@ -13,6 +16,30 @@ namespace ik {
 
 struct Mlp;
 struct Coarse;
+
+// Allocator whose value-less construct default-initialises: a HostVec made with a size (or resized) leaves its
+// elements unwritten, so the pattern build's large arrays are first touched by the parallel loops that fill
+// them instead of being zeroed on one thread beforehand. Construction with a value still writes it.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() noexcept = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using HostVec = std::vector<T, NoInitAlloc<T>>;
 
 struct Error {
   int code;
@@ -138,7 +165,7 @@ struct ik_handle {
 
   // 3x3 block CSR pattern (block row = node)
   bool patternBuilt = false;
-  std::vector<int32_t> hrowptr, hcolidx, hdiag;
+  ik::HostVec<int32_t> hrowptr, hcolidx, hdiag;
   int64_t nnzb = 0;
   ik::DevBuf<int32_t> rowptr, colidx, diagBlk;
   // the upper blocks (column >= row) the gather assembles, in gather order (staging chunk by staging chunk, by
@@ -182,7 +209,7 @@ struct ik_handle {
   // itself, with zero values). Every K-addressing index array (diagBlk, upBlk, upT, the direct groups' gdst and
   // gpt) holds slots.
   int64_t nsell = 0;
-  std::vector<int32_t> hpos;
+  ik::HostVec<int32_t> hpos;
   ik::DevBuf<int32_t> sellStart, scol;
   ik::DevBuf<int32_t> blockPos;  // hpos on the device, uploaded when the block view asks for it
   ik::DevBuf<double> K64;

@@ -60,6 +60,38 @@ static void parallelChunks(int64_t n, F&& f, int64_t grain = 2048) {
   if (err) std::rethrow_exception(err);
 }
 
+// out[i] = sum of w(k) over k < i, for i = 0 .. n (out has n + 1 entries): per-chunk sums, their prefix, then
+// each chunk's running sum; the same values for any thread count (integer sums)
+template <class W>
+static void parallelPrefix(int64_t n, W&& w, HostVec<int64_t>& out) {
+  out.clear();
+  out.resize(n + 1);
+  const int64_t T = chunkCount(n);
+  std::vector<int64_t> part(T + 1, 0);
+  parallelChunks(n, [&](int64_t i0, int64_t i1, int64_t c) {
+    int64_t s = 0;
+    for (int64_t i = i0; i < i1; ++i) s += w(i);
+    part[c + 1] = s;
+  });
+  for (int64_t c = 0; c < T; ++c) part[c + 1] += part[c];
+  parallelChunks(n, [&](int64_t i0, int64_t i1, int64_t c) {
+    int64_t s = part[c];
+    for (int64_t i = i0; i < i1; ++i) {
+      out[i] = s;
+      s += w(i);
+    }
+    if (i1 == n) out[n] = s;
+  });
+}
+
+// n copies of v, written by the build's threads
+template <class T>
+static HostVec<T> filled(int64_t n, T v) {
+  HostVec<T> b(n);
+  parallelChunks(n, [&](int64_t i0, int64_t i1, int64_t) { std::fill(b.data() + i0, b.data() + i1, v); });
+  return b;
+}
+
 // elements per staging chunk; IKARUS_AMD_CHUNK_ELEMENTS overrides it. Splitting the 64^3 mesh into chunks
 // whose gathers overlap the next chunk's element work measured slower than one launch of each (the two
 // kernels contend for the same memory path), so the default takes meshes up to 1 M elements in one chunk.
@@ -90,12 +122,12 @@ static uint64_t spread3(uint32_t v) {
 // On a structured grid the lattice is then the grid itself, so 16 consecutive elements form a 4x2x2 box
 // (45 nodes) whose 576 upper element blocks fall on 341 distinct K blocks; on other meshes the order still
 // keeps each tile compact.
-static std::vector<int32_t> mortonOrder(const ik_handle* h) {
+static HostVec<int32_t> mortonOrder(const ik_handle* h) {
   const int64_t ne = h->ne, nn = h->nn;
   const double* X  = h->hX.data();
   const int32_t* conn = h->hconn.data();
   double lo[3], step[3];
-  std::vector<double> ext(ne);
+  HostVec<double> ext(ne);
   for (int d = 0; d < 3; ++d) {
     lo[d] = X[d];
     for (int64_t n = 1; n < nn; ++n) lo[d] = std::min(lo[d], X[n * 3 + d]);
@@ -114,7 +146,7 @@ static std::vector<int32_t> mortonOrder(const ik_handle* h) {
     std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
     step[d] = ext[ne / 2] > 0.0 ? ext[ne / 2] : 1.0;
   }
-  std::vector<std::pair<uint64_t, int32_t>> key(ne);
+  HostVec<std::pair<uint64_t, int32_t>> key(ne);
   parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) {
     for (int64_t e = e0; e < e1; ++e) {
       uint64_t k = 0;
@@ -128,9 +160,25 @@ static std::vector<int32_t> mortonOrder(const ik_handle* h) {
       key[e] = {k, (int32_t)e};
     }
   });
-  std::sort(key.begin(), key.end());
-  std::vector<int32_t> order(ne);
-  for (int64_t i = 0; i < ne; ++i) order[i] = key[i].second;
+  // sorted runs per thread, then merged pairwise (keys are distinct: the element index breaks ties)
+  {
+    const int64_t T = chunkCount(ne);
+    std::vector<int64_t> b(T + 1);
+    for (int64_t c = 0; c <= T; ++c) b[c] = ne * c / T;
+    parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) { std::sort(key.begin() + e0, key.begin() + e1); });
+    for (int64_t w = 1; w < T; w *= 2) {
+      std::vector<std::thread> th;
+      for (int64_t c = 0; c + w < T; c += 2 * w)
+        th.emplace_back([&, c] {
+          std::inplace_merge(key.begin() + b[c], key.begin() + b[c + w], key.begin() + b[std::min(T, c + 2 * w)]);
+        });
+      for (auto& x : th) x.join();
+    }
+  }
+  HostVec<int32_t> order(ne);
+  parallelChunks(ne, [&](int64_t i0, int64_t i1, int64_t) {
+    for (int64_t i = i0; i < i1; ++i) order[i] = key[i].second;
+  });
   return order;
 }
 
@@ -143,7 +191,7 @@ void buildPattern(ik_handle* h) {
     for (int a = 0; a < 8; ++a) cnt[conn[e * 8 + a] + 1]++;
   for (int64_t i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
   require(cnt[nn] < (int64_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 incidence lists");
-  std::vector<int32_t> nlist(cnt[nn]);
+  HostVec<int32_t> nlist(cnt[nn]);
   {
     std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
     for (int64_t e = 0; e < ne; ++e)
@@ -154,7 +202,7 @@ void buildPattern(ik_handle* h) {
   h->hrowptr.assign(nn + 1, 0);
   {
     std::vector<std::vector<int32_t>> part(chunkCount(nn));
-    std::vector<int64_t> rc(nn);
+    HostVec<int64_t> rc(nn);
     parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t c) {
       std::vector<int32_t>& cols = part[c];
       cols.reserve((size_t)(i1 - i0) * 27);
@@ -177,6 +225,7 @@ void buildPattern(ik_handle* h) {
       require(tot < (int64_t)INT32_MAX, IK_EINVAL, "pattern too large for int32 block indices");
       h->hrowptr[i + 1] = (int32_t)tot;
     }
+    h->hcolidx.clear();
     h->hcolidx.resize(tot);
     parallelChunks(nn, [&](int64_t i0, int64_t, int64_t c) {
       std::copy(part[c].begin(), part[c].end(), h->hcolidx.begin() + h->hrowptr[i0]);
@@ -207,8 +256,9 @@ void buildPattern(ik_handle* h) {
     }
   }
   h->nsell = sstart[nslices];
-  h->hpos.assign(h->nnzb, 0);
-  std::vector<int32_t> scol(h->nsell);
+  h->hpos.clear();
+  h->hpos.resize(h->nnzb);  // every block gets its slot below
+  HostVec<int32_t> scol(h->nsell);
   parallelChunks(nslices, [&](int64_t s0, int64_t s1, int64_t) {
     for (int64_t s = s0; s < s1; ++s) {
       const int32_t w = (sstart[s + 1] - sstart[s]) / kSellRows;
@@ -230,7 +280,7 @@ void buildPattern(ik_handle* h) {
   {
     // every block in its row's slice, at a slot of its own holding its column (slices are disjoint slot ranges,
     // so each thread marks only its own slices' slots)
-    std::vector<uint8_t> seen(h->nsell, 0);
+    HostVec<uint8_t> seen = filled<uint8_t>(h->nsell, 0);
     parallelChunks(nslices, [&](int64_t s0, int64_t s1, int64_t) {
       for (int64_t i = s0 * kSellRows; i < std::min(nn, s1 * kSellRows); ++i)
         for (int32_t b = h->hrowptr[i]; b < h->hrowptr[i + 1]; ++b) {
@@ -250,7 +300,7 @@ void buildPattern(ik_handle* h) {
   // ordered by staging unit (tile or element) inside each group, so the gather reads contiguous segments
   // and sums them in a fixed order.
   require((int64_t)ne * 36 < (int64_t)INT32_MAX / 2, IK_EINVAL, "mesh too large for int32 staging slots");
-  std::vector<int32_t> slots((size_t)ne * 36);  // block index p of each element block
+  HostVec<int32_t> slots((size_t)ne * 36);  // block index p of each element block
   parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) {
     for (int64_t e = e0; e < e1; ++e)
       for (int a = 0; a < 8; ++a)
@@ -265,7 +315,7 @@ void buildPattern(ik_handle* h) {
   // upper blocks and their transposes
   std::vector<int64_t> uptr(nn + 1, 0);  // first upper block of each row
   for (int64_t i = 0; i < nn; ++i) uptr[i + 1] = uptr[i] + (h->hrowptr[i + 1] - h->hdiag[i]);
-  std::vector<int32_t> up(uptr[nn]), upT(uptr[nn]);
+  HostVec<int32_t> up(uptr[nn]), upT(uptr[nn]);
   parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
     for (int64_t i = i0; i < i1; ++i)
       for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
@@ -277,12 +327,12 @@ void buildPattern(ik_handle* h) {
       }
   });
   const int64_t nup = uptr[nn];
-  std::vector<int32_t> qOf(h->nnzb, -1);
+  HostVec<int32_t> qOf = filled<int32_t>(h->nnzb, -1);
   parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
     for (int64_t q = q0; q < q1; ++q) qOf[up[q]] = (int32_t)q;
   });
   // FULL Dirichlet treatment per upper block: bit 3k+l zeroes entry (k,l), bit 9+k sets a unit diagonal
-  std::vector<uint16_t> mask(nup, 0);
+  HostVec<uint16_t> mask(nup);
   parallelChunks(nn, [&](int64_t i0, int64_t i1, int64_t) {
     for (int64_t i = i0; i < i1; ++i)
       for (int32_t p = h->hdiag[i]; p < h->hrowptr[i + 1]; ++p) {
@@ -297,15 +347,15 @@ void buildPattern(ik_handle* h) {
         mask[uptr[i] + (p - h->hdiag[i])] = m;
       }
   });
-  std::vector<uint8_t> gathered(nup, 1);  // upper blocks the gather assembles (the rest are written directly)
-  std::vector<int64_t> unitOf(ne);  // staging unit of each element: its tile (tiled) or itself
+  HostVec<uint8_t> gathered = filled<uint8_t>(nup, 1);  // upper blocks the gather assembles (the rest are written directly)
+  HostVec<int64_t> unitOf(ne);  // staging unit of each element: its tile (tiled) or itself
   h->tiled = h->neas == 0;
   int64_t ntiles = 0;
-  std::vector<int32_t> tileElem;
+  HostVec<int32_t> tileElem;
   if (h->tiled) {
     // plain element: tiles of kTileElems, consecutive runs of the Morton order (compact boxes), then sorted by
     // their lowest node, so tiles running at the same time write neighbouring parts of the segments.
-    const std::vector<int32_t> order = mortonOrder(h);
+    const HostVec<int32_t> order = mortonOrder(h);
     ntiles = (ne + kTileElems - 1) / kTileElems;
     std::vector<std::pair<int32_t, int32_t>> tkey(ntiles);  // (lowest node, Morton tile)
     parallelChunks(ntiles, [&](int64_t m0, int64_t m1, int64_t) {
@@ -317,7 +367,7 @@ void buildPattern(ik_handle* h) {
       }
     });
     std::sort(tkey.begin(), tkey.end());
-    tileElem.assign((size_t)ntiles * kTileElems, -1);
+    tileElem = filled<int32_t>(ntiles * kTileElems, -1);
     parallelChunks(ntiles, [&](int64_t t0, int64_t t1, int64_t) {
       for (int64_t t = t0; t < t1; ++t) {
         const int64_t m = tkey[t].second;
@@ -359,10 +409,11 @@ void buildPattern(ik_handle* h) {
   // Gather order of the upper blocks: chunk by chunk (the rows each chunk completes), inside a chunk by storage
   // slot. For a fixed j the rows of a slice have adjacent slots, so the gather's consecutive lanes, and a tile's
   // consecutive groups, write neighbouring slots. Staging segments follow this order; qrank[q] = position of q.
-  std::vector<int32_t> qord, qrank(nup, -1);
+  std::vector<int32_t> qord;
+  HostVec<int32_t> qrank = filled<int32_t>(nup, -1);
   std::vector<int64_t> qchunk(nchunks + 1, 0);  // qord[qchunk[c] .. qchunk[c+1]) = upper blocks of chunk c
   {
-    std::vector<int32_t> qAtSlot(h->nsell, -1);
+    HostVec<int32_t> qAtSlot = filled<int32_t>(h->nsell, -1);
     parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
       for (int64_t q = q0; q < q1; ++q) qAtSlot[h->hpos[up[q]]] = (int32_t)q;
     });
@@ -391,14 +442,15 @@ void buildPattern(ik_handle* h) {
     require((int64_t)qord.size() == nup, IK_EINVAL, "internal error: gather order misses upper blocks");
   }
   // staging segment of upper block q: [segBeg[q], segBeg[q] + segCnt[q]), segments laid out in gather order
-  std::vector<int32_t> segBeg(nup, 0), segCnt(nup, 0);
+  HostVec<int32_t> segBeg(nup), segCnt(nup);  // both written for every upper block below
   auto layoutSegments = [&]() {
-    int64_t tot = 0;
-    for (const int32_t q : qord) {
-      segBeg[q] = (int32_t)tot;
-      tot += segCnt[q];
-      require(tot < (int64_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
-    }
+    HostVec<int64_t> beg;
+    parallelPrefix((int64_t)qord.size(), [&](int64_t k) { return (int64_t)segCnt[qord[k]]; }, beg);
+    const int64_t tot = beg.back();
+    require(tot < (int64_t)INT32_MAX, IK_EINVAL, "mesh too large for int32 staging slots");
+    parallelChunks((int64_t)qord.size(), [&](int64_t k0, int64_t k1, int64_t) {
+      for (int64_t k = k0; k < k1; ++k) segBeg[qord[k]] = (int32_t)beg[k];
+    });
     return tot;
   };
   if (!h->tiled) {
@@ -408,7 +460,7 @@ void buildPattern(ik_handle* h) {
     for (int64_t q = 0; q < nup; ++q) segCnt[q] = kcnt[up[q]];
     h->nstage = layoutSegments();
     std::vector<int32_t> dest((size_t)ne * 36);
-    std::vector<int32_t> pos(segBeg);
+    std::vector<int32_t> pos(segBeg.begin(), segBeg.end());
     for (int64_t e = 0; e < ne; ++e)
       for (int a = 0; a < 8; ++a)
         for (int c = a; c < 8; ++c) {
@@ -427,9 +479,10 @@ void buildPattern(ik_handle* h) {
     // epos[t*576 + j] = group | rank << 10 | transposed << 15; padding entries target spare accumulators
     // ng + (j - 36 nvalid), rank 0, which the tile never stores. tmaxr[t] = the largest group size.
     // Each chunk of tiles lists its groups' upper blocks in its own buffer; tgptr places them.
-    std::vector<uint16_t> epos((size_t)ntiles * kTileEntries);
+    HostVec<uint16_t> epos((size_t)ntiles * kTileEntries);
     std::vector<uint8_t> tmaxr(ntiles);
-    std::vector<int32_t> tgptr(ntiles + 1, 0), gq;
+    std::vector<int32_t> tgptr(ntiles + 1, 0);
+    HostVec<int32_t> gq;
     std::vector<int64_t> tng(ntiles);
     // a tile's upper blocks -> group: open-addressing table of 2048 slots (a tile touches at most 576 upper blocks),
     // a slot valid when its stamp is the current tile
@@ -508,7 +561,7 @@ void buildPattern(ik_handle* h) {
     // writes straight into K (with the FULL mask and the mirror block). The others are staged, in gather
     // order: segments in the order qord, each ordered by tile. gdst[g] = slot, or -(K block + 1) with
     // gpt[g] = the mirror block and gmsk[g] = the mask of a direct group.
-    std::vector<int32_t> ntile(nup, 0);
+    HostVec<int32_t> ntile = filled<int32_t>(nup, 0);
     parallelChunks((int64_t)gq.size(), [&](int64_t g0, int64_t g1, int64_t) {
       for (int64_t g = g0; g < g1; ++g) __atomic_fetch_add(&ntile[gq[g]], 1, __ATOMIC_RELAXED);
     });
@@ -519,35 +572,55 @@ void buildPattern(ik_handle* h) {
       }
     });
     h->nstage = layoutSegments();
-    std::vector<int32_t> pos(segBeg), gdst(gq.size()), gpt(gq.size(), 0);
-    std::vector<uint16_t> gmsk(gq.size(), 0);
-    for (size_t g = 0; g < gq.size(); ++g) {
-      const int32_t q = gq[g];
-      if (gathered[q]) {
-        gdst[g] = pos[q]++;
-      } else {
-        gdst[g] = -(up[q] + 1);
-        gpt[g]  = upT[q];
-        gmsk[g] = mask[q];
+    // A staged group's slot is its upper block's segment start plus its rank among that block's groups by
+    // increasing group index (= tile order). Groups are dropped into their segment through a per-block cursor
+    // (any order), each segment (at most 8 groups: one per tile touching the block) is then sorted, and the
+    // k-th group of segment q gets slot segBeg[q] + k. A direct group stores -(K slot + 1), its mirror's K
+    // slot and its FULL mask.
+    const int64_t ngq = (int64_t)gq.size();
+    HostVec<int32_t> gdst(ngq), gpt(ngq), cursor = filled<int32_t>(nup, 0), segG(h->nstage);
+    HostVec<uint16_t> gmsk(ngq);
+    parallelChunks(ngq, [&](int64_t g0, int64_t g1, int64_t) {
+      for (int64_t g = g0; g < g1; ++g) {
+        const int32_t q = gq[g];
+        if (gathered[q]) {
+          segG[segBeg[q] + __atomic_fetch_add(&cursor[q], 1, __ATOMIC_RELAXED)] = (int32_t)g;
+          gpt[g]  = 0;
+          gmsk[g] = 0;
+        } else {
+          gdst[g] = -(h->hpos[up[q]] + 1);
+          gpt[g]  = h->hpos[upT[q]];
+          gmsk[g] = mask[q];
+        }
       }
-    }
+    });
+    parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
+      for (int64_t q = q0; q < q1; ++q) {
+        if (!gathered[q]) continue;
+        require(cursor[q] == segCnt[q], IK_EINVAL, "internal error: staging segment not filled");
+        int32_t* b = segG.data() + segBeg[q];
+        std::sort(b, b + segCnt[q]);
+        for (int32_t k = 0; k < segCnt[q]; ++k) gdst[b[k]] = segBeg[q] + k;
+      }
+    });
     // the kernel's indexing relies on these: in every tile each valid entry has a distinct (group, rank)
     // with group < ng and rank < its group's size < tmaxr + 1, every group is non-empty, padding entries
     // target spare accumulators in [ng, 576); the slots are a permutation of [0, nstage) with each group
-    // inside its own upper block's segment (the segments are disjoint consecutive ranges, each slot handed out
-    // by a counter that ends exactly at its segment's end), and every upper block is either staged by two or
-    // more tiles or written by exactly one
+    // inside its own upper block's segment (the segments are disjoint consecutive ranges, each filled exactly
+    // by its cursor), and every upper block is either staged by two or more tiles or written by exactly one
     {
       parallelChunks(nup, [&](int64_t q0, int64_t q1, int64_t) {
         for (int64_t q = q0; q < q1; ++q)
-          require(ntile[q] >= 1 && segCnt[q] == (gathered[q] ? ntile[q] : 0) && pos[q] == segBeg[q] + segCnt[q],
+          require(ntile[q] >= 1 && segCnt[q] == (gathered[q] ? ntile[q] : 0) &&
+                      cursor[q] == (gathered[q] ? segCnt[q] : 0),
                   IK_EINVAL, "internal error: upper block without a tile or a segment not filled");
       });
       parallelChunks((int64_t)gq.size(), [&](int64_t g0, int64_t g1, int64_t) {
         for (int64_t g = g0; g < g1; ++g) {
           const int32_t sl = gdst[g], q = gq[g];
           if (sl < 0)
-            require(!gathered[q] && -sl - 1 == up[q], IK_EINVAL, "internal error: bad direct group");
+            require(!gathered[q] && -sl - 1 == h->hpos[up[q]] && gpt[g] == h->hpos[upT[q]], IK_EINVAL,
+                    "internal error: bad direct group");
           else
             require(sl >= segBeg[q] && sl < segBeg[q] + segCnt[q], IK_EINVAL,
                     "internal error: staging slot outside its segment");
@@ -583,12 +656,6 @@ void buildPattern(ik_handle* h) {
       });
     }
     h->ntiles = ntiles;
-    // K blocks of the direct groups as storage slots
-    for (size_t g = 0; g < gq.size(); ++g)
-      if (gdst[g] < 0) {
-        gdst[g] = -(h->hpos[-gdst[g] - 1] + 1);
-        gpt[g]  = h->hpos[gpt[g]];
-      }
     h->tileElem.upload(tileElem.data(), tileElem.size(), h->stream);
     h->tgptr.upload(tgptr.data(), tgptr.size(), h->stream);
     h->gdst.upload(gdst.data(), gdst.size(), h->stream);
@@ -597,23 +664,33 @@ void buildPattern(ik_handle* h) {
     h->epos.upload(epos.data(), epos.size(), h->stream);
     h->tmaxr.upload(tmaxr.data(), tmaxr.size(), h->stream);
   }
-  // the gather's upper blocks: the gathered ones in gather order (chunkQ[c]: first one of chunk c)
+  // the gather's upper blocks: the gathered ones in gather order (chunkQ[c]: first one of chunk c), placed by
+  // a prefix count over qord; each one's segment must start where the previous one's ends
   {
-    std::vector<int32_t> cup, cupT, cseg(1, 0);
-    std::vector<uint16_t> cmask;
-    h->chunkQ.assign(1, 0);
-    for (int64_t c = 0; c < nchunks; ++c) {
-      for (int64_t k = qchunk[c]; k < qchunk[c + 1]; ++k) {
+    HostVec<int64_t> at;
+    parallelPrefix(nup, [&](int64_t k) { return (int64_t)gathered[qord[k]]; }, at);
+    const int64_t ng = at[nup];
+    HostVec<int32_t> cup(ng), cupT(ng), cseg(ng + 1);
+    HostVec<uint16_t> cmask(ng);
+    cseg[0] = 0;
+    parallelChunks(nup, [&](int64_t k0, int64_t k1, int64_t) {
+      for (int64_t k = k0; k < k1; ++k) {
         const int32_t q = qord[k];
         if (!gathered[q]) continue;
-        cup.push_back(h->hpos[up[q]]);
-        cupT.push_back(h->hpos[upT[q]]);
-        cmask.push_back(mask[q]);
-        cseg.push_back(segBeg[q] + segCnt[q]);
-        require(segBeg[q] == cseg[cseg.size() - 2], IK_EINVAL, "internal error: gather segments not contiguous");
+        const int64_t i = at[k];
+        cup[i]          = h->hpos[up[q]];
+        cupT[i]         = h->hpos[upT[q]];
+        cmask[i]        = mask[q];
+        cseg[i + 1]     = segBeg[q] + segCnt[q];
       }
-      h->chunkQ.push_back((int64_t)cup.size());
-    }
+    });
+    parallelChunks(nup, [&](int64_t k0, int64_t k1, int64_t) {
+      for (int64_t k = k0; k < k1; ++k)
+        if (gathered[qord[k]])
+          require(segBeg[qord[k]] == cseg[at[k]], IK_EINVAL, "internal error: gather segments not contiguous");
+    });
+    h->chunkQ.assign(nchunks + 1, 0);
+    for (int64_t c = 0; c < nchunks; ++c) h->chunkQ[c + 1] = at[qchunk[c + 1]];
     h->upMask.upload(cmask.data(), cmask.size(), h->stream);
     h->upBlk.upload(cup.data(), cup.size(), h->stream);
     h->upT.upload(cupT.data(), cupT.size(), h->stream);
@@ -628,7 +705,7 @@ void buildPattern(ik_handle* h) {
     h->diagBlk.upload(dslot.data(), dslot.size(), h->stream);
   }
   // residual staging slot of every element corner: its position in the node's incidence list
-  std::vector<int32_t> rdest((size_t)ne * 8);
+  HostVec<int32_t> rdest((size_t)ne * 8);
   parallelChunks((int64_t)nlist.size(), [&](int64_t j0, int64_t j1, int64_t) {
     for (int64_t j = j0; j < j1; ++j) rdest[nlist[j]] = (int32_t)j;
   });

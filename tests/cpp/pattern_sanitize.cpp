@@ -94,8 +94,8 @@ void put(std::string& out, const ik::DevBuf<T>& b) {
   out.append(reinterpret_cast<const char*>(&n), sizeof(n));
   if (n) out.append(reinterpret_cast<const char*>(b.p), n * sizeof(T));
 }
-template <class T>
-void put(std::string& out, const std::vector<T>& v) {
+template <class T, class A>
+void put(std::string& out, const std::vector<T, A>& v) {
   const uint64_t n = v.size();
   out.append(reinterpret_cast<const char*>(&n), sizeof(n));
   if (n) out.append(reinterpret_cast<const char*>(v.data()), n * sizeof(T));
