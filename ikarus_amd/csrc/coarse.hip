@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "host_parallel.hh"
 #include "ik_internal.hh"
 
 namespace ik {
@@ -480,31 +481,12 @@ void jacobiEigen6(double a[6][6], double v[6][6]) {
 
 // aggregates, node offsets, orthonormalising transforms and aggregate adjacency (host, once per handle)
 void buildAggregates(ik_handle* h, Coarse& cs) {
-  const int64_t nn = h->nn, ne = h->ne;
+  const int64_t nn = h->nn;
   const double* X = h->hX.data();
-  const int32_t* conn = h->hconn.data();
-  // median element extent per axis (largest component of the element's mean edge vectors)
-  double lo[3], hi[3], step[3];
-  std::vector<double> ext(ne);
-  for (int d = 0; d < 3; ++d) {
-    lo[d] = hi[d] = X[d];
-    for (int64_t n = 1; n < nn; ++n) {
-      lo[d] = std::min(lo[d], X[n * 3 + d]);
-      hi[d] = std::max(hi[d], X[n * 3 + d]);
-    }
-    for (int64_t e = 0; e < ne; ++e) {
-      double best = 0.0;
-      for (int m = 0; m < 3; ++m) {
-        double v = 0.0;
-        for (int a = 0; a < 8; ++a)
-          if (!((a >> m) & 1)) v += X[conn[e * 8 + (a | (1 << m))] * 3 + d] - X[conn[e * 8 + a] * 3 + d];
-        best = std::max(best, std::fabs(0.25 * v));
-      }
-      ext[e] = best;
-    }
-    std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
-    step[d] = ext[ne / 2] > 0.0 ? ext[ne / 2] : 1.0;
-  }
+  // boxes of a x a x a cells of the mesh lattice (node bounds, median element extent per axis)
+  const double* lo = h->latLo;
+  const double* hi = h->latHi;
+  const double* step = h->latStep;
   auto boxes = [&](int a, int64_t nb[3]) {
     for (int d = 0; d < 3; ++d) nb[d] = std::max<int64_t>(1, (int64_t)std::floor((hi[d] - lo[d]) / (a * step[d]) + 1e-9));
     return nb[0] * nb[1] * nb[2];
@@ -513,95 +495,118 @@ void buildAggregates(ik_handle* h, Coarse& cs) {
   int64_t nb[3];
   while (boxes(a, nb) > kMaxAggregates) ++a;
   // box of each node, then compact aggregate ids in box order
-  std::vector<int64_t> key(nn);
-  for (int64_t n = 0; n < nn; ++n) {
-    int64_t k[3];
-    for (int d = 0; d < 3; ++d)
-      k[d] = std::min<int64_t>(nb[d] - 1, std::max<int64_t>(0, (int64_t)std::floor((X[n * 3 + d] - lo[d]) / (a * step[d]) + 1e-9)));
-    key[n] = (k[2] * nb[1] + k[1]) * nb[0] + k[0];
-  }
-  std::vector<int32_t> boxId(nb[0] * nb[1] * nb[2], -1), agg(nn);
+  HostVec<int64_t> key(nn);
+  parallelChunks(nn, [&](int64_t n0, int64_t n1, int64_t) {
+    for (int64_t n = n0; n < n1; ++n) {
+      int64_t k[3];
+      for (int d = 0; d < 3; ++d)
+        k[d] = std::min<int64_t>(nb[d] - 1,
+                                 std::max<int64_t>(0, (int64_t)std::floor((X[n * 3 + d] - lo[d]) / (a * step[d]) + 1e-9)));
+      key[n] = (k[2] * nb[1] + k[1]) * nb[0] + k[0];
+    }
+  });
+  std::vector<int32_t> boxId(nb[0] * nb[1] * nb[2], -1);
+  HostVec<int32_t> agg(nn);
   int32_t na = 0;
   for (int64_t n = 0; n < nn; ++n)
     if (boxId[key[n]] < 0) boxId[key[n]] = 0;
   for (auto& b : boxId)
     if (b == 0) b = na++;
-  for (int64_t n = 0; n < nn; ++n) agg[n] = boxId[key[n]];
-  std::vector<int32_t> ptr(na + 1, 0), nodes(nn);
+  parallelChunks(nn, [&](int64_t n0, int64_t n1, int64_t) {
+    for (int64_t n = n0; n < n1; ++n) agg[n] = boxId[key[n]];
+  });
+  // nodes of each aggregate, ascending
+  std::vector<int32_t> ptr(na + 1, 0);
+  HostVec<int32_t> nodes(nn);
   for (int64_t n = 0; n < nn; ++n) ptr[agg[n] + 1]++;
   for (int32_t k = 0; k < na; ++k) ptr[k + 1] += ptr[k];
   {
     std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
     for (int64_t n = 0; n < nn; ++n) nodes[pos[agg[n]]++] = (int32_t)n;
   }
-  // centroids and offsets
+  // centroids (node sums in ascending node order) and offsets
   std::vector<double> cen((size_t)na * 3, 0.0);
-  for (int64_t n = 0; n < nn; ++n)
-    for (int d = 0; d < 3; ++d) cen[agg[n] * 3 + d] += X[n * 3 + d];
-  for (int32_t k = 0; k < na; ++k)
-    for (int d = 0; d < 3; ++d) cen[k * 3 + d] /= (ptr[k + 1] - ptr[k]);
-  std::vector<float> dd(nn * 3);
-  for (int64_t n = 0; n < nn; ++n)
-    for (int d = 0; d < 3; ++d) dd[n * 3 + d] = (float)(X[n * 3 + d] - cen[agg[n] * 3 + d]);
+  parallelChunks(na, [&](int64_t k0, int64_t k1, int64_t) {
+    for (int64_t k = k0; k < k1; ++k) {
+      for (int32_t t = ptr[k]; t < ptr[k + 1]; ++t)
+        for (int d = 0; d < 3; ++d) cen[k * 3 + d] += X[(int64_t)nodes[t] * 3 + d];
+      for (int d = 0; d < 3; ++d) cen[k * 3 + d] /= (ptr[k + 1] - ptr[k]);
+    }
+  }, 16);
+  HostVec<float> dd(nn * 3);
+  parallelChunks(nn, [&](int64_t n0, int64_t n1, int64_t) {
+    for (int64_t n = n0; n < n1; ++n)
+      for (int d = 0; d < 3; ++d) dd[n * 3 + d] = (float)(X[n * 3 + d] - cen[agg[n] * 3 + d]);
+  });
   // T_a from the Gram matrix of the masked modes (evaluated with the same float offsets the device uses)
   std::vector<double> T((size_t)na * 36, 0.0);
   std::vector<uint8_t> drop((size_t)na * 6, 0);
-  for (int32_t k = 0; k < na; ++k) {
-    double g[6][6] = {}, v[6][6];
-    for (int32_t t = ptr[k]; t < ptr[k + 1]; ++t) {
-      const int32_t n = nodes[t];
-      const double dx = dd[n * 3], dy = dd[n * 3 + 1], dz = dd[n * 3 + 2];
-      const double Bm[3][6] = {{1, 0, 0, 0, dz, -dy}, {0, 1, 0, -dz, 0, dx}, {0, 0, 1, dy, -dx, 0}};
-      for (int r = 0; r < 3; ++r) {
-        if (h->hfixed[3 * n + r]) continue;
-        for (int i = 0; i < 6; ++i)
-          for (int j = 0; j < 6; ++j) g[i][j] += Bm[r][i] * Bm[r][j];
-      }
-    }
-    double g0[6][6];
-    std::copy(&g[0][0], &g[0][0] + 36, &g0[0][0]);
-    jacobiEigen6(g, v);
-    double lmax = 0.0;
-    for (int i = 0; i < 6; ++i) lmax = std::max(lmax, g[i][i]);
-    for (int j = 0; j < 6; ++j) {
-      const double l = g[j][j];
-      if (!(lmax > 0.0) || !(l > 1e-10 * lmax)) {
-        drop[k * 6 + j] = 1;
-        continue;
-      }
-      for (int i = 0; i < 6; ++i) T[k * 36 + i * 6 + j] = v[i][j] / std::sqrt(l);
-    }
-    // the kept modes are orthonormal over the aggregate: T^T G T = I on them
-    for (int p = 0; p < 6; ++p)
-      for (int q = 0; q < 6; ++q) {
-        if (drop[k * 6 + p] || drop[k * 6 + q]) continue;
-        double e = 0.0;
-        for (int i = 0; i < 6; ++i)
-          for (int j = 0; j < 6; ++j) e += T[k * 36 + i * 6 + p] * g0[i][j] * T[k * 36 + j * 6 + q];
-        require(std::fabs(e - (p == q ? 1.0 : 0.0)) < 1e-6, IK_EINVAL,
-                "internal error: aggregate modes not orthonormalised");
-      }
-  }
-  // aggregate adjacency from the block pattern
-  std::vector<int32_t> nptr(na + 1, 0), nlist;
-  {
-    std::vector<int32_t> mark(na, -1), tmp;
-    for (int32_t k = 0; k < na; ++k) {
-      tmp.clear();
+  parallelChunks(na, [&](int64_t k0, int64_t k1, int64_t) {
+    for (int64_t k = k0; k < k1; ++k) {
+      double g[6][6] = {}, v[6][6];
       for (int32_t t = ptr[k]; t < ptr[k + 1]; ++t) {
         const int32_t n = nodes[t];
-        for (int32_t b = h->hrowptr[n]; b < h->hrowptr[n + 1]; ++b) {
-          const int32_t o = agg[h->hcolidx[b]];
-          if (mark[o] != k) {
-            mark[o] = k;
-            tmp.push_back(o);
-          }
+        const double dx = dd[n * 3], dy = dd[n * 3 + 1], dz = dd[n * 3 + 2];
+        const double Bm[3][6] = {{1, 0, 0, 0, dz, -dy}, {0, 1, 0, -dz, 0, dx}, {0, 0, 1, dy, -dx, 0}};
+        for (int r = 0; r < 3; ++r) {
+          if (h->hfixed[3 * n + r]) continue;
+          for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) g[i][j] += Bm[r][i] * Bm[r][j];
         }
       }
-      std::sort(tmp.begin(), tmp.end());
-      nlist.insert(nlist.end(), tmp.begin(), tmp.end());
-      nptr[k + 1] = (int32_t)nlist.size();
+      double g0[6][6];
+      std::copy(&g[0][0], &g[0][0] + 36, &g0[0][0]);
+      jacobiEigen6(g, v);
+      double lmax = 0.0;
+      for (int i = 0; i < 6; ++i) lmax = std::max(lmax, g[i][i]);
+      for (int j = 0; j < 6; ++j) {
+        const double l = g[j][j];
+        if (!(lmax > 0.0) || !(l > 1e-10 * lmax)) {
+          drop[k * 6 + j] = 1;
+          continue;
+        }
+        for (int i = 0; i < 6; ++i) T[k * 36 + i * 6 + j] = v[i][j] / std::sqrt(l);
+      }
+      // the kept modes are orthonormal over the aggregate: T^T G T = I on them
+      for (int p = 0; p < 6; ++p)
+        for (int q = 0; q < 6; ++q) {
+          if (drop[k * 6 + p] || drop[k * 6 + q]) continue;
+          double e = 0.0;
+          for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) e += T[k * 36 + i * 6 + p] * g0[i][j] * T[k * 36 + j * 6 + q];
+          require(std::fabs(e - (p == q ? 1.0 : 0.0)) < 1e-6, IK_EINVAL,
+                  "internal error: aggregate modes not orthonormalised");
+        }
     }
+  }, 16);
+  // aggregate adjacency from the block pattern: per chunk of aggregates, each aggregate's sorted neighbours,
+  // placed by the counts afterwards
+  std::vector<int32_t> nptr(na + 1, 0), nlist;
+  {
+    std::vector<std::vector<int32_t>> part(chunkCount(na, 16));
+    std::vector<int32_t> cnt(na);
+    parallelChunks(na, [&](int64_t k0, int64_t k1, int64_t c) {
+      std::vector<int32_t> mark(na, -1), tmp;
+      for (int64_t k = k0; k < k1; ++k) {
+        tmp.clear();
+        for (int32_t t = ptr[k]; t < ptr[k + 1]; ++t) {
+          const int32_t n = nodes[t];
+          for (int32_t b = h->hrowptr[n]; b < h->hrowptr[n + 1]; ++b) {
+            const int32_t o = agg[h->hcolidx[b]];
+            if (mark[o] != (int32_t)k) {
+              mark[o] = (int32_t)k;
+              tmp.push_back(o);
+            }
+          }
+        }
+        std::sort(tmp.begin(), tmp.end());
+        part[c].insert(part[c].end(), tmp.begin(), tmp.end());
+        cnt[k] = (int32_t)tmp.size();
+      }
+    }, 16);
+    for (int32_t k = 0; k < na; ++k) nptr[k + 1] = nptr[k] + cnt[k];
+    nlist.reserve(nptr[na]);
+    for (const auto& v : part) nlist.insert(nlist.end(), v.begin(), v.end());
   }
   cs.na = na;
   cs.nc = 6 * (int64_t)na;

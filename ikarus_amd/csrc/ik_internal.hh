@@ -243,6 +243,9 @@ struct ik_handle {
   // learned predictor
   ik::Mlp* mlp = nullptr;
   bool mlpLoaded = false;
+  // mesh lattice (set by the pattern build): node bounds and the median element extent per axis; the Morton
+  // order of the element tiles and the boxes of the coarse aggregates are cells of it
+  double latLo[3] = {0, 0, 0}, latHi[3] = {0, 0, 0}, latStep[3] = {1, 1, 1};
   double mlpTraction[3] = {0, 0, 0};
   double mlpCharLen = 1.0;
 

@@ -10,87 +10,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
-#include <exception>
-#include <mutex>
 #include <thread>
 
+#include "host_parallel.hh"
 #include "ik_internal.hh"
 
 namespace ik {
-
-// Host threads of the build: ranges of nodes, elements or tiles whose outputs sit at positions fixed
-// before the threads start (variable-length outputs go to per-chunk buffers placed by a prefix sum after
-// the join), so every array is the same for any thread count. IK_PATTERN_THREADS sets the count (1:
-// serial); the default is the host's hardware threads, at most 16.
-static int patternThreads() {
-  if (const char* v = std::getenv("IK_PATTERN_THREADS")) {
-    const int t = std::atoi(v);
-    if (t >= 1) return std::min(t, 64);
-  }
-  const unsigned hc = std::thread::hardware_concurrency();
-  return (int)std::max(1u, std::min(hc, 16u));
-}
-// chunks of at least `grain` items (2048 nodes, elements or tiles; 32 slices of 64 rows)
-static int64_t chunkCount(int64_t n, int64_t grain = 2048) {
-  return std::min<int64_t>(patternThreads(), std::max<int64_t>(1, n / grain));
-}
-
-// f(lo, hi, chunk) over chunk = 0 .. chunkCount(n) - 1; the first exception thrown is rethrown here
-template <class F>
-static void parallelChunks(int64_t n, F&& f, int64_t grain = 2048) {
-  const int64_t T = chunkCount(n, grain);
-  if (T <= 1) {
-    f(int64_t(0), n, int64_t(0));
-    return;
-  }
-  std::vector<std::thread> th;
-  std::exception_ptr err;
-  std::mutex m;
-  th.reserve(T);
-  for (int64_t c = 0; c < T; ++c)
-    th.emplace_back([&, c] {
-      try {
-        f(n * c / T, n * (c + 1) / T, c);
-      } catch (...) {
-        std::lock_guard<std::mutex> g(m);
-        if (!err) err = std::current_exception();
-      }
-    });
-  for (auto& x : th) x.join();
-  if (err) std::rethrow_exception(err);
-}
-
-// out[i] = sum of w(k) over k < i, for i = 0 .. n (out has n + 1 entries): per-chunk sums, their prefix, then
-// each chunk's running sum; the same values for any thread count (integer sums)
-template <class W>
-static void parallelPrefix(int64_t n, W&& w, HostVec<int64_t>& out) {
-  out.clear();
-  out.resize(n + 1);
-  const int64_t T = chunkCount(n);
-  std::vector<int64_t> part(T + 1, 0);
-  parallelChunks(n, [&](int64_t i0, int64_t i1, int64_t c) {
-    int64_t s = 0;
-    for (int64_t i = i0; i < i1; ++i) s += w(i);
-    part[c + 1] = s;
-  });
-  for (int64_t c = 0; c < T; ++c) part[c + 1] += part[c];
-  parallelChunks(n, [&](int64_t i0, int64_t i1, int64_t c) {
-    int64_t s = part[c];
-    for (int64_t i = i0; i < i1; ++i) {
-      out[i] = s;
-      s += w(i);
-    }
-    if (i1 == n) out[n] = s;
-  });
-}
-
-// n copies of v, written by the build's threads
-template <class T>
-static HostVec<T> filled(int64_t n, T v) {
-  HostVec<T> b(n);
-  parallelChunks(n, [&](int64_t i0, int64_t i1, int64_t) { std::fill(b.data() + i0, b.data() + i1, v); });
-  return b;
-}
 
 // elements per staging chunk; IKARUS_AMD_CHUNK_ELEMENTS overrides it. Splitting the 64^3 mesh into chunks
 // whose gathers overlap the next chunk's element work measured slower than one launch of each (the two
@@ -116,21 +41,34 @@ static uint64_t spread3(uint32_t v) {
   return x;
 }
 
-// Elements in Morton (Z) order of their centroids on a lattice of the median element extent per axis, ties
-// by element index. An element's extent along axis d is the largest |component d| of its three mean edge
-// vectors (the average of the four edges along each reference axis), which node jitter leaves unbiased.
-// On a structured grid the lattice is then the grid itself, so 16 consecutive elements form a 4x2x2 box
-// (45 nodes) whose 576 upper element blocks fall on 341 distinct K blocks; on other meshes the order still
-// keeps each tile compact.
-static HostVec<int32_t> mortonOrder(const ik_handle* h) {
+// The mesh lattice: node bounds and, per axis, the median element extent, an element's extent along axis d
+// being the largest |component d| of its three mean edge vectors (the average of the four edges along each
+// reference axis), which node jitter leaves unbiased. On a structured grid the lattice is the grid itself.
+static void meshLattice(ik_handle* h) {
   const int64_t ne = h->ne, nn = h->nn;
-  const double* X  = h->hX.data();
+  const double* X     = h->hX.data();
   const int32_t* conn = h->hconn.data();
-  double lo[3], step[3];
+  const int64_t T     = chunkCount(nn);
+  std::vector<double> plo((size_t)T * 3, 0.0), phi((size_t)T * 3, 0.0);
+  parallelChunks(nn, [&](int64_t n0, int64_t n1, int64_t c) {
+    for (int d = 0; d < 3; ++d) {
+      double lo = X[n0 * 3 + d], hi = lo;
+      for (int64_t n = n0 + 1; n < n1; ++n) {
+        lo = std::min(lo, X[n * 3 + d]);
+        hi = std::max(hi, X[n * 3 + d]);
+      }
+      plo[c * 3 + d] = lo;
+      phi[c * 3 + d] = hi;
+    }
+  });
   HostVec<double> ext(ne);
   for (int d = 0; d < 3; ++d) {
-    lo[d] = X[d];
-    for (int64_t n = 1; n < nn; ++n) lo[d] = std::min(lo[d], X[n * 3 + d]);
+    h->latLo[d] = plo[d];
+    h->latHi[d] = phi[d];
+    for (int64_t c = 1; c < T; ++c) {
+      h->latLo[d] = std::min(h->latLo[d], plo[c * 3 + d]);
+      h->latHi[d] = std::max(h->latHi[d], phi[c * 3 + d]);
+    }
     parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) {
       for (int64_t e = e0; e < e1; ++e) {
         double best = 0.0;
@@ -144,8 +82,19 @@ static HostVec<int32_t> mortonOrder(const ik_handle* h) {
       }
     });
     std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
-    step[d] = ext[ne / 2] > 0.0 ? ext[ne / 2] : 1.0;
+    h->latStep[d] = ext[ne / 2] > 0.0 ? ext[ne / 2] : 1.0;
   }
+}
+
+// Elements in Morton (Z) order of their centroids on the mesh lattice, ties by element index. On a
+// structured grid 16 consecutive elements then form a 4x2x2 box (45 nodes) whose 576 upper element blocks
+// fall on 341 distinct K blocks; on other meshes the order still keeps each tile compact.
+static HostVec<int32_t> mortonOrder(const ik_handle* h) {
+  const int64_t ne = h->ne;
+  const double* X     = h->hX.data();
+  const int32_t* conn = h->hconn.data();
+  const double* lo = h->latLo;
+  const double* step = h->latStep;
   HostVec<std::pair<uint64_t, int32_t>> key(ne);
   parallelChunks(ne, [&](int64_t e0, int64_t e1, int64_t) {
     for (int64_t e = e0; e < e1; ++e) {
@@ -185,6 +134,7 @@ static HostVec<int32_t> mortonOrder(const ik_handle* h) {
 void buildPattern(ik_handle* h) {
   const int64_t nn = h->nn, ne = h->ne;
   const int32_t* conn = h->hconn.data();
+  meshLattice(h);
   // node -> incidences e*8 + a (CSR, ordered by element): nlist; the incident elements are nlist / 8
   std::vector<int64_t> cnt(nn + 1, 0);
   for (int64_t e = 0; e < ne; ++e)
