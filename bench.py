@@ -160,10 +160,10 @@ NEWTON_MODES = {
     "mixed_mlp": dict(precision=capi.FP32, kind=capi.PCG_BJACOBI_FP32_IR, rtol=1e-10, forcing=0.9,
                       pred=capi.PRED_MLP, eta_max=0.01),
     # the mixed form with the two-level preconditioner (block Jacobi + rigid-body-mode coarse correction over
-    # box aggregates) in the FP32 PCG; eta_max tuned the same way (64^3, 40 load steps, seeds 1000-1003:
-    # constant 0.03, secant 0.01, learned 0.001)
+    # about 729 box aggregates) in the FP32 PCG; eta_max tuned the same way (64^3, 40 load steps, seeds
+    # 1000-1003: constant 0.1, secant 0.01, learned 0.001)
     "tl_constant": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
-                        pred=capi.PRED_CONSTANT, eta_max=0.03),
+                        pred=capi.PRED_CONSTANT, eta_max=0.1),
     "tl_secant": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,
                       pred=capi.PRED_LINEAR, eta_max=0.01),
     "tl_mlp": dict(precision=capi.FP32, kind=capi.PCG_TWOLEVEL_FP32_IR, rtol=1e-10, forcing=0.9,

@@ -53,7 +53,7 @@ struct Coarse {
 
 namespace {
 
-constexpr int kTargetAggregates = 512;
+constexpr int kTargetAggregates = 729;
 constexpr int kMaxAggregates    = 1024;  // the update's 2 na partials fit the first 2 kReduceBlocks entries
 // threads of the per-aggregate update workgroup (64^3 learned path: 14.3 us per launch with 256, 16.4 with 512,
 // 51.9 with 1024)
