@@ -7,7 +7,7 @@
 //
 // Aggregates: boxes of a x a x a median element extents over the reference coordinates (the last box along each
 // axis takes the remainder, so no aggregate is a thin layer); a is chosen from the node count so that there are
-// about 512 aggregates. Node i of aggregate A carries the six rigid-body modes about the aggregate's centroid,
+// about 729 aggregates (at 64^3 the fastest of 343, 512, 729 and 1000 on training load paths). Node i of aggregate A carries the six rigid-body modes about the aggregate's centroid,
 // B(d_i) = [I | w -> w x d_i] with d_i = X_i - c_A, rows of constrained DOFs zero. Per aggregate the modes are
 // orthonormalised over its nodes (T_A = V L^-1/2 from the eigenpairs of sum_i B_i^T B_i; modes with an eigenvalue
 // below 1e-10 of the largest, e.g. rotations of a collinear aggregate, are dropped), so P_i = B(d_i) T_A and
