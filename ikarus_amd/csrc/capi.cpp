@@ -217,6 +217,25 @@ void assembleK(ik_handle* h, int dbc, int precision) {
   h->kDbc       = dbc;
   h->kValid     = true;
 }
+bool kReady(const ik_handle* h, int dbc, int precision) {
+  return h->kValid && h->kDbc == dbc && h->kPrecision == precision;
+}
+// The residual at the state a Newton correction just reached. The EAS element's residual-only pass still forms
+// D, L and Rt and eliminates (about 70 % of its R+K pass), so for EAS handles the pass also assembles the
+// condensed K (FULL, the given precision): the next iteration solves with it instead of assembling again, and R
+// is the same, bit for bit, as the residual-only pass gives. The plain element's residual pass is a tenth of
+// its tangent, so there R is assembled alone.
+void assembleRAfterUpdate(ik_handle* h, int precision, bool withK) {
+  if (!(withK && h->neas)) {
+    assembleR(h, IK_DBC_FULL);
+    return;
+  }
+  ensurePattern(h);
+  ik::assembleKR(h, IK_DBC_FULL, precision, true, true);
+  h->kPrecision = precision;
+  h->kDbc       = IK_DBC_FULL;
+  h->kValid     = true;
+}
 
 void syncIdbc(ik_handle* h) {
   if (!h->hasUbar || h->nfixed == 0) return;
@@ -277,7 +296,7 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
   int iter            = 0;
   h->kValid           = false;
   while ((rNorm > s->tol && iter < s->max_iter) || iter < s->min_iter) {
-    assembleK(h, IK_DBC_FULL, s->precision);
+    if (!kReady(h, IK_DBC_FULL, s->precision)) assembleK(h, IK_DBC_FULL, s->precision);
     // K x = R ; correction = -x (solve(correction, -rx), newtonraphson.hh:222-223)
     double rtol = s->linear.rtol;
     if (s->forcing > 0.0) {
@@ -306,7 +325,7 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     syncIdbc(h);  // syncParameterAndGlobalSolution (newtonraphson.hh:236-237)
     h->kValid        = false;
     h->easCacheValid = false;
-    assembleR(h, IK_DBC_FULL);
+    assembleRAfterUpdate(h, s->precision, true);
     rNorm               = normR(h);
     info->residual_norm = rNorm;
     ++iter;
@@ -389,7 +408,7 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
       ik::axpy(h, -1.0, h->idbcF.p, h->arcRhs.p, n);
       rhsL = h->arcRhs.p;
     }
-    assembleK(h, IK_DBC_FULL, s->precision);
+    if (!kReady(h, IK_DBC_FULL, s->precision)) assembleK(h, IK_DBC_FULL, s->precision);
     ik_solver_info li{};
     linearSolve(h, h->R.p, h->arcDR.p, &s->linear, s->linear.rtol, &li);  // K x = R, dR = -x
     info->linear_iterations += li.iterations;
@@ -412,7 +431,7 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
     syncIdbc(h);  // syncParameterAndGlobalSolution (:276-277)
     h->kValid        = false;
     h->easCacheValid = false;
-    assembleR(h, IK_DBC_FULL);
+    assembleRAfterUpdate(h, s->precision, !idbc);  // with IDBC every iteration assembles K_raw for F_idbc first
     arcSubsidiary(h, a, f, invRoot, dfdDl);
     rNorm               = std::sqrt(ik::dot(h, h->R.p, h->R.p, n) + f * f);
     info->residual_norm = rNorm;
