@@ -7,15 +7,16 @@
 //
 // Aggregates: boxes of a x a x a median element extents over the reference coordinates (the last box along each
 // axis takes the remainder, so no aggregate is a thin layer); a is chosen from the node count so that there are
-// about 729 aggregates (at 64^3 the fastest of 343, 512, 729 and 1000 on training load paths). Node i of aggregate A carries the six rigid-body modes about the aggregate's centroid,
-// B(d_i) = [I | w -> w x d_i] with d_i = X_i - c_A, rows of constrained DOFs zero. Per aggregate the modes are
+// about 729 aggregates (at 64^3 the fastest of 343, 512, 729 and 1000 on training load paths). Node i of
+// aggregate A carries the six rigid-body modes about the aggregate's centroid, B(d_i) = [I | w -> w x d_i] with
+// d_i = X_i - c_A, rows of constrained DOFs zero. Per aggregate the modes are
 // orthonormalised over its nodes (T_A = V L^-1/2 from the eigenpairs of sum_i B_i^T B_i; modes with an eigenvalue
 // below 1e-10 of the largest, e.g. rotations of a collinear aggregate, are dropped), so P_i = B(d_i) T_A and
 // P^T K P is positive definite for positive definite K.
 //
 // Set-up (first two-level solve, again when a solve needs more than twice the inner iterations per decade of
-// residual reduction of the first solve after the last set-up, plus 5): A_raw = sum over K blocks B_i^T K_ij B_j per aggregate pair on the device (FP64,
-// fixed summation order), A = T^T A_raw T, Cholesky factorisation and inverse (rocSOLVER dpotrf / dpotri), and
+// residual reduction of the first solve after the last set-up, plus 5): A_raw = sum over K blocks
+// B_i^T K_ij B_j per aggregate pair on the device (FP64, fixed summation order), A = T^T A_raw T, Cholesky factorisation and inverse (rocSOLVER dpotrf / dpotri), and
 // G = T A^-1 T^T stored dense in FP32. Per PCG iteration: s = B^T r per aggregate (one workgroup per aggregate),
 // w = G s (one wavefront per row), z_i = D_i^-1 r_i + B(d_i) w_A (fused with the r.z partials).
 #include <rocblas/rocblas.h>
@@ -488,7 +489,8 @@ void buildAggregates(ik_handle* h, Coarse& cs) {
   const double* hi = h->latHi;
   const double* step = h->latStep;
   auto boxes = [&](int a, int64_t nb[3]) {
-    for (int d = 0; d < 3; ++d) nb[d] = std::max<int64_t>(1, (int64_t)std::floor((hi[d] - lo[d]) / (a * step[d]) + 1e-9));
+    for (int d = 0; d < 3; ++d)
+      nb[d] = std::max<int64_t>(1, (int64_t)std::floor((hi[d] - lo[d]) / (a * step[d]) + 1e-9));
     return nb[0] * nb[1] * nb[2];
   };
   int a = std::max(2, (int)std::lround(std::cbrt((double)nn / kTargetAggregates)));
@@ -499,9 +501,10 @@ void buildAggregates(ik_handle* h, Coarse& cs) {
   parallelChunks(nn, [&](int64_t n0, int64_t n1, int64_t) {
     for (int64_t n = n0; n < n1; ++n) {
       int64_t k[3];
-      for (int d = 0; d < 3; ++d)
-        k[d] = std::min<int64_t>(nb[d] - 1,
-                                 std::max<int64_t>(0, (int64_t)std::floor((X[n * 3 + d] - lo[d]) / (a * step[d]) + 1e-9)));
+      for (int d = 0; d < 3; ++d) {
+        const int64_t c = (int64_t)std::floor((X[n * 3 + d] - lo[d]) / (a * step[d]) + 1e-9);
+        k[d]            = std::min<int64_t>(nb[d] - 1, std::max<int64_t>(0, c));
+      }
       key[n] = (k[2] * nb[1] + k[1]) * nb[0] + k[0];
     }
   });

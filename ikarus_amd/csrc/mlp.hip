@@ -235,7 +235,8 @@ __global__ __launch_bounds__(256) void mlpPredictFused(const uint16_t* __restric
                                                        const uint8_t* __restrict__ fixed) {
   __shared__ __align__(16) uint16_t sW[kWbN];  // 78 KB
   __shared__ float sB[3 * kHid + 16];
-  for (int i = threadIdx.x; i < kWbN / 8; i += 256) reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(Wb)[i];
+  for (int i = threadIdx.x; i < kWbN / 8; i += 256)
+    reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(Wb)[i];
   for (int i = threadIdx.x; i < 3 * kHid + 16; i += 256) {
     float v = 0.f;
     if (i < kHid) v = P[oB1 + i];

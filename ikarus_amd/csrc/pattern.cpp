@@ -297,7 +297,8 @@ void buildPattern(ik_handle* h) {
         mask[uptr[i] + (p - h->hdiag[i])] = m;
       }
   });
-  HostVec<uint8_t> gathered = filled<uint8_t>(nup, 1);  // upper blocks the gather assembles (the rest are written directly)
+  // upper blocks the gather assembles (the rest are written directly)
+  HostVec<uint8_t> gathered = filled<uint8_t>(nup, 1);
   HostVec<int64_t> unitOf(ne);  // staging unit of each element: its tile (tiled) or itself
   h->tiled = h->neas == 0;
   int64_t ntiles = 0;
