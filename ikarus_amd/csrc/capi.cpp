@@ -221,9 +221,9 @@ bool kReady(const ik_handle* h, int dbc, int precision) {
   return h->kValid && h->kDbc == dbc && h->kPrecision == precision;
 }
 // The residual at the state a Newton correction just reached. The EAS element's residual-only pass still forms
-// D, L and Rt and eliminates (about 70 % of its R+K pass), so for EAS handles the pass also assembles the
-// condensed K (FULL, the given precision): the next iteration solves with it instead of assembling again, and R
-// is the same, bit for bit, as the residual-only pass gives. The plain element's residual pass is a tenth of
+// D, L and Rt and eliminates (about half the time of its R+K pass), so for EAS handles the pass also assembles
+// the condensed K (FULL, the given precision): the next iteration solves with it instead of assembling again, and
+// R is the same, bit for bit, as the residual-only pass gives. The plain element's residual pass is a tenth of
 // its tangent, so there R is assembled alone.
 void assembleRAfterUpdate(ik_handle* h, int precision, bool withK) {
   if (!(withK && h->neas)) {
