@@ -243,8 +243,9 @@ void syncIdbc(ik_handle* h) {
                      h->ubar.p, h->lambda, h->u.p);
 }
 
+// rhsNorm: ||rhs|| when the caller has it, negative otherwise
 void linearSolve(ik_handle* h, const double* rhs, double* x, const ik_solver_opts* o, double rtol,
-                 ik_solver_info* info) {
+                 ik_solver_info* info, double rhsNorm = -1.0) {
   require(h->kValid && h->kDbc == IK_DBC_FULL, IK_EINVAL, "ik_solve needs a FULL-DBC matrix assembled first");
   if (o->kind == IK_PCG_JACOBI_FP64) {
     require(h->kPrecision == IK_FP64, IK_EINVAL, "FP64 PCG needs an FP64 tangent");
@@ -259,7 +260,7 @@ void linearSolve(ik_handle* h, const double* rhs, double* x, const ik_solver_opt
     require(h->kPrecision == IK_FP32, IK_EINVAL, "mixed PCG needs an FP32 tangent");
     ik_solver_opts oo = *o;
     oo.rtol           = rtol;
-    ik::pcgMixed(h, rhs, x, &oo, info);
+    ik::pcgMixed(h, rhs, x, &oo, info, rhsNorm);
   } else {
     throw Error{IK_ENOTIMPL, "unknown linear solver kind"};
   }
@@ -316,10 +317,9 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     }
     rPrev = rNorm;
     ik_solver_info li{};
-    linearSolve(h, h->R.p, h->du.p, &s->linear, rtol, &li);
+    linearSolve(h, h->R.p, h->du.p, &s->linear, rtol, &li, rNorm);  // rNorm = ||R|| of this very R
     info->linear_iterations += li.iterations;
     hipLaunchKernelGGL(negateKernel, dim3(grid(n)), dim3(256), 0, h->stream, h->du.p, n);
-    info->correction_norm = std::sqrt(ik::dot(h, h->du.p, h->du.p, n));
     if (h->neas) ik::easUpdate(h, h->du.p);  // CORRECTION_UPDATED (newtonraphson.hh:230)
     ik::axpy(h, 1.0, h->du.p, h->u.p, n);
     syncIdbc(h);  // syncParameterAndGlobalSolution (newtonraphson.hh:236-237)
@@ -331,6 +331,8 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     ++iter;
     if (!std::isfinite(rNorm)) break;
   }
+  // the last correction's norm (reported only, so read once: du still holds it)
+  if (iter > 0) info->correction_norm = std::sqrt(ik::dot(h, h->du.p, h->du.p, n));
   // success = iter != maxIter (newtonraphson.hh:251-252); a residual norm that is not finite is a failure here
   // (the reference's loop ends on it, since NaN > tol is false, and reports success)
   if (iter == s->max_iter || !std::isfinite(rNorm)) info->success = 0;

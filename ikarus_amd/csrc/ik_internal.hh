@@ -278,7 +278,9 @@ double foldPartials(ik_handle* h, int nb);
 double energy(ik_handle* h);  // potential energy at the bound requirement (plain element)
 void stressAt(ik_handle* h, int kind, const double xi[3], double* out);  // out: device, ne x 6
 void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxIter, int* iters, double* relres);
-void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info);
+// bnorm: ||b|| when the caller has it (the Newton residual's norm), negative to compute it here
+void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info,
+              double bnorm = -1.0);
 // coarse.hip
 void coarseDestroy(Coarse* c);
 void coarsePrepare(ik_handle* h);

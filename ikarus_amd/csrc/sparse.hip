@@ -626,7 +626,8 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
   *relres = h->hsc->bb > 0 ? std::sqrt(h->hsc->rr / h->hsc->bb) : 0.0;
 }
 
-void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info) {
+void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o, ik_solver_info* info,
+              double bnormKnown) {
   ensureWork(h);
   const bool twoLevel = o->kind == IK_PCG_TWOLEVEL_FP32_IR;
   const int64_t n = h->ndof;
@@ -647,7 +648,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
   // x = 0, r = b
   launchZero(x, n, h->stream);
   IK_HIP(hipMemcpyAsync(h->pr.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-  const double bnorm = std::sqrt(dot(h, b, b, n));
+  const double bnorm = bnormKnown >= 0.0 ? bnormKnown : std::sqrt(dot(h, b, b, n));
   info->iterations  = 0;
   info->refinements = 0;
   info->converged   = 0;
