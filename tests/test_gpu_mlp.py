@@ -198,3 +198,11 @@ def test_fused_predictor_errors():
     dev.check(L.ik_mlp_set_params(dev.h, capi.ptr(learned.MLP(seed=1).params()), capi.HOST))
     assert L.ik_mlp_predict(dev.h, capi.ptr(z), capi.ptr(z), 0.1, 0.1, 0.0, 0.0, capi.HOST) == capi.IK_ENOTBOUND
     dev.close()
+    # plane-strain handles: the network's features are 3D
+    p2 = pb.cantilever2d((8, 2), seed=1)
+    dev = ia.Device(p2)
+    dev.check(L.ik_mlp_set_params(dev.h, capi.ptr(learned.MLP(seed=1).params()), capi.HOST))
+    dev.set_requirement(np.zeros(p2.ndof), 0.0)
+    z2 = np.zeros(3 * dev.ndof)
+    assert L.ik_mlp_predict(dev.h, capi.ptr(z2), capi.ptr(z2), 0.1, 0.1, 0.0, 0.0, capi.HOST) == capi.IK_ENOTIMPL
+    dev.close()
