@@ -13,15 +13,18 @@
 //
 // One 64-lane workgroup (one wave) per element, all lanes busy in every phase: Gauss-point kinematics and
 // material per (point, node) lane pair, K_uu in the Ci form per (node, node) lane pair, D / L / Rt / R_u by
-// owned entries, Gauss-Jordan elimination of [D | L | Rt] one column per lane, and the rank-21 update of K and R.
-// Outputs go to the same element-by-element staging as the plain element (36 upper blocks + 24 residual
-// entries), plus the cached Y = D^{-1} L (21x24) and y = D^{-1} Rt (21) used by the alpha update.
+// owned entries, forward elimination of [D | L | Rt] one column per lane (D = L_D diag(d) L_D^T), and the
+// rank-21 update of K and R. Outputs go to the same element-by-element staging as the plain element (36 upper
+// blocks + 24 residual entries), plus the cache of the alpha update: diag(d)^{-1} L_D^{-1} L (21x24),
+// diag(d)^{-1} L_D^{-1} Rt (21) and L_D (210), so that D^{-1} v = L_D^{-T} diag(d)^{-1} L_D^{-1} v.
 #include "ik_internal.hh"
 
 namespace ik {
 namespace {
 
 constexpr int NE = 21;  // enhanced modes
+// doubles per element of the alpha-update cache: What (24 columns of 21), what (21), the strictly lower L_D (210)
+constexpr int kEasCache = NE * 24 + NE + NE * (NE - 1) / 2;
 
 __host__ __device__ constexpr int vI(int v) { return v < 3 ? v : (v == 3 ? 1 : 0); }
 __host__ __device__ constexpr int vJ(int v) { return v < 3 ? v : (v == 5 ? 1 : 2); }
@@ -158,9 +161,9 @@ struct EasLds {
       double ru[8][8][3]; // R_u contribution F S dV g_a of point gp, node a
     } g;
     struct {
-      double A[NE][24];   // D (columns 0-20), then Y = D^-1 L
-      double Lm[NE][24];  // L
-      double rt[NE];      // Rt, then y = D^-1 Rt
+      double A[NE][24];   // D (columns 0-20), then What = diag(d)^-1 L_D^-1 L
+      double Lm[NE][24];  // L, then W = L_D^-1 L (row j written at elimination step j)
+      double rt[NE];      // Rt, then what = diag(d)^-1 L_D^-1 Rt
       double Ru[24];
     } c;
   } u;
@@ -232,8 +235,8 @@ __device__ inline void symMul(const double* A6, const double x[3], double y[3]) 
 //   p_a = F Ci g_a, Phi = F Ci F^T (the material part of B^T C B written with C = lambda Ci(x)Ci + 2 c2 sym(Ci box
 //   Ci); SVK: Ci = I, c2 = mu), lane (a, b) per 3x3 block; then D, L = sum f_i Z[kRow_i], Rt and R_u
 //   = sum (F S dV g_b)_l with Z[w][3b+l] = (F Q^w g_b)_l, Q^w the symmetric tensor of column w of C dV T0^-1;
-// (3) Y = D^-1 [L | Rt] by Gauss-Jordan, one column per lane; (4) K = K_uu - L^T Y, lane (a, b) per block (the
-// diagonal blocks mirrored from their upper triangle), R = R_u - L^T y.
+// (3) W = L_D^-1 [L | Rt] and What = diag(d)^-1 W by forward elimination, one column per lane; (4) K = K_uu -
+// W^T What, lane (a, b) per block (the diagonal blocks mirrored from their upper triangle), R = R_u - W^T what.
 template <int KIND, bool WANT_K>
 __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, const double* __restrict__ U,
                                                  const int32_t* __restrict__ conn,
@@ -517,11 +520,16 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   }
   __syncthreads();
 
-  // (3) Y = D^-1 [L | Rt] by Gauss-Jordan elimination on the augmented 21 x 46 matrix without pivoting (D is SPD
-  // while the material tangent is positive definite; enhancedassumedstrains.hh inverts D by LU, :243,294,344). Lane c < 46 keeps column
-  // c in registers; each step every lane reads the pivot column from lane j's registers (readlane). The
-  // LDS broadcast this replaced (lane j stores, barrier, every lane loads) made the kernel LDS-issue bound:
-  // 7.98 -> 5.69 ms per R+K pass at 96x96x48, results bitwise the same.
+  // (3) The condensation L^T D^-1 [L | Rt] (enhancedassumedstrains.hh inverts D by LU, :243,294,344) through
+  // D = L_D diag(d) L_D^T without pivoting (D is SPD while the material tangent is positive definite):
+  // L^T D^-1 L = W^T diag(d)^-1 W with W = L_D^-1 L, so forward elimination of the augmented 21 x 46 matrix
+  // [D | L | Rt] suffices. Lane c < 46 keeps column c in registers; step j reads rows j..20 of the pivot
+  // column from lane j's registers (readlane), and every lane divides its row j, final from then on, by d_j and
+  // subtracts the multiples from its rows below j. The L lanes first save their row j of W over the same row
+  // of L in LDS. Afterwards the L lanes hold What = diag(d)^-1 W, the Rt lane what = diag(d)^-1 L_D^-1 Rt, and
+  // lane c < 21 holds the multipliers (L_D)_cj = U_jc / d_j in rows j < c (U = diag(d) L_D^T, the eliminated
+  // D). Half the lane reads and updates of a Gauss-Jordan elimination; the back substitution D^-1 needs for
+  // the alpha update runs there, on 21 values per element.
   double col[NE];
 #pragma unroll
   for (int i = 0; i < NE; ++i)
@@ -536,24 +544,25 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   bool singular = !(dmax < INFINITY);
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
-    double piv[NE];  // lane j's column, read into scalar registers (no LDS round trip, no barrier)
+    double piv[NE];  // rows j.. of lane j's column, read into scalar registers (no LDS round trip, no barrier)
 #pragma unroll
-    for (int i = 0; i < NE; ++i) piv[i] = readLane(col[i], j);
+    for (int i = j; i < NE; ++i) piv[i] = readLane(col[i], j);
     singular |= !(fabs(piv[j]) > 1e-13 * dmax && fabs(piv[j]) < INFINITY);
+    if (t >= NE && t < NE + 24) sc.Lm[j][t - NE] = col[j];  // W_jc; lane c alone reads and writes column c
     const double f = col[j] / piv[j];
 #pragma unroll
-    for (int i = 0; i < NE; ++i)
-      if (i != j) col[i] -= piv[i] * f;
+    for (int i = j + 1; i < NE; ++i) col[i] -= piv[i] * f;
     col[j] = f;
   }
   if (singular && t == 0) atomicMin(err + 1, (unsigned long long)e);
-  // lanes NE..NE+23 hold Y = D^-1 L (column c = t - NE), lane NE+24 holds y = D^-1 Rt; cache them for the
-  // alpha update (layout per element: Y 21 x 24 row-major, then y)
-  double* cy = cache + e * (NE * 24 + NE);
+  // cache for the alpha update, per element: What column by column (entry (i, c) at 21 c + i, so that the
+  // update's lane per mode reads 21 adjacent values at a time), what, then (L_D)_ij for i > j row by row
+  // (offset 525 + i (i - 1) / 2 + j)
+  double* cy = cache + e * kEasCache;
   if (t >= NE && t < NE + 24) {
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
-      cy[i * 24 + (t - NE)] = col[i];
+      cy[(t - NE) * NE + i] = col[i];
       sc.A[i][t - NE]       = col[i];
     }
   } else if (t == NE + 24) {
@@ -562,10 +571,15 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
       cy[NE * 24 + i] = col[i];
       sc.rt[i]        = col[i];
     }
+  } else if (t > 0 && t < NE) {
+    double* lrow = cy + NE * 25 + t * (t - 1) / 2;
+#pragma unroll
+    for (int j = 0; j < NE - 1; ++j)
+      if (j < t) lrow[j] = col[j];
   }
   __syncthreads();
-  // (4) K = K_uu - L^T Y, block (ka, kb) per lane; upper blocks stored, the diagonal ones mirrored from their
-  // upper triangle (enhancedassumedstrains.hh:291-296)
+  // (4) K = K_uu - W^T What = K_uu - L^T D^-1 L, block (ka, kb) per lane; upper blocks stored, the diagonal
+  // ones mirrored from their upper triangle (enhancedassumedstrains.hh:291-296)
   if constexpr (WANT_K) {
     const int blk = upperBlock(ka <= kbn ? ka : kbn, ka <= kbn ? kbn : ka);
     double kb[9];
@@ -614,22 +628,32 @@ __global__ __launch_bounds__(64) void easElement(const double* __restrict__ X, c
   }
 }
 
-// alpha_e -= y_e + Y_e du_e (thread per (element, mode))
+// alpha_e -= D_e^-1 (Rt_e + L_e du_e) = L_D^-T (what_e + What_e du_e): 21 lanes per element, 3 elements per
+// wavefront (lane 63 idle). Lane i forms z_i = what_i + (What du)_i; the back substitution with the unit upper
+// triangle L_D^T then runs from the last mode up, x_k = z_k reaching lanes i < k of its element by a shuffle.
+constexpr int kUpdElemsPerWave = 64 / NE;
 __global__ __launch_bounds__(256) void easUpdateKernel(const double* __restrict__ cache,
                                                        const int32_t* __restrict__ conn,
                                                        const double* __restrict__ du, int64_t ne,
                                                        double* __restrict__ alpha) {
-  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (tid >= ne * NE) return;
-  const int64_t e = tid / NE;
-  const int i     = (int)(tid - e * NE);
-  const double* cy = cache + e * (NE * 24 + NE);
-  double acc       = cy[NE * 24 + i];
-  for (int a = 0; a < 8; ++a) {
-    const int64_t nd = conn[e * 8 + a];
-    for (int k = 0; k < 3; ++k) acc += cy[i * 24 + a * 3 + k] * du[nd * 3 + k];
+  const int lane = threadIdx.x & 63, sub = lane / NE, i = lane - NE * sub;
+  const int64_t e   = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kUpdElemsPerWave + sub;
+  const bool active = sub < kUpdElemsPerWave && e < ne;
+  const double* cy  = cache + (active ? e : 0) * kEasCache;
+  double z          = 0.0;
+  if (active) {
+    z = cy[NE * 24 + i];
+    for (int a = 0; a < 8; ++a) {
+      const int64_t nd = conn[e * 8 + a];
+      for (int k = 0; k < 3; ++k) z += cy[(a * 3 + k) * NE + i] * du[nd * 3 + k];
+    }
   }
-  alpha[tid] -= acc;
+  // every lane takes part in the shuffles; lanes of other elements and idle lanes read values they drop
+  for (int k = NE - 1; k > 0; --k) {
+    const double xk = __shfl(z, NE * sub + k, 64);
+    if (active && i < k) z -= cy[NE * 25 + k * (k - 1) / 2 + i] * xk;
+  }
+  if (active) alpha[e * NE + i] -= z;
 }
 
 template <bool WK>
@@ -647,7 +671,7 @@ void launchEas(ik_handle* h, int64_t e0, int64_t count, const int32_t* dest, con
 }  // namespace
 
 void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t* dest, const int32_t* rdest) {
-  h->easCache.ensure((size_t)h->ne * (NE * 24 + NE));
+  h->easCache.ensure((size_t)h->ne * kEasCache);
   if (wantK)
     launchEas<true>(h, e0, count, dest, rdest, h->ebeK64.p);
   else
@@ -656,8 +680,8 @@ void easStage(ik_handle* h, int64_t e0, int64_t count, bool wantK, const int32_t
 
 void easUpdate(ik_handle* h, const double* duDev) {
   require(h->easCacheValid, IK_EINVAL, "EAS update needs D, L, Rt of the current state (assemble first)");
-  const int64_t n = h->ne * NE;
-  hipLaunchKernelGGL(easUpdateKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->easCache.p,
+  const int64_t perBlock = 4 * kUpdElemsPerWave;
+  hipLaunchKernelGGL(easUpdateKernel, dim3((unsigned)((h->ne + perBlock - 1) / perBlock)), dim3(256), 0, h->stream, h->easCache.p,
                      h->conn.p, duDev, h->ne, h->alpha.p);
   h->easCacheValid = false;
 }
