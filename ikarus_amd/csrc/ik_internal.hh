@@ -222,7 +222,8 @@ struct ik_handle {
   double lambda = 0;
   ik::DevBuf<double> u, R, du, rhs, tmp;
 
-  // EAS: internal variables and, per element, Y = D^{-1} L (21x24) and y = D^{-1} Rt of the last staged state
+  // EAS: internal variables and, per element, the alpha-update cache of the last staged state: with
+  // D = L_D diag(d) L_D^T, diag(d)^{-1} L_D^{-1} L (21x24, column by column), diag(d)^{-1} L_D^{-1} Rt and L_D
   ik::DevBuf<double> alpha, easCache;
   bool easCacheValid = false;
 
