@@ -999,10 +999,9 @@ static void ensureK(ik::DevBuf<float>& K, int64_t n, hipStream_t s) {
   launchZeroF(K.p, n, s);
 }
 
-static void checkErrFlag(ik_handle* h) {
-  if (!h->herr) IK_HIP(hipHostMalloc((void**)&h->herr, 2 * sizeof(unsigned long long), hipHostMallocDefault));
-  IK_HIP(hipMemcpyAsync(h->herr, h->errFlag.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
-  IK_HIP(hipStreamSynchronize(h->stream));
+// the flag words copied to h->herr by a completed copy: clears the device flag and throws what they report
+static void raiseFlagged(ik_handle* h) {
+  h->errPending                = false;
   const unsigned long long det = h->herr[0], sing = h->herr[1];
   if (det == ~0ull && sing == ~0ull) return;
   const unsigned long long reset[2] = {~0ull, ~0ull};
@@ -1015,6 +1014,26 @@ static void checkErrFlag(ik_handle* h) {
   h->bad_elem = (int64_t)sing;
   throw Error{IK_ESINGULAR, "the enhanced-strain matrix D of element " + std::to_string(sing) +
                                 " is singular (a pivot of its elimination is zero or not finite)"};
+}
+
+// Reads the error flag after the kernels enqueued so far. mayDefer (assemblies): inside a Newton loop the copy is
+// only enqueued, and the loop's next synchronisation, which a host decision needs anyway, inspects it
+// (finishErrCheck): each such read otherwise leaves the device idle for a host round trip.
+static void checkErrFlag(ik_handle* h, bool mayDefer = false) {
+  if (!h->herr) IK_HIP(hipHostMalloc((void**)&h->herr, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  IK_HIP(hipMemcpyAsync(h->herr, h->errFlag.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+  if (mayDefer && h->deferErr) {
+    h->errPending = true;
+    return;
+  }
+  IK_HIP(hipStreamSynchronize(h->stream));
+  raiseFlagged(h);
+}
+
+void finishErrCheck(ik_handle* h, bool sync) {
+  if (!h->errPending) return;
+  if (sync) IK_HIP(hipStreamSynchronize(h->stream));
+  raiseFlagged(h);
 }
 
 void stressAt(ik_handle* h, int kind, const double xi[3], double* out) {
@@ -1182,7 +1201,7 @@ void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR) {
     hipLaunchKernelGGL(pointLoadKernel, dim3(gridFor(h->npl, 128)), dim3(128), 0, h->stream, h->plDof.p,
                        h->plCoef.p, h->npl, h->lambda, h->R.p);
   }
-  checkErrFlag(h);
+  checkErrFlag(h, true);
 }
 
 void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re) {

@@ -269,6 +269,13 @@ void linearSolve(ik_handle* h, const double* rhs, double* x, const ik_solver_opt
 void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_info* info) {
   requireBound(h);
   ensurePattern(h);
+  // the loop's assemblies leave their error-flag reads to the synchronisations the loop makes anyway (the PCG's
+  // done-flag reads, the residual norm), so an invalid element is still reported from within this call
+  struct DeferScope {
+    ik_handle* h;
+    ~DeferScope() { h->deferErr = false; }
+  } deferScope{h};
+  h->deferErr = true;
   const int64_t n = h->ndof;
   h->du.ensure(n);
   *info              = ik_nr_info{};
@@ -337,6 +344,7 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
   // (the reference's loop ends on it, since NaN > tol is false, and reports success)
   if (iter == s->max_iter || !std::isfinite(rNorm)) info->success = 0;
   info->iterations = iter;
+  ik::finishErrCheck(h, true);
 }
 
 // ---- arc-length path following -------------------------------------------------------------------

@@ -648,6 +648,7 @@ void coarsePrepare(ik_handle* h) {
   Coarse& cs = *h->coarse;
   if (!cs.ready) buildAggregates(h, cs);
   if (cs.valid) return;
+  finishErrCheck(h, true);  // a tangent with an invalid element reports that, not a failed coarse factorisation
   const int64_t nc = cs.nc;
   DevBuf<double> raw;
   raw.ensure((size_t)(nc * nc));

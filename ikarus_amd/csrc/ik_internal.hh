@@ -229,6 +229,9 @@ struct ik_handle {
 
   // error flag (lowest element index with det C <= 0)
   ik::DevBuf<unsigned long long> errFlag;
+  // deferErr: inside a Newton loop an assembly enqueues its error-flag copy without waiting for it, and the next
+  // synchronisation of `stream` inspects it (errPending until then)
+  bool deferErr = false, errPending = false;
 
   // solver workspace
   ik::DevBuf<double> pr, pz, pp, pq, px, pdiag, pb, partial;
@@ -264,6 +267,9 @@ void buildPattern(ik_handle* h);
 void launchZero(double* p, int64_t n, hipStream_t s);
 void launchZeroF(float* p, int64_t n, hipStream_t s);
 void assembleKR(ik_handle* h, int dbc, int precision, bool wantK, bool wantR);
+// throws the error an assembly's deferred flag copy reported; call after a synchronisation of h->stream that
+// followed the copy, or with sync = true
+void finishErrCheck(ik_handle* h, bool sync = false);
 void elementHost(ik_handle* h, int64_t e, double* Ke, double* Re);
 void easUpdate(ik_handle* h, const double* duDev);
 // element blocks go to staging slots dest[e*36+sb], corner residuals to rdest[e*8+a] (nullptr: element order)

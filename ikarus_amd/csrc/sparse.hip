@@ -559,6 +559,7 @@ double dot(ik_handle* h, const double* a, const double* b, int64_t n) {
   hipLaunchKernelGGL(foldKernel, dim3(1), dim3(1024), 0, h->stream, h->partial.p, g, h->partial.p + h->partial.n - 1);
   IK_HIP(hipMemcpyAsync(h->hred, h->partial.p + h->partial.n - 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   IK_HIP(hipStreamSynchronize(h->stream));
+  finishErrCheck(h);
   return h->hred[0];
 }
 
@@ -572,6 +573,7 @@ double foldPartials(ik_handle* h, int nb) {
   hipLaunchKernelGGL(foldKernel, dim3(1), dim3(1024), 0, h->stream, h->partial.p, nb, h->partial.p + h->partial.n - 1);
   IK_HIP(hipMemcpyAsync(h->hred, h->partial.p + h->partial.n - 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   IK_HIP(hipStreamSynchronize(h->stream));
+  finishErrCheck(h);
   return h->hred[0];
 }
 
@@ -586,8 +588,11 @@ void residualBlock64(ik_handle* h, const double* b, const double* x, double* r) 
                      h->scol.p, h->K64.p, x, b, r, h->nn, h->partial.p);
 }
 
-// Poll interval: iterations enqueued between host checks of the done flag.
-static constexpr int kChunk = 16;
+// Poll interval: iterations enqueued before the first host check of the done flag (32: a two-level inner solve
+// takes 24-32 at 64^3, so most need no second check) and between later checks (64). Iterations enqueued after
+// convergence return at once (a few us of launches each); a check leaves the device idle for a host round trip
+// (20-45 us) while the next iterations are enqueued.
+static int pollChunk(int enq) { return enq == 0 ? 32 : 64; }
 
 void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxIter, int* iters, double* relres) {
   ensureWork(h);
@@ -606,7 +611,7 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
   if (maxIter <= 0) maxIter = (int)(2 * n);
   int enq = 0;
   for (;;) {
-    for (int c = 0; c < kChunk && enq < maxIter; ++c, ++enq) {
+    for (int c = 0, cn = pollChunk(enq); c < cn && enq < maxIter; ++c, ++enq) {
       {
         ScopedKernelTimer t(h, KSPMV);
         hipLaunchKernelGGL((spmvKernel<double, double, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream, h->rowptr.p,
@@ -628,6 +633,7 @@ void pcgJacobi64(ik_handle* h, const double* b, double* x, double rtol, int maxI
     }
     IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
     IK_HIP(hipStreamSynchronize(h->stream));
+    finishErrCheck(h);
     if (h->hsc->done || enq >= maxIter) break;
   }
   *iters  = h->hsc->iters;
@@ -684,7 +690,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
                        h->sc.p);
     int enq = 0;
     for (;;) {
-      for (int c = 0; c < kChunk && enq < maxIter; ++c, ++enq) {
+      for (int c = 0, cn = pollChunk(enq); c < cn && enq < maxIter; ++c, ++enq) {
         {
           ScopedKernelTimer t(h, KSPMV);
           hipLaunchKernelGGL((spmvKernel<float, float, true>), dim3(sgrid), dim3(SPMV_BS), 0, h->stream,
@@ -710,6 +716,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
       }
       IK_HIP(hipMemcpyAsync(h->hsc, h->sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, h->stream));
       IK_HIP(hipStreamSynchronize(h->stream));
+      finishErrCheck(h);
       if (h->hsc->done || enq >= maxIter) break;
     }
     info->iterations += h->hsc->iters + 1;
@@ -723,6 +730,7 @@ void pcgMixed(ik_handle* h, const double* b, double* x, const ik_solver_opts* o,
     IK_HIP(hipMemcpyAsync(h->hred, h->partial.p + h->partial.n - 1, sizeof(double), hipMemcpyDeviceToHost,
                           h->stream));
     IK_HIP(hipStreamSynchronize(h->stream));
+    finishErrCheck(h);
     rnorm              = std::sqrt(h->hred[0]);
     info->rel_residual = rnorm / bnorm;
     if (info->rel_residual <= o->rtol) {
