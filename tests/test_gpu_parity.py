@@ -172,6 +172,44 @@ def test_invalid_state_reports_element():
     assert np.all(dev.vector() == 0.0)
 
 
+@pytest.mark.parametrize("kind", [capi.PCG_JACOBI_FP64, capi.PCG_TWOLEVEL_FP32_IR])
+def test_invalid_state_reported_from_newton(kind):
+    """Inside ik_newton_solve the assemblies leave their error-flag reads to the loop's next synchronisation:
+    a non-finite displacement still makes the call return IK_ENEGDET naming the lowest element touching the
+    node, and the same handle then solves a load step normally."""
+    p = pb.hex_block((3, 3, 3), jitter=0.0, seed=0, traction=(0.0, 0.0, 0.05))
+    print(f"case: invalid state in newton kind={kind}", flush=True)
+    prec = capi.FP64 if kind == capi.PCG_JACOBI_FP64 else capi.FP32
+    s = ia.nr_settings(tol=1e-8, max_iter=10, precision=prec, linear=ia.solver_opts(kind=kind, rtol=1e-10))
+    dev = ia.Device(p)
+    u = np.zeros(p.ndof)
+    node = 21  # vertex (1,1,1)
+    u[3 * node] = np.nan
+    dev.set_requirement(u, 0.0)
+    with pytest.raises(capi.IkError) as ei:
+        dev.newton(s)
+    assert ei.value.code == capi.IK_ENEGDET
+    assert ei.value.bad_element == int(np.nonzero(np.any(p.conn == node, axis=1))[0].min())
+    dev.set_requirement(np.zeros(p.ndof), 1.0)
+    info = dev.newton(s)
+    assert info.success == 1 and info.iterations >= 1 and info.residual_norm <= 1e-8
+    dev.close()
+
+
+def test_singular_eas_reported_from_newton():
+    """The EAS elimination's pivot guard, read at the Newton loop's next synchronisation: IK_ESINGULAR naming
+    element 0 when every D vanishes (lambda = mu = 0)."""
+    p = pb.hex_block((2, 2, 2), material=capi.SVK, jitter=0.0, seed=0, eas=21, traction=(0.0, 0.0, 1.0))
+    print("case: singular EAS D in newton", flush=True)
+    p.lam, p.mu = 0.0, 0.0
+    dev = ia.Device(p)
+    dev.set_requirement(np.zeros(p.ndof), 1.0)
+    with pytest.raises(capi.IkError) as ei:
+        dev.newton(ia.nr_settings(tol=1e-8, max_iter=3, linear=ia.solver_opts(rtol=1e-10, max_iter=20)))
+    assert ei.value.code == capi.IK_ESINGULAR and ei.value.bad_element == 0
+    dev.close()
+
+
 @pytest.mark.parametrize("eas", [0, 21])
 def test_chunked_assembly_is_bitwise_identical(monkeypatch, eas):
     """Element chunks whose gathers overlap the next chunk on the second stream (IKARUS_AMD_CHUNK_ELEMENTS)
