@@ -220,8 +220,9 @@ void assembleK(ik_handle* h, int dbc, int precision) {
 bool kReady(const ik_handle* h, int dbc, int precision) {
   return h->kValid && h->kDbc == dbc && h->kPrecision == precision;
 }
-// The residual at the state a Newton correction just reached. The EAS element's residual-only pass still forms
-// D, L and Rt and eliminates (about half the time of its R+K pass), so for EAS handles the pass also assembles
+// The residual at a new state (the start of a Newton solve, or the state a correction just reached). The EAS
+// element's residual-only pass still forms
+// D, L and Rt and eliminates (three quarters of the time of its R+K pass), so for EAS handles the pass also assembles
 // the condensed K (FULL, the given precision): the next iteration solves with it instead of assembling again, and
 // R is the same, bit for bit, as the residual-only pass gives. The plain element's residual pass is a tenth of
 // its tangent, so there R is assembled alone.
@@ -290,7 +291,12 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
     hipLaunchKernelGGL(zeroListKernel, dim3(grid(h->nfixed)), dim3(256), 0, h->stream, h->fixedList.p, h->nfixed,
                        h->idbcF.p);
   }
-  assembleR(h, IK_DBC_FULL);
+  // the first residual; an EAS handle takes it from the R+K pass (its residual-only pass forms D, L and Rt and
+  // eliminates as well, and R is the same bit for bit), and the first iteration solves with that K
+  if (h->neas)
+    assembleRAfterUpdate(h, s->precision, true);
+  else
+    assembleR(h, IK_DBC_FULL);
   if (idbcF) ik::axpy(h, stepSize, h->idbcF.p, h->R.p, n);
   double rNorm        = normR(h);
   if (!std::isfinite(rNorm)) {
@@ -302,7 +308,7 @@ void newtonSolve(ik_handle* h, const ik_nr_settings* s, double stepSize, ik_nr_i
   double etaPrev      = 0.0;
   info->residual_norm = rNorm;
   int iter            = 0;
-  h->kValid           = false;
+  if (!h->neas) h->kValid = false;
   while ((rNorm > s->tol && iter < s->max_iter) || iter < s->min_iter) {
     if (!kReady(h, IK_DBC_FULL, s->precision)) assembleK(h, IK_DBC_FULL, s->precision);
     // K x = R ; correction = -x (solve(correction, -rx), newtonraphson.hh:222-223)
