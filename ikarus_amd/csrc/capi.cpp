@@ -404,7 +404,12 @@ static void arcNewton(ik_handle* h, const ik_nr_settings* s, ArcState& a, ik_nr_
     IK_HIP(hipMemcpyAsync(h->u.p, h->tmp.p, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     h->lambda = lam0;
   }
-  assembleR(h, IK_DBC_FULL);
+  // an EAS handle takes the residual from the R+K pass, and the first iteration solves with its K (with
+  // inhomogeneous Dirichlet values every iteration assembles K_raw first and then K)
+  if (h->neas && !idbc)
+    assembleRAfterUpdate(h, s->precision, true);
+  else
+    assembleR(h, IK_DBC_FULL);
   double f, invRoot, dfdDl;
   arcSubsidiary(h, a, f, invRoot, dfdDl);
   double rNorm        = normR(h);  // the first norm leaves f out (:244)
