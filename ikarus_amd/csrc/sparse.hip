@@ -16,11 +16,11 @@ namespace {
 
 constexpr int SPMV_BS = 256;  // threads per block
 
-// Block rows per wavefront of the SpMV lane layouts: FP64 values 9 lanes per row (7 rows, lane 63 idles),
+// Block rows per wavefront of the SpMV lane layouts: FP64 values 3 lanes per row (21 rows, lane 63 idles),
 // FP32 values one lane per row (64 rows).
 template <typename TV>
 constexpr int spmvRowsPerWave() {
-  return sizeof(TV) == 4 ? 64 : 7;
+  return sizeof(TV) == 4 ? 64 : 21;
 }
 template <typename TV>
 constexpr int spmvRowsPerBlock() {
@@ -56,40 +56,18 @@ __device__ inline double blockSum(double v) {
 // y = K x over the sliced-ELL block storage (slice of kSellRows = 64 block rows, block j of row r of slice s
 // at slot sellStart[s] + 64 j + r, 9 values per slot); optional fused partial dot d = sum_i x_i y_i per
 // block. Two lane layouts:
-//  FP64 values: lane (r, m), row r = lane / 9 (7 rows per wavefront), entry m = 3k + l of every block; for a
-//   fixed j the wavefront's 7 rows read 7 adjacent 72-byte blocks (one 504-byte run per load instruction);
-//   the three partial sums of row k are combined with two lane shifts. Each row runs over its own blocks in
-//   column order, so the result equals a CSR SpMV's bit for bit.
+//  FP64 values: lane (r, k), row r = lane / 3 (21 rows per wavefront), output component k: for a fixed j the
+//   wavefront's 21 rows read 21 adjacent 72-byte blocks (1,512 contiguous bytes), each lane its block row.
+//   Each row runs over its own blocks in column order, so the result equals a CSR SpMV's bit for bit.
 //  FP32 values: one lane per block row, the 3 outputs of the row per lane, a wavefront = one slice: for a
 //   fixed j the 64 lanes read 64 adjacent 36-byte blocks (2304 contiguous bytes). Each lane runs to the
 //   slice's longest row; padding slots hold zeros with the row's own column.
 // 64^3 pattern, alone: FP32 48.6 us against 62.3 us for 3 lanes per row over CSR (21 rows per wavefront);
-// FP64 126.1 us against 127.9 us for the same 9-lane layout over CSR.
-// FP64 issues the loads of 9 blocks (a third of an interior 27-block row) before summing their products;
-// 9 blocks in flight beat 3, 4, 6, 12, 14 and 18.
-constexpr int SPMV_UNROLL = 9;
+// FP64 108.4 us against 119.3 us for the former layout with one lane per block entry (7 rows per wavefront).
+// FP64 issues the loads of SPMV_UNROLL blocks (3 x 24 bytes and 3 column indices per lane) before summing their
+// products: 3 beat 2, 4, 5, 6 and 9 (111.3, 111.9, 114.8, 116.5 and 121.1 us).
+constexpr int SPMV_UNROLL = 3;
 static_assert(kSellRows == 64, "one slice per wavefront");
-
-// acc += sum over the row's n blocks of V[j*64*9] * x[C[j*64]*3 + l] (in block order)
-template <typename TV, typename TX, typename TA>
-__device__ inline TA rowDotSell(const TV* __restrict__ V, const int32_t* __restrict__ C, int n,
-                                const TX* __restrict__ x, int l) {
-  TA acc = TA(0);
-  int j  = 0;
-  for (; j + SPMV_UNROLL <= n; j += SPMV_UNROLL) {
-    int32_t c[SPMV_UNROLL];
-    TV v[SPMV_UNROLL];
-#pragma unroll
-    for (int u = 0; u < SPMV_UNROLL; ++u) {
-      c[u] = C[(j + u) * kSellRows];
-      v[u] = V[(int64_t)(j + u) * (kSellRows * 9)];
-    }
-#pragma unroll
-    for (int u = 0; u < SPMV_UNROLL; ++u) acc += (TA)v[u] * (TA)x[(int64_t)c[u] * 3 + l];
-  }
-  for (; j < n; ++j) acc += (TA)V[(int64_t)j * (kSellRows * 9)] * (TA)x[(int64_t)C[j * kSellRows] * 3 + l];
-  return acc;
-}
 
 // a[k] = (K x)[3 row + k], k = 0..2, for one lane per row: the slice's w slots of row `row`
 template <typename TV, typename TX, typename TA>
@@ -116,28 +94,54 @@ __device__ inline void rowSell3(const int32_t* __restrict__ sellStart, const int
   a[2] = a2;
 }
 
-// (K x)[3 row + k] accumulated in TA with the FP64 lane layout; valid on lanes with `out` set
+// (K x)[3 row + k] accumulated in TA with the FP64 lane layout, lane = 3 rl + k (row rl of the wavefront's 21,
+// output component k); valid on lanes with `out` set. For a fixed j the wavefront's 21 rows read 21 adjacent
+// 72-byte blocks (1,512 contiguous bytes), each lane the 24 bytes of its block row. The lane keeps one sum per
+// column l, a_l = sum_j K_j[k][l] x_j[l] in block order, and returns a_0 + (a_1 + a_2): the order of the former
+// layout with one lane per block entry (7 rows per wavefront, combined by two lane shifts), so y is bitwise the
+// same as then and as a CSR SpMV's. The loads of SPMV_UNROLL blocks are issued before their products are summed.
 template <typename TV, typename TX, typename TA>
-__device__ inline TA spmvRow9(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ sellStart,
+__device__ inline TA spmvRow3(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ sellStart,
                               const int32_t* __restrict__ scol, const TV* __restrict__ val,
                               const TX* __restrict__ x, int64_t nrows, unsigned rb, int64_t& row, int& k,
                               bool& out) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int rl = lane / 9;
-  const int m  = lane - rl * 9;
-  k            = m / 3;
-  const int l  = m - 3 * k;
+  const int rl = lane / 3;
+  k            = lane - 3 * rl;
   row          = ((int64_t)rb * (SPMV_BS / 64) + wave) * spmvRowsPerWave<TV>() + rl;
-  const bool active = rl < spmvRowsPerWave<TV>() && row < nrows;
-  TA acc = TA(0);
-  if (active) {
-    const int64_t base = (int64_t)sellStart[row >> 6] + (row & 63);
-    acc = rowDotSell<TV, TX, TA>(val + base * 9 + m, scol + base, rowptr[row + 1] - rowptr[row], x, l);
+  out          = rl < spmvRowsPerWave<TV>() && row < nrows;
+  if (!out) return TA(0);
+  const int64_t base = (int64_t)sellStart[row >> 6] + (row & 63);
+  const TV* V        = val + base * 9 + 3 * k;
+  const int32_t* C   = scol + base;
+  const int n        = rowptr[row + 1] - rowptr[row];
+  TA a0 = TA(0), a1 = TA(0), a2 = TA(0);
+  int j = 0;
+  for (; j + SPMV_UNROLL <= n; j += SPMV_UNROLL) {
+    int32_t c[SPMV_UNROLL];
+    TV v[SPMV_UNROLL][3];
+#pragma unroll
+    for (int u = 0; u < SPMV_UNROLL; ++u) {
+      c[u] = C[(j + u) * kSellRows];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) v[u][l] = V[(int64_t)(j + u) * (kSellRows * 9) + l];
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_UNROLL; ++u) {
+      const TX* xc = x + (int64_t)c[u] * 3;
+      a0 += (TA)v[u][0] * (TA)xc[0];
+      a1 += (TA)v[u][1] * (TA)xc[1];
+      a2 += (TA)v[u][2] * (TA)xc[2];
+    }
   }
-  const TA a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);
-  acc += a1 + a2;
-  out = active && l == 0;
-  return acc;
+  for (; j < n; ++j) {
+    const TV* vj = V + (int64_t)j * (kSellRows * 9);
+    const TX* xc = x + (int64_t)C[j * kSellRows] * 3;
+    a0 += (TA)vj[0] * (TA)xc[0];
+    a1 += (TA)vj[1] * (TA)xc[1];
+    a2 += (TA)vj[2] * (TA)xc[2];
+  }
+  return a0 + (a1 + a2);
 }
 
 // rev: workgroup i takes row block gridDim.x - 1 - i (the partial stays at the row block's index, so the fold
@@ -171,7 +175,7 @@ __global__ __launch_bounds__(SPMV_BS) void spmvKernel(const int32_t* __restrict_
     int64_t row;
     int k;
     bool out;
-    const TX yk = spmvRow9<TV, TX, TX>(rowptr, sellStart, scol, val, x, nrows, rb, row, k, out);
+    const TX yk = spmvRow3<TV, TX, TX>(rowptr, sellStart, scol, val, x, nrows, rb, row, k, out);
     if (out) {
       y[row * 3 + k] = yk;
       if (DOT) d = (double)x[row * 3 + k] * (double)yk;
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(SPMV_BS) void residualKernel(const int32_t* __restr
     int64_t row;
     int k;
     bool out;
-    const double kx = spmvRow9<TV, double, double>(rowptr, sellStart, scol, val, x, nrows, blockIdx.x, row, k, out);
+    const double kx = spmvRow3<TV, double, double>(rowptr, sellStart, scol, val, x, nrows, blockIdx.x, row, k, out);
     if (out) {
       const double rk = b[row * 3 + k] - kx;
       r[row * 3 + k]  = rk;
